@@ -1,0 +1,789 @@
+"""NumPy restatement of the EasyFsi FV pressure projection (test oracle).
+
+TEST INFRASTRUCTURE ONLY -- never imported by the product package.
+
+Arrays are indexed [i, j, k] with k fastest, exactly like the reference's
+Taichi fields.  Velocity is (nx, ny, nz, 3); component d of cell c is the
+normal flux on the LOW d-face of c (shifted MAC storage, fluid.py:3450-3486).
+
+Precision: by default everything is float64.  The reference stores its 1D
+metric fields (cell widths, centre distances) as float32 and promotes them to
+float64 inside the operator (fluid.py:1013-1024, 4103-4106), so ``FvLevel``
+rounds the metrics through float32 unless ``metric_dtype=np.float64`` is
+requested.  ``coarse_dtype`` selects the storage of multigrid levels >= 1
+(float32 in the reference, fluid.py:1055-1075).  The right-hand side scale
+``rho/dt``, the smoother weight and ``dt/rho`` are Taichi f32 kernel arguments
+in the reference and are rounded through float32 here as well.
+
+Pinning (see tests/test_oracle_fluid.py): reference known-answer tests
+(divergence 5/-5/0 next to an obstacle, outlet face, interface diagonal 7.5,
+restriction 80/8=10, mixed-obstacle restriction), invariants (V-weighted
+self-adjointness, div(grad p) = -(dt/rho) L p, global conservation), closed
+forms (second-order Richardson convergence) and scipy.sparse spsolve for the
+solution the PCG converges to.  Exact PCG iteration counts are parity
+unpinned: no reference test fixes them, only upper bounds.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+F32 = np.float32
+
+# ---------------------------------------------------------------------------
+# Level geometry
+# ---------------------------------------------------------------------------
+
+
+def center_distances(widths: np.ndarray, origin: float = 0.0) -> np.ndarray:
+    """Entry 0 is w0, entry n is c_n - c_{n-1} = (w_{n-1} + w_n)/2 computed
+    from running centres starting at ``origin`` (fluid.py:66-87)."""
+    w = [float(v) for v in widths]
+    centers, face = [], float(origin)
+    for v in w:
+        centers.append(face + 0.5 * v)
+        face += v
+    out = [w[0]] + [centers[n] - centers[n - 1] for n in range(1, len(w))]
+    return np.asarray(out, dtype=np.float64)
+
+
+def _centers_with_origin(origin: float, widths) -> list[float]:
+    centers, face = [], float(origin)
+    for v in widths:
+        centers.append(face + 0.5 * float(v))
+        face += float(v)
+    return centers
+
+
+@dataclass
+class FvLevel:
+    """One grid level of the FV operator (metrics + masks).
+
+    ``w`` / ``d``: per-axis widths and centre distances (float64 values of the
+    stored metric dtype).  ``obstacle``: int array.  ``fw``: per-cell weight
+    of the cell's LOW faces (1 unless the cell has an active velocity-Dirichlet
+    row, then clamp(weight, 0, 1); the HIGH face of cell c uses ``fw`` of
+    c + e_d), fluid.py:4099-4110.  ``iface_diag``: extra diagonal.
+    """
+
+    w: tuple[np.ndarray, np.ndarray, np.ndarray]
+    d: tuple[np.ndarray, np.ndarray, np.ndarray]
+    obstacle: np.ndarray
+    fw: np.ndarray
+    iface_diag: np.ndarray
+    vd_active: np.ndarray
+    vd_weight: np.ndarray
+    storage: type = np.float64
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        return self.obstacle.shape
+
+    @property
+    def free(self) -> np.ndarray:
+        return self.obstacle == 0
+
+    def volume64(self) -> np.ndarray:
+        """V = f64(wx) f64(wy) f64(wz), as in the weighted dots (fluid.py:4537-4541)."""
+        return self.w[0][:, None, None] * self.w[1][None, :, None] * self.w[2][None, None, :]
+
+    def volume_metric_product(self) -> np.ndarray:
+        """V as the product of the stored metric values in their own dtype
+        (f32*f32*f32 in the reference's restriction, fluid.py:5356-5360)."""
+        dt = F32 if self.metric_dtype == F32 else np.float64
+        wx, wy, wz = (a.astype(dt) for a in self.w)
+        return (wx[:, None, None] * wy[None, :, None] * wz[None, None, :]).astype(np.float64)
+
+    metric_dtype: type = F32
+
+
+def make_level(widths, obstacle, *, vd_active=None, vd_weight=None, iface_diag=None,
+               metric_dtype=F32, origin=(0.0, 0.0, 0.0), storage=np.float64) -> FvLevel:
+    """Level-0 description from grid widths (fluid.py:1161-1199 metric upload)."""
+    shape = tuple(len(w) for w in widths)
+    w, d = [], []
+    for a in range(3):
+        wa = np.asarray(widths[a], dtype=np.float64)
+        ca = _centers_with_origin(origin[a], wa)
+        da = [float(wa[0])] + [ca[n] - ca[n - 1] for n in range(1, len(ca))]
+        w.append(np.asarray(wa, dtype=metric_dtype).astype(np.float64))
+        d.append(np.asarray(da, dtype=metric_dtype).astype(np.float64))
+    obstacle = np.asarray(obstacle, dtype=np.int32).reshape(shape)
+    vd_active = np.zeros(shape, np.int32) if vd_active is None else np.asarray(vd_active, np.int32)
+    vd_weight = np.zeros(shape, np.float64) if vd_weight is None else np.asarray(vd_weight, np.float32).astype(np.float64)
+    iface = np.zeros(shape, np.float64) if iface_diag is None else np.asarray(iface_diag, np.float64)
+    fw = np.where(vd_active != 0, np.clip(vd_weight, 0.0, 1.0), 1.0)
+    lvl = FvLevel(w=tuple(w), d=tuple(d), obstacle=obstacle, fw=fw, iface_diag=iface,
+                  vd_active=vd_active, vd_weight=vd_weight, storage=storage)
+    lvl.metric_dtype = metric_dtype
+    return lvl
+
+
+# ---------------------------------------------------------------------------
+# Operator pieces
+# ---------------------------------------------------------------------------
+
+def _axis_view(a: np.ndarray, axis: int, sl: slice):
+    idx = [slice(None)] * a.ndim
+    idx[axis] = sl
+    return tuple(idx)
+
+
+def neighbor_sum_denominator(p: np.ndarray, lvl: FvLevel, outlet: bool):
+    """(sum c*w*p_nb, sum c*w) over open faces (fluid.py:4079-4221).
+
+    Face to the lower neighbour of cell i on axis a: c = 1/(w_a[i] d_a[i]);
+    to the upper neighbour: c = 1/(w_a[i] d_a[i+1]).  A face is open when the
+    neighbour exists and is not an obstacle.  The z-min outlet adds the
+    Dirichlet ghost 2/w_z[0]^2 to the denominator only.  Accumulation order
+    x-, x+, y-, y+, z-, z+ matches the reference.
+    """
+    p = np.asarray(p, dtype=np.float64)
+    nb = np.zeros(lvl.shape, np.float64)
+    den = np.zeros(lvl.shape, np.float64)
+    obst = lvl.obstacle
+    fw = lvl.fw
+    for a in range(3):
+        n = lvl.shape[a]
+        if n < 2:
+            continue
+        wa, da = lvl.w[a], lvl.d[a]
+        bshape = [1, 1, 1]
+        bshape[a] = n - 1
+        # low face of cells 1..n-1
+        c_lo = (1.0 / (wa[1:] * da[1:])).reshape(bshape)
+        hi_cells = _axis_view(nb, a, slice(1, None))
+        lo_cells = _axis_view(nb, a, slice(None, -1))
+        open_lo = obst[lo_cells] == 0
+        coef = c_lo * fw[hi_cells]
+        nb[hi_cells] += np.where(open_lo, coef * p[lo_cells], 0.0)
+        den[hi_cells] += np.where(open_lo, coef, 0.0)
+        # high face of cells 0..n-2 (owner of that face is cell +1)
+        c_hi = (1.0 / (wa[:-1] * da[1:])).reshape(bshape)
+        open_hi = obst[hi_cells] == 0
+        coef = c_hi * fw[hi_cells]
+        nb[lo_cells] += np.where(open_hi, coef * p[hi_cells], 0.0)
+        den[lo_cells] += np.where(open_hi, coef, 0.0)
+        # the reference adds x-lo, x-hi, ... per cell; with disjoint
+        # additions per face family the float result is the same.
+    if outlet:
+        den[:, :, 0] += 2.0 / (lvl.w[2][0] * lvl.w[2][0])
+    return nb, den
+
+
+def fv_apply(p: np.ndarray, lvl: FvLevel, outlet: bool) -> np.ndarray:
+    """q = (den + iface_diag) p - nb; obstacle rows 0 (fluid.py:4311-4340).
+
+    The interface row-list and coupling-slot terms (fluid.py:4341-4456) are
+    zero on the plain path and are not part of this oracle.
+    """
+    nb, den = neighbor_sum_denominator(p, lvl, outlet)
+    q = (den + lvl.iface_diag) * np.asarray(p, np.float64) - nb
+    return np.where(lvl.free, q, 0.0)
+
+
+def fv_diagonal(lvl: FvLevel, outlet: bool) -> np.ndarray:
+    """Jacobi diagonal den + iface_diag (fluid.py:4280-4309)."""
+    _, den = neighbor_sum_denominator(np.zeros(lvl.shape), lvl, outlet)
+    return np.where(lvl.free, den + lvl.iface_diag, 0.0)
+
+
+def jacobi_precondition(r: np.ndarray, diag: np.ndarray, lvl: FvLevel) -> np.ndarray:
+    """z = r/diag where diag > 0 (fluid.py:4502-4516)."""
+    safe = np.where(diag > 0.0, diag, 1.0)
+    return np.where(lvl.free & (diag > 0.0), r / safe, 0.0)
+
+
+def fv_jacobi_sweep(p, rhs, lvl: FvLevel, outlet: bool, omega: float) -> np.ndarray:
+    """One weighted-Jacobi sweep on L p = rhs with L = nb - den p (fluid.py:4223-4273).
+
+    Returns tmp (stored at ``lvl.storage``); the caller copies it back.
+    """
+    nb, den = neighbor_sum_denominator(p, lvl, outlet)
+    den = den + lvl.iface_diag
+    p64 = np.asarray(p, np.float64)
+    om = float(F32(omega))
+    safe = np.where(den > 0.0, den, 1.0)
+    jac = (nb - np.asarray(rhs, np.float64)) / safe
+    tmp = p64 + om * (jac - p64)
+    tmp = np.where(lvl.free & (den > 0.0), tmp, 0.0)
+    return tmp.astype(lvl.storage)
+
+
+def legacy_residual(p, rhs, lvl: FvLevel, outlet: bool) -> np.ndarray:
+    """res = rhs - (nb - den p) (fluid.py:5261-5307)."""
+    nb, den = neighbor_sum_denominator(p, lvl, outlet)
+    den = den + lvl.iface_diag
+    res = np.asarray(rhs, np.float64) - (nb - den * np.asarray(p, np.float64))
+    return np.where(lvl.free, res, 0.0).astype(lvl.storage)
+
+
+def _pad_even(a: np.ndarray, fill=0):
+    pads = [(0, n % 2) for n in a.shape]
+    return np.pad(a, pads, constant_values=fill)
+
+
+def restrict_masks(fine: FvLevel, coarse_widths, coarse_storage, origin=(0.0, 0.0, 0.0)) -> FvLevel:
+    """Coarse obstacle (any child obstacle), vd flag (any child) with the max
+    child weight, V-averaged interface diagonal (fluid.py:5309-5388)."""
+    cshape = tuple(len(w) for w in coarse_widths)
+    obst = _pad_even(fine.obstacle)
+    vda = _pad_even(fine.vd_active)
+    vdw = _pad_even(fine.vd_weight)
+    vol = _pad_even(fine.volume_metric_product())
+    diag = _pad_even(fine.iface_diag)
+    obst_count = np.zeros(cshape, np.int64)
+    vd_count = np.zeros(cshape, np.int64)
+    vd_wmax = np.zeros(cshape, np.float64)
+    dsum = np.zeros(cshape, np.float64)
+    vsum = np.zeros(cshape, np.float64)
+    for oi in range(2):
+        for oj in range(2):
+            for ok in range(2):
+                sl = (slice(oi, None, 2), slice(oj, None, 2), slice(ok, None, 2))
+                o = obst[sl]
+                obst_count += (o == 1)
+                act = vda[sl] != 0
+                vd_count += act
+                vd_wmax = np.where(act, np.maximum(vd_wmax, vdw[sl]), vd_wmax)
+                fr = o != 1
+                dsum += np.where(fr, diag[sl] * vol[sl], 0.0)
+                vsum += np.where(fr, vol[sl], 0.0)
+    c_obst = (obst_count > 0).astype(np.int32)
+    c_diag = np.where(c_obst == 1, 0.0, dsum / np.maximum(vsum, 1.0e-30)).astype(coarse_storage).astype(np.float64)
+    c_vda = (vd_count > 0).astype(np.int32)
+    c_vdw = np.where(c_vda == 1, vd_wmax, 0.0).astype(F32).astype(np.float64)
+    lvl = make_level(coarse_widths, c_obst, vd_active=c_vda, vd_weight=c_vdw, iface_diag=c_diag,
+                     metric_dtype=fine.metric_dtype, storage=coarse_storage, origin=origin)
+    return lvl
+
+
+def restrict_residual(fine_res, fine: FvLevel, coarse: FvLevel) -> np.ndarray:
+    """Coarse rhs = V-weighted mean of the free children's residual (fluid.py:5334-5379)."""
+    obst = _pad_even(fine.obstacle)
+    res = _pad_even(np.asarray(fine_res, np.float64))
+    vol = _pad_even(fine.volume_metric_product())
+    rsum = np.zeros(coarse.shape, np.float64)
+    vsum = np.zeros(coarse.shape, np.float64)
+    for oi in range(2):
+        for oj in range(2):
+            for ok in range(2):
+                sl = (slice(oi, None, 2), slice(oj, None, 2), slice(ok, None, 2))
+                fr = obst[sl] != 1
+                rsum += np.where(fr, res[sl] * vol[sl], 0.0)
+                vsum += np.where(fr, vol[sl], 0.0)
+    out = np.where(coarse.obstacle == 1, 0.0, rsum / np.maximum(vsum, 1.0e-30))
+    return out.astype(coarse.storage)
+
+
+def prolongate_add(fine_p, fine: FvLevel, coarse_p, scale: float) -> np.ndarray:
+    """e_f += s * e_c[min(i//2, nc-1), ...] on free fine cells (fluid.py:5390-5406)."""
+    nf = fine.shape
+    nc = np.asarray(coarse_p).shape
+    ci = np.minimum(np.arange(nf[0]) // 2, nc[0] - 1)
+    cj = np.minimum(np.arange(nf[1]) // 2, nc[1] - 1)
+    ck = np.minimum(np.arange(nf[2]) // 2, nc[2] - 1)
+    up = np.asarray(coarse_p, np.float64)[np.ix_(ci, cj, ck)]
+    s = float(F32(scale))
+    out = np.asarray(fine_p, np.float64) + np.where(fine.free, s * up, 0.0)
+    return out.astype(fine.storage)
+
+
+# ---------------------------------------------------------------------------
+# Multigrid hierarchy and V-cycle
+# ---------------------------------------------------------------------------
+
+def multigrid_shapes(shape):
+    """fluid.py:692-701."""
+    shapes = [tuple(shape)]
+    while min(shapes[-1]) > 3:
+        nxt = tuple(max(2, (n + 1) // 2) for n in shapes[-1])
+        if nxt == shapes[-1]:
+            break
+        shapes.append(nxt)
+    return shapes
+
+
+def coarsen_widths(w):
+    """Pairwise sums (fluid.py:704-708)."""
+    w = [float(v) for v in w]
+    return [sum(w[n:n + 2]) for n in range(0, len(w), 2)]
+
+
+@dataclass
+class Hierarchy:
+    levels: list = field(default_factory=list)
+    uniform: bool = True
+
+
+def build_hierarchy(widths, obstacle, *, vd_active=None, vd_weight=None, iface_diag=None,
+                    metric_dtype=F32, coarse_dtype=F32, level0_storage=np.float64,
+                    origin=(0.0, 0.0, 0.0)) -> Hierarchy:
+    """Level metrics for every MG level; coarse masks by restriction (static per solve)."""
+    shapes = multigrid_shapes(tuple(len(w) for w in widths))
+    lvl0 = make_level(widths, obstacle, vd_active=vd_active, vd_weight=vd_weight, iface_diag=iface_diag,
+                      metric_dtype=metric_dtype, storage=level0_storage, origin=origin)
+    levels = [lvl0]
+    cur_w = [list(map(float, w)) for w in widths]
+    for _ in shapes[1:]:
+        cur_w = [coarsen_widths(w) for w in cur_w]
+        levels.append(restrict_masks(levels[-1], cur_w, coarse_dtype, origin))
+    uniform = all(max(abs(v - w[0]) for v in w) <= max(abs(w[0]), 1.0) * 1.0e-12 for w in
+                  [list(map(float, ww)) for ww in widths])
+    return Hierarchy(levels=levels, uniform=uniform)
+
+
+def smooth(p, rhs, lvl, outlet, sweeps, omega=0.8):
+    """``sweeps`` weighted-Jacobi sweeps, each followed by the copy back (fluid.py:5862-5904)."""
+    for _ in range(max(0, int(sweeps))):
+        p = fv_jacobi_sweep(p, rhs, lvl, outlet, omega)
+    return p
+
+
+def vcycle(r: np.ndarray, hier: Hierarchy, outlet: bool, pre=2, coarse=24, post=2) -> np.ndarray:
+    """One V-cycle on L e = -r from e = 0; returns z = e_0 (fluid.py:6085-6262, 4484-4500)."""
+    pre, coarse, post = max(1, int(pre)), max(1, int(coarse)), max(1, int(post))
+    levels = hier.levels
+    l0 = levels[0]
+    rhs = [np.where(l0.free, -np.asarray(r, np.float64), 0.0).astype(l0.storage)]
+    p = [np.zeros(l0.shape, l0.storage)]
+    if len(levels) <= 1:
+        return smooth(p[0], rhs[0], l0, outlet, max(pre + post, coarse)).astype(np.float64)
+    scale = 1.0 if hier.uniform else 0.7
+    for n in range(len(levels) - 1):
+        p[n] = smooth(p[n], rhs[n], levels[n], outlet, pre)
+        res = legacy_residual(p[n], rhs[n], levels[n], outlet)
+        rhs.append(restrict_residual(res, levels[n], levels[n + 1]))
+        p.append(np.zeros(levels[n + 1].shape, levels[n + 1].storage))
+    last = len(levels) - 1
+    p[last] = smooth(p[last], rhs[last], levels[last], outlet, coarse)
+    for n in range(len(levels) - 2, -1, -1):
+        p[n] = prolongate_add(p[n], levels[n], p[n + 1], scale)
+        p[n] = smooth(p[n], rhs[n], levels[n], outlet, post)
+    return np.asarray(p[0], np.float64)
+
+
+# ---------------------------------------------------------------------------
+# Reductions
+# ---------------------------------------------------------------------------
+
+def weighted_dot(a, b, lvl: FvLevel) -> float:
+    """sum over free cells of V a b in float64 (fluid.py:4533-4564)."""
+    v = lvl.volume64()
+    return float(np.sum(np.where(lvl.free, np.asarray(a, np.float64) * np.asarray(b, np.float64) * v, 0.0)))
+
+
+def weighted_mean(a, lvl: FvLevel) -> float:
+    """V-weighted mean over free cells (fluid.py:4676-4697)."""
+    v = np.where(lvl.free, lvl.volume64(), 0.0)
+    vol = float(np.sum(v))
+    if vol <= 0.0:
+        return 0.0
+    return float(np.sum(np.asarray(a, np.float64) * v)) / vol
+
+
+def subtract_weighted_mean(a, lvl: FvLevel) -> np.ndarray:
+    """a -= mean on free cells, 0 on obstacles (fluid.py:4715-4726)."""
+    m = weighted_mean(a, lvl)
+    return np.where(lvl.free, np.asarray(a, np.float64) - m, 0.0)
+
+
+# ---------------------------------------------------------------------------
+# Solvers
+# ---------------------------------------------------------------------------
+
+def level0_rhs(divergence, source, iface_rhs, lvl: FvLevel, rhs_scale: float) -> np.ndarray:
+    """rhs0 = f32(rho/dt) * (div - source) in f32, minus the interface rhs (fluid.py:5228-5243)."""
+    s = F32(rhs_scale)
+    core = (s * (np.asarray(divergence, F32) - np.asarray(source, F32))).astype(np.float64)
+    if iface_rhs is not None:
+        core = core - np.asarray(iface_rhs, np.float64)
+    return np.where(lvl.free, core, 0.0)
+
+
+@dataclass
+class PcgResult:
+    pressure: np.ndarray
+    iterations: int = 0
+    initial_relative_residual: float = math.inf
+    relative_residual: float = math.inf
+    converged: bool = False
+    breakdown: str = ""
+    breakdown_code: int = 0
+    breakdown_dAd: float = 0.0
+    host_residual_checks: int = 0
+    mean_projection_count: int = 0
+    restart_count: int = 0
+    used_multigrid: bool = False
+
+
+def pcg(rhs0: np.ndarray, pressure0: np.ndarray, hier: Hierarchy, *, outlet: bool, tolerance: float,
+        max_iters: int, preconditioner: str = "auto", remove_mean: bool = True,
+        interface_active: bool = False) -> PcgResult:
+    """Flexible-beta PCG of ``_solve_pressure_poisson_fv_cg`` (fluid.py:6535-6811).
+
+    ``rhs0`` is the legacy-sign level-0 right-hand side; the CG system is
+    A p = b with b = -rhs0 (minus its V-mean when ``remove_mean``).
+    """
+    if preconditioner not in ("auto", "jacobi", "fv_multigrid", "fv_multigrid_light"):
+        raise ValueError(f"unsupported cg_preconditioner: {preconditioner!r}")
+    lvl = hier.levels[0]
+    free = lvl.free
+    max_iters = max(1, int(max_iters))
+    tol = max(0.0, float(tolerance))
+    out = PcgResult(pressure=np.asarray(pressure0, np.float64).copy())
+    diag = fv_diagonal(lvl, outlet)
+    if remove_mean:
+        mean = -weighted_mean(rhs0, lvl)  # cg_weighted_mean with multiplier -1
+        out.mean_projection_count += 1
+        b = np.where(free, -rhs0 - mean, 0.0)
+    else:
+        b = np.where(free, -rhs0, 0.0)
+    b_norm = math.sqrt(max(weighted_dot(b, b, lvl), 0.0))
+    if b_norm <= 1.0e-30:
+        out.pressure = np.zeros_like(out.pressure)
+        out.initial_relative_residual = 0.0
+        out.relative_residual = 0.0
+        out.converged = True
+        return out
+
+    def project(v):
+        if remove_mean:
+            out.mean_projection_count += 1
+            return subtract_weighted_mean(v, lvl)
+        return v
+
+    x = out.pressure
+    r = np.where(free, b - fv_apply(x, lvl, outlet), 0.0)
+    r = project(r)
+    out.host_residual_checks += 1
+    rel = math.sqrt(max(weighted_dot(r, r, lvl), 0.0)) / b_norm
+    out.initial_relative_residual = out.relative_residual = rel
+    if rel * b_norm <= tol * b_norm:
+        out.converged = True
+        return out
+
+    use_mg = (len(hier.levels) > 1 and not interface_active and (
+        preconditioner in ("fv_multigrid", "fv_multigrid_light")
+        or (preconditioner == "auto" and not hier.uniform)))
+    light = preconditioner == "fv_multigrid_light"
+    sweeps = (1, 12, 1) if light else (2, 24, 2)
+
+    def precondition(res):
+        if use_mg:
+            return vcycle(res, hier, outlet, *sweeps)
+        return jacobi_precondition(res, diag, lvl)
+
+    z = project(precondition(r))
+    rz = weighted_dot(r, z, lvl)
+    if not math.isfinite(rz) or abs(rz) <= 1.0e-300:
+        use_mg = False
+        z = jacobi_precondition(r, diag, lvl)
+        rz = weighted_dot(r, z, lvl)
+    if not math.isfinite(rz) or rz <= 0.0:
+        out.breakdown = "initial residual/preconditioner dot is not positive finite"
+        out.pressure = x
+        out.used_multigrid = use_mg
+        return out
+    d = z.copy()
+    code, code_dAd = 0, 0.0
+    restart_interval = 256 if interface_active else 512
+    it = 0
+    while it < max_iters:
+        it += 1
+        q = fv_apply(d, lvl, outlet)
+        dAd = weighted_dot(d, q, lvl)
+        alpha = 0.0
+        if dAd <= 0.0:
+            if code == 0:
+                code_dAd = dAd
+            code = 1
+        else:
+            alpha = rz / dAd
+        r_old = np.where(free, r, 0.0)
+        x = np.where(free, x + alpha * d, 0.0)
+        r = np.where(free, r - alpha * q, 0.0)
+        if remove_mean and it % 16 == 0:
+            x = subtract_weighted_mean(x, lvl)
+            r = subtract_weighted_mean(r, lvl)
+            out.mean_projection_count += 2
+        out.iterations = it
+        if it % 16 == 0 or it == max_iters:
+            out.host_residual_checks += 1
+            rel = math.sqrt(max(weighted_dot(r, r, lvl), 0.0)) / b_norm
+            out.relative_residual = rel
+            if rel <= tol:
+                out.converged = True
+                break
+            restart = False
+            if code != 0:
+                out.breakdown_dAd = code_dAd
+                if it >= max_iters:
+                    out.breakdown = "device-side CG scalar update failed"
+                    break
+                restart = True
+            elif it < max_iters and it % restart_interval == 0:
+                restart = True
+            if restart:
+                r = project(np.where(free, b - fv_apply(x, lvl, outlet), 0.0))
+                out.host_residual_checks += 1
+                rel = math.sqrt(max(weighted_dot(r, r, lvl), 0.0)) / b_norm
+                out.relative_residual = rel
+                if rel <= tol:
+                    out.converged = True
+                    break
+                z = project(precondition(r))
+                rz = weighted_dot(r, z, lvl)
+                if math.isfinite(rz) and rz > 0.0:
+                    d = z.copy()
+                    code, code_dAd = 0, 0.0
+                    out.restart_count += 1
+                    continue
+                out.breakdown = ("device-side CG scalar update failed" if code != 0 else
+                                 "restarted residual/preconditioner dot is not positive finite")
+                break
+        z = project(precondition(r))
+        rz_new = weighted_dot(r, z, lvl)
+        beta_num = weighted_dot(z, np.where(free, r - r_old, 0.0), lvl) if use_mg else 0.0
+        beta = 0.0
+        if rz <= 1.0e-300 or rz_new <= 0.0:
+            code = 2
+        else:
+            beta = max(beta_num / rz, 0.0) if use_mg else rz_new / rz
+        d = np.where(free, z + beta * d, 0.0)
+        rz = rz_new
+    out.pressure = x
+    out.breakdown_code = code
+    out.used_multigrid = use_mg
+    return out
+
+
+def fv_jacobi_solve(rhs0, pressure0, hier: Hierarchy, *, outlet: bool, iterations: int) -> np.ndarray:
+    """``iterations`` damped Jacobi sweeps on level 0 (fluid.py:5977-5990)."""
+    lvl = hier.levels[0]
+    return smooth(np.asarray(pressure0, np.float64), rhs0, lvl, outlet, iterations).astype(np.float64)
+
+
+def fv_multigrid_solve(rhs0, pressure0, hier: Hierarchy, *, outlet: bool, iterations: int,
+                       cycles: int | None) -> np.ndarray:
+    """Standalone MG: 3/36/3 Jacobi V-cycles then a final level-0 smooth on
+    graded grids (fluid.py:5992-6083)."""
+    levels = hier.levels
+    l0 = levels[0]
+    p0 = np.asarray(pressure0, np.float64).copy()
+    if len(levels) <= 1:
+        return smooth(p0, rhs0, l0, outlet, max(1, int(iterations)))
+    ncyc = int(cycles) if cycles is not None else (12 if hier.uniform else 24)
+    if ncyc <= 0:
+        raise ValueError("multigrid_cycles must be positive")
+    scale = 1.0 if hier.uniform else 0.7
+    for _ in range(ncyc):
+        p = [p0]
+        rhs = [rhs0]
+        for n in range(len(levels) - 1):
+            p[n] = smooth(p[n], rhs[n], levels[n], outlet, 3)
+            res = legacy_residual(p[n], rhs[n], levels[n], outlet)
+            rhs.append(restrict_residual(res, levels[n], levels[n + 1]))
+            p.append(np.zeros(levels[n + 1].shape, levels[n + 1].storage))
+        last = len(levels) - 1
+        p[last] = smooth(p[last], rhs[last], levels[last], outlet, 36)
+        for n in range(len(levels) - 2, -1, -1):
+            p[n] = prolongate_add(p[n], levels[n], p[n + 1], scale)
+            p[n] = smooth(p[n], rhs[n], levels[n], outlet, 3)
+        p0 = np.asarray(p[0], np.float64)
+    if not hier.uniform:
+        p0 = smooth(p0, rhs0, l0, outlet, max(1, int(iterations)))
+    return p0
+
+
+def legacy_jacobi_solve(pressure0, divergence, source, obstacle, *, rhs_scale, spacing, outlet,
+                        iterations) -> np.ndarray:
+    """Uniform-grid compact Jacobi, the ``project()`` default (fluid.py:3996-4053, 5842-5860)."""
+    p = np.asarray(pressure0, np.float64).copy()
+    obst = np.asarray(obstacle, np.int32)
+    nx, ny, nz = p.shape
+    ix2, iy2, iz2 = (float(F32(1.0 / (h * h))) for h in spacing)
+    rhs = (F32(rhs_scale) * (np.asarray(divergence, F32) - np.asarray(source, F32))).astype(np.float64)
+    im = np.maximum(np.arange(nx) - 1, 0)
+    ip = np.minimum(np.arange(nx) + 1, nx - 1)
+    jm = np.maximum(np.arange(ny) - 1, 0)
+    jp = np.minimum(np.arange(ny) + 1, ny - 1)
+    km = np.maximum(np.arange(nz) - 1, 0)
+    kp = np.minimum(np.arange(nz) + 1, nz - 1)
+    adj = ((obst[im] == 1) | (obst[ip] == 1) | (obst[:, jm] == 1) | (obst[:, jp] == 1)
+           | (obst[:, :, km] == 1) | (obst[:, :, kp] == 1))
+    special = adj.copy()
+    if outlet:
+        special[:, :, 0] = True
+    for _ in range(int(iterations)):
+        nb = (ix2 * (p[im] + p[ip]) + iy2 * (p[:, jm] + p[:, jp]) + iz2 * (p[:, :, km] + p[:, :, kp]))
+        den = np.full(p.shape, 2.0 * (ix2 + iy2 + iz2))
+        if special.any():
+            nb2 = np.zeros_like(p)
+            den2 = np.zeros_like(p)
+            for a, inv in ((0, ix2), (1, iy2), (2, iz2)):
+                n = p.shape[a]
+                lo = _axis_view(p, a, slice(None, -1))
+                hi = _axis_view(p, a, slice(1, None))
+                o_lo = obst[lo] == 0
+                o_hi = obst[hi] == 0
+                nb2[hi] += np.where(o_lo, inv * p[lo], 0.0)
+                den2[hi] += np.where(o_lo, inv, 0.0)
+                nb2[lo] += np.where(o_hi, inv * p[hi], 0.0)
+                den2[lo] += np.where(o_hi, inv, 0.0)
+            if outlet:
+                den2[:, :, 0] += 2.0 * iz2
+            nb = np.where(special, nb2, nb)
+            den = np.where(special, den2, den)
+        safe = np.where(den > 0.0, den, 1.0)
+        p = np.where((obst == 1) | (den <= 0.0), 0.0, (nb - rhs) / safe)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Velocity-side kernels
+# ---------------------------------------------------------------------------
+
+def divergence(velocity: np.ndarray, obstacle: np.ndarray, widths, outlet: bool,
+               metric_dtype=F32, out_dtype=F32) -> np.ndarray:
+    """div = sum_d (u_d[c+e_d] - u_d[c]) / w_d over open faces (fluid.py:3450-3486).
+
+    Faces to obstacles or the domain boundary carry zero flux except the z-min
+    face of k = 0 cells when the outlet is on.  Computed in float32 like the
+    reference (f32 velocity and metrics), stored as ``out_dtype``.
+    """
+    u = np.asarray(velocity, dtype=metric_dtype)
+    obst = np.asarray(obstacle, np.int32)
+    free = obst == 0
+    nxyz = obst.shape
+    w = [np.asarray(widths[a], dtype=metric_dtype) for a in range(3)]
+    terms = []
+    for a in range(3):
+        n = nxyz[a]
+        left = np.zeros(nxyz, metric_dtype)
+        right = np.zeros(nxyz, metric_dtype)
+        hi = _axis_view(left, a, slice(1, None))
+        lo = _axis_view(left, a, slice(None, -1))
+        comp = u[..., a]
+        left[hi] = np.where(obst[lo] == 0, comp[hi], 0)
+        right[lo] = np.where(obst[hi] == 0, comp[hi], 0)
+        if a == 2 and outlet:
+            left[:, :, 0] = comp[:, :, 0]
+        shape = [1, 1, 1]
+        shape[a] = n
+        terms.append((right - left) / w[a].reshape(shape))
+    div = (terms[0] + terms[1]) + terms[2]
+    return np.where(free, div, 0).astype(out_dtype)
+
+
+def subtract_pressure_gradient(velocity, pressure, obstacle, widths, dt_over_rho: float, outlet: bool,
+                               vd_active=None, vd_weight=None, vd_value=None, metric_dtype=F32,
+                               origin=(0.0, 0.0, 0.0)):
+    """u_d -= (dt/rho) w (p_c - p_{c-e_d}) / d_d[c] on open low faces (fluid.py:5538-5582).
+
+    Outlet: grad_z = 2 p / w_z[0] at k = 0.  Obstacle cells get u = 0.  Rows
+    with an active velocity-Dirichlet flag use clamp(weight, 0, 1) and take
+    the prescribed value when the weight is <= 0.
+    """
+    u = np.asarray(velocity, np.float32).copy()
+    p = np.asarray(pressure, np.float64)
+    obst = np.asarray(obstacle, np.int32)
+    shape = obst.shape
+    dist = []
+    for a in range(3):
+        wa = np.asarray(widths[a], np.float64)
+        dist.append(np.asarray(center_distances(wa, origin[a]), metric_dtype).astype(np.float64))
+    wz0 = float(np.asarray(widths[2], metric_dtype)[0])
+    grad = np.zeros(shape + (3,), np.float64)
+    for a in range(3):
+        hi = _axis_view(p, a, slice(1, None))
+        lo = _axis_view(p, a, slice(None, -1))
+        bshape = [1, 1, 1]
+        bshape[a] = shape[a] - 1
+        g = np.zeros(shape, np.float64)
+        g[hi] = np.where(obst[lo] == 0, (p[hi] - p[lo]) / dist[a][1:].reshape(bshape), 0.0)
+        if a == 2 and outlet:
+            g[:, :, 0] = 2.0 * p[:, :, 0] / wz0
+        grad[..., a] = g
+    weight = np.ones(shape, np.float64)
+    if vd_active is not None:
+        act = np.asarray(vd_active, np.int32) != 0
+        wv = np.clip(np.asarray(vd_weight, np.float32).astype(np.float64), 0.0, 1.0)
+        weight = np.where(act, wv, 1.0)
+        if vd_value is not None:
+            fix = act & (weight <= 0.0)
+            u = np.where(fix[..., None], np.asarray(vd_value, np.float32), u)
+    corr = (weight[..., None] * float(F32(dt_over_rho)) * grad).astype(np.float32)
+    u = np.where((weight > 0.0)[..., None], u - corr, u)
+    u[obst == 1] = 0.0
+    return u
+
+
+def closed_wall_clamp(velocity, outlet: bool):
+    """Zero the normal flux on the low wall faces (fluid.py:5528-5536)."""
+    u = np.asarray(velocity, np.float32).copy()
+    u[0, :, :, 0] = 0.0
+    u[:, 0, :, 1] = 0.0
+    if not outlet:
+        u[:, :, 0, 2] = 0.0
+    return u
+
+
+def divergence_l2_stats(div, source, obstacle, interior_only=False, residual=True):
+    """max |div (- source)| and sqrt(sum/count) over free cells (fluid.py:3493-3565, 3827-3848)."""
+    d = np.asarray(div, np.float64)
+    if residual:
+        d = d - np.asarray(source, np.float64)
+    mask = np.asarray(obstacle) == 0
+    if interior_only:
+        inner = np.zeros_like(mask)
+        inner[1:-1, 1:-1, 1:-1] = True
+        mask &= inner
+    vals = np.abs(d[mask])
+    count = max(1, vals.size)
+    return {"max_abs": float(vals.max()) if vals.size else 0.0,
+            "l2": math.sqrt(float(np.sum(vals * vals)) / count)}
+
+
+def project_plain(velocity, pressure, obstacle, widths, *, rho: float, dt: float, solver: str = "fv_cg",
+                  iterations: int = 40, tolerance: float = 1.0e-6, preconditioner: str = "auto",
+                  outlet: bool = False, source=None, reset_pressure: bool = False,
+                  multigrid_cycles=None, metric_dtype=F32, coarse_dtype=F32):
+    """Plain-path ``project()`` (fluid.py:6889-7350) without interface terms,
+    velocity-Dirichlet rows, constraints or local cleanup: divergence, solve,
+    gradient subtraction, divergence again, closed-wall clamp.
+
+    Returns (velocity, pressure, divergence_after, info).
+    """
+    obst = np.asarray(obstacle, np.int32)
+    src = np.zeros(obst.shape, np.float32) if source is None else np.asarray(source, np.float32)
+    div0 = divergence(velocity, obst, widths, outlet, metric_dtype)
+    hier = build_hierarchy(widths, obst, metric_dtype=metric_dtype, coarse_dtype=coarse_dtype)
+    lvl = hier.levels[0]
+    p = np.zeros(obst.shape) if reset_pressure else np.asarray(pressure, np.float64)
+    rhs_scale = rho / dt
+    rhs0 = level0_rhs(div0, src, None, lvl, rhs_scale)
+    info = {}
+    if solver == "fv_cg":
+        res = pcg(rhs0, p, hier, outlet=outlet, tolerance=tolerance, max_iters=iterations,
+                  preconditioner=preconditioner, remove_mean=not outlet)
+        p = res.pressure if res.converged else np.zeros_like(p)
+        info["cg"] = res
+    elif solver == "fv_jacobi":
+        p = fv_jacobi_solve(rhs0, p, hier, outlet=outlet, iterations=iterations)
+    elif solver == "fv_multigrid":
+        p = fv_multigrid_solve(rhs0, p, hier, outlet=outlet, iterations=iterations, cycles=multigrid_cycles)
+    else:
+        raise ValueError(f"unsupported pressure_solver: {solver!r}")
+    u = subtract_pressure_gradient(velocity, p, obst, widths, dt / rho, outlet, metric_dtype=metric_dtype)
+    div1 = divergence(u, obst, widths, outlet, metric_dtype)
+    if not outlet:
+        u = closed_wall_clamp(u, outlet)
+    div2 = divergence(u, obst, widths, outlet, metric_dtype)
+    info["pre"] = divergence_l2_stats(div0, src, obst)
+    info["projection"] = divergence_l2_stats(div1, src, obst)
+    info["final"] = divergence_l2_stats(div2, src, obst)
+    return u, p, div2, info

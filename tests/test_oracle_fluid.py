@@ -1,0 +1,415 @@
+"""Pins the NumPy fluid oracle (oracle/fluid.py).
+
+Each test re-expresses a reference known-answer or invariant test
+(tests/test_core_fluid.py in the reference) against the oracle, or checks
+the oracle against a library routine (scipy.sparse spsolve) or a closed form.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import fluid as F
+from easyfsi_amd.simulation_core.grid import GradedGridSpec, RefinementRegion, build_graded_grid
+
+
+def _uniform(n, length=1.0):
+    return [np.full(n[a], length / n[a]) for a in range(3)]
+
+
+def _graded_obstacle_case():
+    """Graded slab-with-aperture case of test_core_fluid.py:15-53."""
+    box = 0.024
+    region = RefinementRegion(bounds_min_m=(0.30 * box, 0.30 * box, 0.76 * box),
+                              bounds_max_m=(0.70 * box, 0.70 * box, 1.24 * box), target_spacing_m=6.0e-4)
+    g = build_graded_grid(GradedGridSpec(bounds_min_m=(0, 0, 0), bounds_max_m=(box, box, 2 * box),
+                                         farfield_spacing_m=2.5e-3, max_growth_ratio=1.2,
+                                         refinement_regions=(region,)))
+    nx, ny, nz = g.grid_nodes
+    obst = np.zeros(g.grid_nodes, np.int32)
+    k = nz // 2
+    a = max(1, nx // 12)
+    cx, cy = nx // 2, ny // 2
+    obst[:, :, k] = 1
+    obst[cx - a:cx + a + 1, cy - a:cy + a + 1, k] = 0
+    widths = [np.asarray(g.cell_widths_x_m), np.asarray(g.cell_widths_y_m), np.asarray(g.cell_widths_z_m)]
+    return g, widths, obst
+
+
+# ---------------------------------------------------------------------------
+# divergence / gradient known answers
+# ---------------------------------------------------------------------------
+
+def test_divergence_zero_flux_on_obstacle_faces():  # test_core_fluid.py:1038-1055
+    obst = np.zeros((4, 4, 4), np.int32)
+    obst[2, 1, 1] = 1
+    u = np.zeros((4, 4, 4, 3), np.float32)
+    u[1, 1, 1, 0] = 1.25
+    div = F.divergence(u, obst, _uniform((4, 4, 4)), False)
+    assert div[0, 1, 1] == pytest.approx(5.0, abs=1e-6)
+    assert div[1, 1, 1] == pytest.approx(-5.0, abs=1e-6)
+    assert div[2, 1, 1] == pytest.approx(0.0, abs=1e-6)
+
+
+def test_outlet_divergence_counts_zmin_face():  # test_core_fluid.py:1018-1036
+    obst = np.zeros((4, 4, 4), np.int32)
+    obst[2, 2, 1] = 1
+    u = np.zeros((4, 4, 4, 3), np.float32)
+    u[2, 2, 0, 2] = -0.25
+    w = _uniform((4, 4, 4))
+    assert F.divergence(u, obst, w, False)[2, 2, 0] == pytest.approx(0.0)
+    assert F.divergence(u, obst, w, True)[2, 2, 0] == pytest.approx(1.0, abs=1e-6)
+
+
+def test_uniform_and_shear_fields_are_divergence_free():  # reference tests/test_fluid.py:28-44
+    n = (8, 8, 8)
+    w = _uniform(n)
+    obst = np.zeros(n, np.int32)
+    u = np.zeros(n + (3,), np.float32)
+    u[..., :] = (0.2, -0.1, 0.3)
+    u = F.closed_wall_clamp(u, False)
+    # interior cells see equal in/out flux
+    assert np.abs(F.divergence(u, obst, w, False)[1:-1, 1:-1, 1:-1]).max() == 0.0
+    y = (np.arange(8) + 0.5) / 8
+    shear = np.zeros(n + (3,), np.float32)
+    shear[..., 0] = (2.0 * (y - 0.5))[None, :, None]
+    shear = F.closed_wall_clamp(shear, False)
+    assert np.abs(F.divergence(shear, obst, w, False)[1:-1]).max() < 1e-6
+
+
+def test_divergence_of_gradient_is_operator():  # test_core_fluid.py:1057-1122
+    n = (5, 4, 4)
+    w = _uniform(n)
+    obst = np.zeros(n, np.int32)
+    obst[3, 1:3, 1:3] = 1
+    i, j, k = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    p = (0.7 * i - 0.3 * j + 0.11 * k + 0.05 * i * j).astype(np.float32).astype(np.float64)
+    p[obst == 1] = 0.0
+    u = F.subtract_pressure_gradient(np.zeros(n + (3,), np.float32), p, obst, w, 0.25, False)
+    div = F.divergence(u, obst, w, False)
+    lvl = F.make_level(w, obst)
+    expected = -0.25 * (-F.fv_apply(p, lvl, False))  # L p = nb - den p = -A p
+    free = obst == 0
+    np.testing.assert_allclose(div[free], expected[free], rtol=2e-5, atol=1e-5)
+
+
+def test_gradient_divergence_is_globally_conservative():  # test_core_fluid.py:1124-1146
+    n = (6, 5, 4)
+    w = _uniform(n)
+    obst = np.zeros(n, np.int32)
+    obst[3, 1:4, 1:3] = 1
+    i, j, k = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    p = (0.17 * i * i - 0.31 * j + 0.13 * k).astype(np.float32).astype(np.float64)
+    p[obst == 1] = 0.0
+    u = F.subtract_pressure_gradient(np.zeros(n + (3,), np.float32), p, obst, w, 0.4, False)
+    # evaluated in float64 so only the face structure is tested, not f32 summation luck
+    div = F.divergence(u, obst, w, False, metric_dtype=np.float64, out_dtype=np.float64)
+    vol = (1 / 6) * (1 / 5) * (1 / 4)
+    assert abs(float(np.sum(div[obst == 0]) * vol)) < 1e-12
+
+
+def test_closed_wall_clamp_preserves_max_faces():  # test_core_fluid.py:823-845
+    u = np.zeros((4, 4, 4, 3), np.float32)
+    u[0, 1, 1, 0], u[-1, 1, 1, 0] = -0.2, 0.3
+    u[1, 0, 1, 1], u[1, -1, 1, 1] = -0.4, 0.5
+    u[1, 1, 0, 2], u[1, 1, -1, 2] = -0.6, 0.7
+    c = F.closed_wall_clamp(u, False)
+    assert c[0, 1, 1, 0] == 0 and c[1, 0, 1, 1] == 0 and c[1, 1, 0, 2] == 0
+    assert c[-1, 1, 1, 0] == pytest.approx(0.3) and c[1, -1, 1, 1] == pytest.approx(0.5)
+    assert c[1, 1, -1, 2] == pytest.approx(0.7)
+    assert F.closed_wall_clamp(u, True)[1, 1, 0, 2] == pytest.approx(-0.6)
+
+
+def test_obstacle_cells_get_zero_velocity():  # test_core_fluid.py:882-898
+    obst = np.zeros((8, 8, 8), np.int32)
+    obst[4, 4, 4] = 1
+    p = np.zeros((8, 8, 8))
+    p[4, 4, 4] = 1000.0
+    u = F.subtract_pressure_gradient(np.zeros((8, 8, 8, 3), np.float32), p, obst, _uniform((8, 8, 8)), 1e-6, False)
+    assert np.abs(u).max() < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# operator
+# ---------------------------------------------------------------------------
+
+def test_interface_diagonal_enters_operator_and_jacobi():  # test_core_fluid.py:1677-1705
+    n = (5, 5, 5)
+    diag = np.zeros(n)
+    diag[2, 2, 2] = 7.5
+    lvl = F.make_level(_uniform(n), np.zeros(n, np.int32), iface_diag=diag)
+    q = F.fv_apply(np.ones(n), lvl, False)
+    dg = F.fv_diagonal(lvl, False)
+    r = np.zeros(n)
+    r[2, 2, 2] = 15.0
+    z = F.jacobi_precondition(r, dg, lvl)
+    base = 6.0 / (0.2 * 0.2)
+    assert q[2, 2, 2] == pytest.approx(7.5, abs=1e-5)
+    assert dg[2, 2, 2] == pytest.approx(base + 7.5, abs=3e-5)
+    assert z[2, 2, 2] == pytest.approx(15.0 / (base + 7.5), abs=1e-6)
+
+
+def test_operator_is_volume_self_adjoint_on_graded_obstacle_grid():  # test_core_fluid.py:1648-1675
+    g, w, obst = _graded_obstacle_case()
+    rng = np.random.default_rng(17)
+    x = rng.normal(size=g.grid_nodes).astype(np.float32).astype(np.float64)
+    y = rng.normal(size=g.grid_nodes).astype(np.float32).astype(np.float64)
+    x[obst == 1] = 0
+    y[obst == 1] = 0
+    lvl = F.make_level(w, obst)
+    lhs = F.weighted_dot(F.fv_apply(x, lvl, True), y, lvl)
+    rhs = F.weighted_dot(x, F.fv_apply(y, lvl, True), lvl)
+    assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) < 1e-12
+
+
+def test_uniform_operator_second_order():  # closed form, test_core_fluid.py:1353-1375
+    errs = []
+    for n in (16, 32, 64):
+        c = (np.arange(n) + 0.5) / n
+        x, y, z = np.meshgrid(c, c, c, indexing="ij")
+        p = np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)
+        lvl = F.make_level(_uniform((n, n, n)), np.zeros((n,) * 3, np.int32), metric_dtype=np.float64)
+        q = F.fv_apply(p, lvl, False)
+        e = (q - 3 * np.pi ** 2 * p)[1:-1, 1:-1, 1:-1]
+        errs.append(math.sqrt(float(np.mean(e * e))))
+    assert errs[1] < 0.30 * errs[0] and errs[2] < 0.30 * errs[1]
+
+
+def test_nonuniform_operator_monotone_convergence():  # test_core_fluid.py:1377-1417
+    rel = []
+    for n in (12, 18, 24):
+        q = (np.arange(n) + 0.5) / n
+        raw = 1.0 + 0.3 * np.sin(2 * np.pi * q)
+        w = raw / raw.sum()
+        widths = [w, w, w]
+        cen = np.cumsum(w) - 0.5 * w
+        x, y, z = np.meshgrid(cen, cen, cen, indexing="ij")
+        p = (np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)).astype(np.float32)
+        lvl = F.make_level(widths, np.zeros((n,) * 3, np.int32))
+        ap = F.fv_apply(p.astype(np.float64), lvl, False)
+        exp = 3 * np.pi ** 2 * p.astype(np.float64)
+        inner = (slice(2, -2),) * 3
+        err = ap[inner] - exp[inner]
+        rel.append(math.sqrt(float(np.mean(err ** 2))) / math.sqrt(float(np.mean(exp[inner] ** 2))))
+    assert rel[1] < rel[0] and rel[2] < rel[1] and rel[2] < 4e-3
+
+
+def test_operator_matches_scalar_face_loop_with_dirichlet_weights():
+    """Vectorised operator against the per-cell face loop of fluid.py:4079-4221
+    on a small graded grid with obstacles, velocity-Dirichlet weights and the
+    outlet (guards index/ownership mistakes in the vectorisation)."""
+    rng = np.random.default_rng(3)
+    n = (5, 6, 7)
+    w = [rng.uniform(0.5, 1.5, size=m) for m in n]
+    obst = (rng.random(n) < 0.15).astype(np.int32)
+    vda = (rng.random(n) < 0.2).astype(np.int32)
+    vdw = rng.uniform(-0.2, 1.2, size=n).astype(np.float32)
+    lvl = F.make_level(w, obst, vd_active=vda, vd_weight=vdw)
+    p = rng.normal(size=n)
+    q = F.fv_apply(p, lvl, True)
+    W, D = lvl.w, lvl.d
+
+    def fw(i, j, k):
+        return min(max(float(vdw[i, j, k]), 0.0), 1.0) if vda[i, j, k] else 1.0
+
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for k in range(n[2]):
+                if obst[i, j, k]:
+                    assert q[i, j, k] == 0.0
+                    continue
+                nb = den = 0.0
+                for (ii, jj, kk, c, wt) in (
+                        (i - 1, j, k, 1 / (W[0][i] * D[0][i]) if i > 0 else 0, fw(i, j, k)),
+                        (i + 1, j, k, 1 / (W[0][i] * D[0][i + 1]) if i < n[0] - 1 else 0, fw(min(i + 1, n[0] - 1), j, k)),
+                        (i, j - 1, k, 1 / (W[1][j] * D[1][j]) if j > 0 else 0, fw(i, j, k)),
+                        (i, j + 1, k, 1 / (W[1][j] * D[1][j + 1]) if j < n[1] - 1 else 0, fw(i, min(j + 1, n[1] - 1), k)),
+                        (i, j, k - 1, 1 / (W[2][k] * D[2][k]) if k > 0 else 0, fw(i, j, k)),
+                        (i, j, k + 1, 1 / (W[2][k] * D[2][k + 1]) if k < n[2] - 1 else 0, fw(i, j, min(k + 1, n[2] - 1)))):
+                    if 0 <= ii < n[0] and 0 <= jj < n[1] and 0 <= kk < n[2] and not obst[ii, jj, kk]:
+                        nb += c * wt * p[ii, jj, kk]
+                        den += c * wt
+                if k == 0:
+                    den += 2.0 / (W[2][0] ** 2)
+                assert q[i, j, k] == pytest.approx(den * p[i, j, k] - nb, rel=1e-12, abs=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# multigrid pieces
+# ---------------------------------------------------------------------------
+
+def test_restriction_marks_mixed_obstacle_block_and_dirichlet():  # test_core_fluid.py:1166-1204
+    obst = np.zeros((8, 8, 8), np.int32)
+    obst[1, 1, 1] = 1
+    vda = np.zeros((8, 8, 8), np.int32)
+    vda[2, 2, 2] = 1
+    h = F.build_hierarchy(_uniform((8, 8, 8)), obst, vd_active=vda)
+    assert h.levels[1].obstacle[0, 0, 0] == 1
+    assert h.levels[1].vd_active[1, 1, 1] == 1
+
+
+def test_restriction_volume_averages_interface_diagonal():  # test_core_fluid.py:1206-1241
+    diag = np.zeros((8, 8, 8))
+    diag[2, 2, 2] = 80.0
+    h = F.build_hierarchy(_uniform((8, 8, 8)), np.zeros((8, 8, 8), np.int32), iface_diag=diag)
+    assert h.levels[1].iface_diag[1, 1, 1] == pytest.approx(10.0, abs=1e-5)
+    assert h.levels[1].iface_diag[0, 0, 0] == pytest.approx(0.0, abs=1e-6)
+
+
+def test_coarse_smoother_uses_coarse_diagonal():  # test_core_fluid.py:1243-1266
+    diag = np.zeros((8, 8, 8))
+    diag[1, 1, 1] = 1.0e8
+    h = F.build_hierarchy(_uniform((8, 8, 8)), np.zeros((8, 8, 8), np.int32), iface_diag=diag)
+    # the reference test clears the coarse diagonal before smoothing (fluid.py:5962-5975)
+    lvl1 = h.levels[1]
+    lvl1.iface_diag = np.zeros_like(lvl1.iface_diag)
+    rhs = np.zeros(lvl1.shape, np.float32)
+    rhs[1, 1, 1] = 1.0
+    p = F.smooth(np.zeros(lvl1.shape, np.float32), rhs, lvl1, False, 1, omega=1.0)
+    assert p[1, 1, 1] < -1e-3
+
+
+def test_restriction_and_prolongation_on_odd_shapes():
+    """Odd fine sizes: coarse cell n-1 has one child per axis; prolongation clamps."""
+    n = (5, 7, 4)
+    lvl0 = F.make_level(_uniform(n), np.zeros(n, np.int32))
+    cw = [F.coarsen_widths(w) for w in _uniform(n)]
+    lvl1 = F.restrict_masks(lvl0, cw, np.float64)
+    ones = np.ones(n)
+    np.testing.assert_allclose(F.restrict_residual(ones, lvl0, lvl1), 1.0)
+    up = F.prolongate_add(np.zeros(n), lvl0, np.arange(np.prod(lvl1.shape), dtype=float).reshape(lvl1.shape), 1.0)
+    assert up[4, 6, 3] == pytest.approx(lvl1.shape[1] * lvl1.shape[2] * 2 + lvl1.shape[2] * 3 + 1)
+
+
+# ---------------------------------------------------------------------------
+# solvers
+# ---------------------------------------------------------------------------
+
+def _assemble(lvl, outlet):
+    """Sparse matrix of A on free cells (built from fv_apply columns on unit vectors)."""
+    free = np.flatnonzero(lvl.free.ravel())
+    n = lvl.obstacle.size
+    cols = []
+    for c in free:
+        e = np.zeros(n)
+        e[c] = 1.0
+        cols.append(F.fv_apply(e.reshape(lvl.shape), lvl, outlet).ravel()[free])
+    return sp.csc_matrix(np.stack(cols, axis=1)), free
+
+
+def test_pcg_matches_spsolve_with_outlet():
+    """Outlet makes A nonsingular: PCG to 1e-12 equals the sparse direct solve."""
+    rng = np.random.default_rng(5)
+    w = [np.asarray(v) for v in (0.1 * (1 + 0.3 * np.sin(np.arange(6))), np.full(5, 0.2), 0.1 + 0.02 * np.arange(7))]
+    obst = np.zeros((6, 5, 7), np.int32)
+    obst[2:4, 2, 3] = 1
+    h = F.build_hierarchy(w, obst)
+    lvl = h.levels[0]
+    rhs0 = np.where(lvl.free, rng.normal(size=lvl.shape), 0.0)
+    res = F.pcg(rhs0, np.zeros(lvl.shape), h, outlet=True, tolerance=1e-12, max_iters=400,
+                preconditioner="fv_multigrid", remove_mean=False)
+    assert res.converged, res.breakdown
+    A, free = _assemble(lvl, True)
+    x = spla.spsolve(A, -rhs0.ravel()[free])
+    np.testing.assert_allclose(res.pressure.ravel()[free], x, rtol=1e-9, atol=1e-9 * np.abs(x).max())
+
+
+def test_pcg_closed_domain_matches_spsolve_up_to_mean():
+    rng = np.random.default_rng(8)
+    n = (6, 6, 5)
+    w = [np.full(6, 1 / 6), 0.1 + 0.05 * np.arange(6), np.full(5, 0.2)]
+    obst = np.zeros(n, np.int32)
+    obst[3, 3, 1:3] = 1
+    h = F.build_hierarchy(w, obst)
+    lvl = h.levels[0]
+    rhs0 = np.where(lvl.free, rng.normal(size=n), 0.0)
+    res = F.pcg(rhs0, np.zeros(n), h, outlet=False, tolerance=1e-12, max_iters=400,
+                preconditioner="fv_multigrid", remove_mean=True)
+    assert res.converged, res.breakdown
+    A, free = _assemble(lvl, False)
+    vol = lvl.volume64().ravel()[free]
+    b = -rhs0.ravel()[free]
+    b = b - np.sum(b * vol) / vol.sum()
+    # bordered system fixes the V-mean of x to zero
+    m = A.shape[0]
+    K = sp.bmat([[A, sp.csc_matrix(vol.reshape(-1, 1))], [sp.csc_matrix(vol.reshape(1, -1)), None]]).tocsc()
+    sol = spla.spsolve(K, np.concatenate([b, [0.0]]))[:m]
+    np.testing.assert_allclose(res.pressure.ravel()[free], sol, rtol=1e-8, atol=1e-8 * np.abs(sol).max())
+    assert abs(F.weighted_mean(res.pressure, lvl)) < 1e-12 * np.abs(sol).max()
+
+
+def test_pcg_closed_domain_mean_is_zero():  # test_core_fluid.py:2997-3056
+    w = [0.03 * (1.0 + 0.1 * ((i % 3) - 1)) for i in range(10)]
+    widths = [np.asarray(w)] * 3
+    vol = np.asarray(w)[:, None, None] * np.asarray(w)[None, :, None] * np.asarray(w)[None, None, :]
+    src = np.zeros((10, 10, 10), np.float32)
+    pos, neg = np.s_[2:4, 2:4, 2:4], np.s_[6:8, 6:8, 6:8]
+    src[pos] = 1e-7 / vol[pos].sum()
+    src[neg] = -1e-7 / vol[neg].sum()
+    obst = np.zeros((10, 10, 10), np.int32)
+    _, p, _, info = F.project_plain(np.zeros((10, 10, 10, 3), np.float32), np.zeros((10, 10, 10)), obst, widths,
+                                   rho=1000.0, dt=1e-3, solver="fv_cg", iterations=96, tolerance=1e-6,
+                                   source=src, reset_pressure=True)
+    res = info["cg"]
+    assert res.converged and res.relative_residual < 1e-6 and res.iterations < 96
+    assert res.used_multigrid  # auto on a graded grid
+    assert abs(float(np.sum(p * vol) / vol.sum())) < 1e-7
+
+
+def test_pcg_graded_outlet_converges_and_balances_flux():  # test_core_fluid.py:1591-1646
+    w = [0.02 * (1.0 + 0.15 * ((i % 4) / 3.0)) for i in range(24)]
+    widths = [np.asarray(w)] * 3
+    W = np.asarray(w)
+    vol = W[:, None, None] * W[None, :, None] * W[None, None, :]
+    src = np.zeros((24,) * 3, np.float32)
+    cells = np.s_[11:13, 11:13, 18:20]
+    src[cells] = 2e-6 / vol[cells].sum()
+    obst = np.zeros((24,) * 3, np.int32)
+    u, _, _, info = F.project_plain(np.zeros((24,) * 3 + (3,), np.float32), np.zeros((24,) * 3), obst, widths,
+                                    rho=1000.0, dt=1e-3, solver="fv_cg", iterations=160, tolerance=1e-6,
+                                    outlet=True, source=src, reset_pressure=True)
+    res = info["cg"]
+    assert res.converged and res.iterations < 160 and res.relative_residual < 1e-6
+    area = W[:, None] * W[None, :]
+    outflow = float(np.sum(-u[:, :, 0, 2] * area))
+    assert abs(1.0 - outflow / 2e-6) < 1e-3
+
+
+def test_pcg_zero_rhs_gives_zero_pressure():  # test_core_fluid.py:3058-3079
+    n = (6, 6, 6)
+    stale = np.broadcast_to(np.linspace(0, 1, 6)[None, None, :], n).copy()
+    u0 = np.zeros(n + (3,), np.float32)
+    u, p, _, info = F.project_plain(u0, stale, np.zeros(n, np.int32), _uniform(n), rho=1000.0, dt=1e-3,
+                                    solver="fv_cg", iterations=8, tolerance=1e-6)
+    assert info["cg"].converged and info["cg"].initial_relative_residual == 0.0
+    assert np.abs(p).max() == 0.0 and np.abs(u).max() == 0.0
+
+
+def test_multigrid_solver_beats_jacobi_on_outlet_balance():  # test_core_fluid.py:3081-3110
+    n = (16, 16, 16)
+    src = np.zeros(n, np.float32)
+    vol = (1 / 16) ** 3
+    src[7:9, 7:9, 10:12] = 2e-6 / (8 * vol)
+    errs = {}
+    for solver, kw in (("fv_jacobi", {}), ("fv_multigrid", {"multigrid_cycles": 8})):
+        u, _, _, _ = F.project_plain(np.zeros(n + (3,), np.float32), np.zeros(n), np.zeros(n, np.int32),
+                                     _uniform(n), rho=1000.0, dt=1e-3, solver=solver, iterations=8, outlet=True,
+                                     source=src, reset_pressure=True, **kw)
+        errs[solver] = abs(1.0 - float(np.sum(-u[:, :, 0, 2]) * (1 / 16) ** 2) / 2e-6)
+    assert errs["fv_multigrid"] < 0.5 * errs["fv_jacobi"]
+
+
+def test_projection_reduces_sinusoidal_divergence():  # test_core_fluid.py:927-941
+    n = (24, 24, 24)
+    c = (np.arange(24, dtype=np.float32) + 0.5) / 24
+    u = np.zeros(n + (3,), np.float32)
+    u[..., 0] = (0.15 * np.sin(2 * np.pi * c))[:, None, None]
+    u[..., 1] = (0.075 * np.sin(2 * np.pi * c))[None, :, None]
+    u[..., 2] = (-0.0375 * np.sin(2 * np.pi * c))[None, None, :]
+    _, _, _, info = F.project_plain(u, np.zeros(n), np.zeros(n, np.int32), _uniform(n), rho=1000.0, dt=2e-3,
+                                    solver="fv_cg", iterations=480, tolerance=1e-6, preconditioner="fv_multigrid")
+    assert info["pre"]["l2"] > 0.1
+    assert info["final"]["l2"] < 0.05 * info["pre"]["l2"]
