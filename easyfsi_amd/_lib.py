@@ -1,0 +1,162 @@
+"""ctypes binding of libhfsi.so (the C ABI declared in include/hfsi.h).
+
+torch is imported before the library is loaded so that libhfsi binds to the
+HIP runtime torch already loaded (same soname), and one runtime owns the
+device, the streams and the allocations on both sides.
+
+There is no fallback: when the library is missing or no gfx950 device is
+visible, ``lib()`` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (must precede loading libhfsi, see module docstring)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhfsi.so")
+
+HFSI_PREC_REF64 = 1
+HFSI_PREC_FP32 = 2
+HFSI_PREC_MIXED = 4
+
+PRECONDITIONERS = {"auto": 0, "jacobi": 1, "fv_multigrid": 2, "fv_multigrid_light": 3}
+
+BREAKDOWN_MESSAGES = {
+    0: "",
+    1: "initial residual/preconditioner dot is not positive finite",
+    2: "device-side CG scalar update failed",
+    3: "restarted residual/preconditioner dot is not positive finite",
+}
+
+
+class CgStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("breakdown_code", ctypes.c_int32),
+        ("breakdown_kind", ctypes.c_int32),
+        ("host_residual_checks", ctypes.c_int32),
+        ("mean_projection_count", ctypes.c_int32),
+        ("restart_count", ctypes.c_int32),
+        ("used_multigrid", ctypes.c_int32),
+        ("true_residual_replacements", ctypes.c_int32),
+        ("vcycles", ctypes.c_int32),
+        ("b_norm", ctypes.c_double),
+        ("initial_relative_residual", ctypes.c_double),
+        ("relative_residual", ctypes.c_double),
+        ("breakdown_dAd", ctypes.c_double),
+        ("true_relative_residual", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class CgOptions(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("preconditioner", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("remove_mean", ctypes.c_int32),
+        ("outlet_zmin", ctypes.c_int32),
+        ("interface_active", ctypes.c_int32),
+        ("check_interval", ctypes.c_int32),
+        ("restart_interval", ctypes.c_int32),
+        ("true_residual", ctypes.c_int32),
+        ("true_residual_at_exit", ctypes.c_int32),
+        ("tolerance", ctypes.c_double),
+        ("rhs_scale", ctypes.c_double),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_D = ctypes.c_double
+_F = ctypes.c_float
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "hfsi_version": (ctypes.c_int, []),
+    "hfsi_last_error": (ctypes.c_char_p, []),
+    "hfsi_device_ok": (ctypes.c_int, []),
+    "hfsi_fluid_create": (ctypes.c_int, [ctypes.POINTER(_P), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
+    "hfsi_fluid_destroy": (ctypes.c_int, [_P]),
+    "hfsi_fluid_num_levels": (_I, [_P]),
+    "hfsi_fluid_level_shape": (ctypes.c_int, [_P, _I, _P]),
+    "hfsi_fluid_device_bytes": (ctypes.c_size_t, [_P]),
+    "hfsi_fluid_set_masks": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "hfsi_fluid_divergence": (ctypes.c_int, [_P, _P, _P, _I, _P]),
+    "hfsi_fluid_apply": (ctypes.c_int, [_P, _P, _P, _I, _P]),
+    "hfsi_fluid_diagonal": (ctypes.c_int, [_P, _P, _I, _P]),
+    "hfsi_fluid_weighted_dot": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(_D), _P]),
+    "hfsi_fluid_vcycle": (ctypes.c_int, [_P, _P, _P, _I, _I, _I, _I, _I, _P]),
+    "hfsi_fluid_solve_cg": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.POINTER(CgOptions), ctypes.POINTER(CgStats), _P]),
+    "hfsi_fluid_solve_fv_jacobi": (ctypes.c_int, [_P, _P, _P, _P, _P, _D, _I, _I, _P]),
+    "hfsi_fluid_solve_fv_multigrid": (ctypes.c_int, [_P, _P, _P, _P, _P, _D, _I, _I, _I, _P]),
+    "hfsi_fluid_solve_legacy_jacobi": (ctypes.c_int, [_P, _P, _P, _P, _P, _D, _D, _D, _D, _I, _I, _P]),
+    "hfsi_fluid_subtract_gradient": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F, _I, _P]),
+    "hfsi_fluid_wall_clamp": (ctypes.c_int, [_P, _P, _I, _P]),
+    "hfsi_fluid_divergence_report": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P]),
+    "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
+    "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),
+}
+
+TIMER_KERNELS = {"l0_sweep": 1, "spmv_dot": 2, "l0_restrict": 3, "l0_prolong_sweep": 4}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class HfsiError(RuntimeError):
+    """A libhfsi call returned a non-zero status."""
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libhfsi.so and bind every declared entry point (no device needed)."""
+    if not os.path.exists(path):
+        raise HfsiError(f"{path} not found; build it with `python -m easyfsi_amd.build`")
+    cdll = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(cdll, name)
+        fn.restype = res
+        fn.argtypes = args
+    return cdll
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library, checked to run on a visible gfx950 device."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not torch.cuda.is_available():
+                raise HfsiError("no ROCm GPU is visible; libhfsi needs an MI355X (gfx950)")
+            torch.cuda.init()
+            cdll = load_library()
+            if cdll.hfsi_device_ok() != 1:
+                raise HfsiError("device 0 is not a gfx950 (MI355X) GPU")
+            _lib = cdll
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = lib().hfsi_last_error().decode(errors="replace")
+        raise HfsiError(f"{what or 'libhfsi call'} failed ({status}): {msg}")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("libhfsi needs contiguous tensors")
+    return t.data_ptr()
+
+
+def stream_handle(stream=None) -> int:
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream

@@ -1,0 +1,154 @@
+// Shared device/host helpers for libhfsi (gfx950).
+//
+// Work decomposition used by every full-grid kernel: the flat cell range is
+// cut into chunks of blockDim.x cells.  Workgroups are dispatched round-robin
+// over the 8 XCDs (workgroup b runs on XCD b % 8), so XCD x is given the
+// contiguous chunk range [x*C/8, (x+1)*C/8) and its workgroups stride over it.
+// Each XCD therefore streams one contiguous slab of the grid and the +-j / +-i
+// stencil neighbours of a chunk are served by that XCD's own L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+
+#ifndef HFSI_NXCD
+#define HFSI_NXCD 8
+#endif
+
+namespace hfsi {
+
+constexpr int kBlock = 256;       // threads per workgroup of the full-grid kernels
+constexpr int kMaxPartials = 4096;  // workgroups of a reducing launch (upper bound)
+constexpr int kMaxSums = 8;       // values reduced by one launch
+
+void set_error(const std::string& msg);
+
+#define HFSI_CHECK(expr)                                                               \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      ::hfsi::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
+      return HFSI_ERR_HIP;                                                             \
+    }                                                                                  \
+  } while (0)
+
+#define HFSI_LAUNCH_CHECK()                                                            \
+  do {                                                                                 \
+    hipError_t _e = hipGetLastError();                                                 \
+    if (_e != hipSuccess) {                                                            \
+      ::hfsi::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));      \
+      return HFSI_ERR_HIP;                                                             \
+    }                                                                                  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// XCD-aware chunk iteration
+// ---------------------------------------------------------------------------
+
+struct ChunkRange {
+  uint32_t first, end, step;
+};
+
+// Requires gridDim.x to be a multiple of HFSI_NXCD.
+__device__ inline ChunkRange xcd_chunks(uint32_t nchunks) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t per = gridDim.x / HFSI_NXCD;
+  const uint32_t x = b % HFSI_NXCD;
+  const uint32_t s = b / HFSI_NXCD;
+  const uint32_t lo = (uint32_t)(((uint64_t)nchunks * x) / HFSI_NXCD);
+  const uint32_t hi = (uint32_t)(((uint64_t)nchunks * (x + 1)) / HFSI_NXCD);
+  return ChunkRange{lo + s, hi, per};
+}
+
+// Number of workgroups for a full-grid launch over n cells.
+inline uint32_t grid_blocks(uint64_t n, uint32_t max_blocks) {
+  uint64_t chunks = (n + kBlock - 1) / kBlock;
+  uint64_t g = chunks < max_blocks ? chunks : max_blocks;
+  g = (g + HFSI_NXCD - 1) / HFSI_NXCD * HFSI_NXCD;
+  if (g == 0) g = HFSI_NXCD;
+  return (uint32_t)g;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront / workgroup reductions (64-wide wavefronts)
+// ---------------------------------------------------------------------------
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum NS values over the workgroup; result valid in thread 0.  `lds` must hold
+// NS * (blockDim.x / 64) doubles.
+template <int NS>
+__device__ inline void block_sum(double (&v)[NS], double* lds) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) v[s] = wave_sum(v[s]);
+  if (lane == 0) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) lds[s * nw + wave] = v[s];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double acc = 0.0;
+      for (int w = 0; w < nw; ++w) acc += lds[s * nw + w];
+      v[s] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// Deterministic grid reduction: every workgroup stores its NS partial sums,
+// the last workgroup to finish adds them in workgroup order and leaves the
+// totals in `totals` (device memory); returns true in that workgroup's
+// threads.  `counter` must be zero on entry and is reset to zero.
+template <int NS>
+__device__ inline bool grid_sum(double (&v)[NS], double* partials, unsigned int* counter, double* totals,
+                                double* lds) {
+  block_sum<NS>(v, lds);
+  __shared__ bool is_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) partials[(size_t)blockIdx.x * NS + s] = v[s];
+    __threadfence();
+    unsigned int done = atomicAdd(counter, 1u);
+    is_last = (done == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();  // acquire side: partial sums of all workgroups are visible
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      acc[s] += __hip_atomic_load(&partials[(size_t)b * NS + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum<NS>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) totals[s] = acc[s];
+    *counter = 0u;
+  }
+  __syncthreads();
+  return true;
+}
+
+}  // namespace hfsi

@@ -1,0 +1,1939 @@
+// libhfsi: FV pressure projection on the graded MAC grid for gfx950.
+//
+// Host orchestration of the reference's flexible PCG with the Jacobi-smoothed
+// V-cycle preconditioner (simulation_core/fluid.py:6535-6811, 6085-6262) on
+// hand-written HIP kernels.  Scalars of the CG recurrence live on the device;
+// the host synchronises only at residual checks (every `check_interval`
+// iterations, 16 in the reference) and at the start of a solve.
+//
+// Precision modes
+//   REF64: float64 CG vectors and level-0 V-cycle, float32 coarse-level storage,
+//          float64 arithmetic everywhere (the reference's dtypes).
+//   FP32 : float32 CG vectors and V-cycle with float32 arithmetic, float64
+//          solution x, right-hand side b and dots; every residual check
+//          replaces r by the float64 true residual b - A x.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstring>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../../include/hfsi.h"
+#include "fluid_kernels.hpp"
+
+namespace hfsi {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+constexpr int kMaxLevels = 16;
+constexpr uint32_t kMaxGrid = 2048;      // workgroups of full-grid launches (8 per CU)
+constexpr uint32_t kTailCells = 4096;    // levels at most this big run in the single-workgroup tail
+constexpr int kTailThreads = 1024;
+
+// ---------------------------------------------------------------------------
+// Device-resident CG scalars
+// ---------------------------------------------------------------------------
+
+struct Scalars {
+  double rz, dAd, alpha, beta, rr, mz, mx, mr, mb, bnorm2, free_vol, code_dAd;
+  int32_t code, pad;
+};
+
+struct FinNone {
+  __device__ void operator()(const double*) const {}
+};
+struct FinStore {  // leave totals in place (read by the host)
+  __device__ void operator()(const double*) const {}
+};
+struct FinFreeVol {
+  Scalars* s;
+  __device__ void operator()(const double* t) const { s->free_vol = t[0]; }
+};
+// alpha = rz / dAd, code 1 and first dAd latched when dAd <= 0 (fluid.py:4566-4574).
+struct FinDad {
+  Scalars* s;
+  __device__ void operator()(const double* t) const {
+    double dAd = t[0];
+    s->dAd = dAd;
+    s->alpha = 0.0;
+    if (dAd <= 0.0) {
+      if (s->code == 0) s->code_dAd = dAd;
+      s->code = 1;
+    } else {
+      s->alpha = s->rz / dAd;
+    }
+  }
+};
+// After the preconditioner: t = {<r,z>, <z,q>, sum Vz, sum Vr, sum Vq}.  The
+// V-mean of z is removed algebraically (fluid.py:4715-4726), the flexible
+// numerator z.(r - r_old) = -alpha z.q (r - r_old = -alpha q on free cells and z
+// has zero mean), beta per fluid.py:4589-4601, then rz = rz_new.
+struct FinDots5 {
+  Scalars* s;
+  int remove_mean, use_mg;
+  __device__ void operator()(const double* t) const {
+    double V = s->free_vol;
+    double mz = (remove_mean && V > 0.0) ? t[2] / V : 0.0;
+    double rz_new = t[0] - mz * t[3];
+    double beta_num = use_mg ? -s->alpha * (t[1] - mz * t[4]) : 0.0;
+    double beta = 0.0;
+    if (s->rz <= 1.0e-300 || rz_new <= 0.0) {
+      s->code = 2;
+    } else if (use_mg) {
+      beta = beta_num / s->rz;
+      if (beta < 0.0) beta = 0.0;
+    } else {
+      beta = rz_new / s->rz;
+    }
+    s->mz = mz;
+    s->beta = beta;
+    s->rz = rz_new;
+  }
+};
+// Start / restart: t = {<r,z>, sum Vz, sum Vr}.
+struct FinDots3 {
+  Scalars* s;
+  int remove_mean;
+  __device__ void operator()(const double* t) const {
+    double V = s->free_vol;
+    double mz = (remove_mean && V > 0.0) ? t[1] / V : 0.0;
+    s->mz = mz;
+    s->rz = t[0] - mz * t[2];
+    s->beta = 0.0;
+  }
+};
+struct FinMeansXR {  // t = {sum Vx, sum Vr}
+  Scalars* s;
+  __device__ void operator()(const double* t) const {
+    double V = s->free_vol;
+    s->mx = V > 0.0 ? t[0] / V : 0.0;
+    s->mr = V > 0.0 ? t[1] / V : 0.0;
+  }
+};
+struct FinMeanR {  // t = {<r,r>, sum Vr}
+  Scalars* s;
+  int remove_mean;
+  __device__ void operator()(const double* t) const {
+    double V = s->free_vol;
+    s->mr = (remove_mean && V > 0.0) ? t[1] / V : 0.0;
+    s->mx = 0.0;
+    s->rr = t[0];
+  }
+};
+struct FinRR {
+  Scalars* s;
+  __device__ void operator()(const double* t) const { s->rr = t[0]; }
+};
+struct FinBMean {
+  Scalars* s;
+  int remove_mean;
+  __device__ void operator()(const double* t) const {
+    double V = s->free_vol;
+    s->mb = (remove_mean && V > 0.0) ? t[0] / V : 0.0;
+  }
+};
+struct FinBNorm {
+  Scalars* s;
+  __device__ void operator()(const double* t) const { s->bnorm2 = t[0]; }
+};
+
+// ---------------------------------------------------------------------------
+// Pointwise / reduction functors
+// ---------------------------------------------------------------------------
+
+struct OpVolume {
+  __device__ void operator()(uint32_t, double V, double (&acc)[1]) const { acc[0] += V; }
+};
+
+template <typename TA, typename TB>
+struct OpDot {
+  const TA* a;
+  const TB* b;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[1]) const {
+    acc[0] += (double)a[c] * (double)b[c] * V;
+  }
+};
+
+template <typename TR, typename TZ, typename TQ>
+struct OpDots5 {
+  const TR* r;
+  const TZ* z;
+  const TQ* q;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[5]) const {
+    double rv = (double)r[c], zv = (double)z[c], qv = (double)q[c];
+    acc[0] += rv * zv * V;
+    acc[1] += zv * qv * V;
+    acc[2] += zv * V;
+    acc[3] += rv * V;
+    acc[4] += qv * V;
+  }
+};
+
+template <typename TR, typename TZ>
+struct OpDots3 {
+  const TR* r;
+  const TZ* z;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[3]) const {
+    double rv = (double)r[c], zv = (double)z[c];
+    acc[0] += rv * zv * V;
+    acc[1] += zv * V;
+    acc[2] += rv * V;
+  }
+};
+
+template <typename TR>
+struct OpSumsXR {
+  const double* x;
+  const TR* r;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[2]) const {
+    acc[0] += x[c] * V;
+    acc[1] += (double)r[c] * V;
+  }
+};
+
+template <typename T>
+struct OpSum {
+  const T* a;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[1]) const { acc[0] += (double)a[c] * V; }
+};
+
+// x += alpha d, r -= alpha q; obstacle rows zeroed (_cg_apply_alpha_kernel, fluid.py:4576-4587).
+template <typename TD, typename TR, typename TQ>
+struct OpCgUpdate {
+  double* x;
+  const TD* d;
+  TR* r;
+  const TQ* q;
+  const Scalars* s;
+  __device__ void operator()(uint32_t c, bool, uint8_t f) const {
+    if (f & kObst) {
+      x[c] = 0.0;
+      r[c] = (TR)0;
+      return;
+    }
+    const double a = s->alpha;
+    x[c] += a * (double)d[c];
+    r[c] = (TR)((double)r[c] - a * (double)q[c]);
+  }
+};
+
+// d = (z - mean_z) + beta d (_cg_update_direction_and_rz_kernel, fluid.py:4603-4611,
+// with _subtract_cg_weighted_mean_kernel folded in).  first: d = z - mean_z.
+template <typename TZ, typename TD>
+struct OpCgDir {
+  const TZ* z;
+  TD* d;
+  const Scalars* s;
+  int remove_mean, first;
+  __device__ void operator()(uint32_t c, bool free, uint8_t f) const {
+    double zv = (double)z[c];
+    if (remove_mean) zv = free ? zv - s->mz : 0.0;
+    if (f & kObst) {
+      d[c] = (TD)0;
+      return;
+    }
+    d[c] = first ? (TD)zv : (TD)(zv + s->beta * (double)d[c]);
+  }
+};
+
+// x -= mx, r -= mr on free cells, 0 elsewhere (fluid.py:4715-4726), with <r,r>.
+template <typename TR>
+struct OpSubMeans {
+  double* x;
+  TR* r;
+  const Scalars* s;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[1]) const {
+    double rv = (double)r[c] - s->mr;
+    if (x) x[c] = x[c] - s->mx;
+    TR rs = (TR)rv;
+    r[c] = rs;
+    acc[0] += (double)rs * (double)rs * V;
+  }
+};
+
+template <typename TR>
+struct OpZeroNonFree {
+  double* x;
+  TR* r;
+  __device__ void operator()(uint32_t c, bool free, uint8_t) const {
+    if (!free) {
+      if (x) x[c] = 0.0;
+      r[c] = (TR)0;
+    }
+  }
+};
+
+// b -= mean_b on non-obstacle rows, with <b,b> (fluid.py:4471-4482 + 6603).
+struct OpBFinal {
+  double* b;
+  const Scalars* s;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[1]) const {
+    double v = b[c] - s->mb;
+    b[c] = v;
+    acc[0] += v * v * V;
+  }
+};
+
+template <typename TS, typename TD>
+struct OpCopy {
+  const TS* a;
+  TD* b;
+  __device__ void operator()(uint32_t c, bool, uint8_t) const { b[c] = (TD)a[c]; }
+};
+
+template <typename T>
+struct OpFill {
+  T* a;
+  T v;
+  __device__ void operator()(uint32_t c, bool, uint8_t) const { a[c] = v; }
+};
+
+// ---------------------------------------------------------------------------
+// Mask / geometry kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_flags0(uint32_t n, const int32_t* obstacle, uint8_t* flags) {
+  HFSI_CELL_LOOP(n) {
+    int32_t v = obstacle[c];
+    flags[c] = (uint8_t)((v == 1 ? kObst : 0) | (v != 0 ? kNotFree : 0));
+  }
+}
+
+// Open-face bits from the neighbours' "not free" bit.
+__global__ void k_open_bits(uint32_t nx, uint32_t ny, uint32_t nz, uint8_t* flags) {
+  const uint32_t n = nx * ny * nz, sx = ny * nz, sy = nz;
+  HFSI_CELL_LOOP(n) {
+    uint32_t i, j, k;
+    decompose(c, ny, nz, i, j, k);
+    uint8_t f = flags[c] & (kObst | kNotFree);
+    if (i > 0 && !(flags[c - sx] & kNotFree)) f |= kXm;
+    if (i + 1 < nx && !(flags[c + sx] & kNotFree)) f |= kXp;
+    if (j > 0 && !(flags[c - sy] & kNotFree)) f |= kYm;
+    if (j + 1 < ny && !(flags[c + sy] & kNotFree)) f |= kYp;
+    if (k > 0 && !(flags[c - 1] & kNotFree)) f |= kZm;
+    if (k + 1 < nz && !(flags[c + 1] & kNotFree)) f |= kZp;
+    flags[c] = f;
+  }
+}
+
+// Level-0 velocity-Dirichlet data: active byte, raw weight, low-face weight.
+__global__ void k_vd0(uint32_t n, const int32_t* vda, const float* vdw, uint8_t* act, float* wraw, float* fw) {
+  HFSI_CELL_LOOP(n) {
+    bool a = vda && vda[c] != 0;
+    float w = (a && vdw) ? vdw[c] : 0.0f;
+    act[c] = a ? 1 : 0;
+    wraw[c] = w;
+    fw[c] = a ? fminf(fmaxf(w, 0.0f), 1.0f) : 1.0f;
+  }
+}
+
+// Coarse obstacle = any child obstacle row; coarse vd = any child active with
+// the max child weight; coarse interface diagonal = V-mean over children that
+// are not obstacle rows (_mg_restrict_residual_kernel, fluid.py:5334-5388).
+struct CoarseMaskArgs {
+  uint32_t fnx, fny, fnz, cnx, cny, cnz;
+  const uint8_t* fflags;
+  uint8_t* cflags;
+  const uint8_t* fact;
+  const float* fwraw;
+  uint8_t* cact;
+  float* cwraw;
+  float* cfw;
+  const double* fdiag;
+  double* cdiag;
+  const float *fwx, *fwy, *fwz;
+};
+
+__global__ void k_coarse_masks(CoarseMaskArgs a) {
+  const uint32_t n = a.cnx * a.cny * a.cnz;
+  const uint32_t fsx = a.fny * a.fnz, fsy = a.fnz;
+  HFSI_CELL_LOOP(n) {
+    uint32_t ci, cj, ck;
+    decompose(c, a.cny, a.cnz, ci, cj, ck);
+    int obst = 0, vdn = 0;
+    float vdw = 0.0f;
+    double dsum = 0.0, vsum = 0.0;
+    for (int oi = 0; oi < 2; ++oi)
+      for (int oj = 0; oj < 2; ++oj)
+        for (int ok = 0; ok < 2; ++ok) {
+          uint32_t fi = 2 * ci + oi, fj = 2 * cj + oj, fk = 2 * ck + ok;
+          if (fi < a.fnx && fj < a.fny && fk < a.fnz) {
+            uint32_t fc = fi * fsx + fj * fsy + fk;
+            if (a.fact && a.fact[fc]) {
+              ++vdn;
+              vdw = fmaxf(vdw, a.fwraw[fc]);
+            }
+            if (a.fflags[fc] & kObst) {
+              ++obst;
+            } else {
+              double vol = (double)(a.fwx[fi] * a.fwy[fj] * a.fwz[fk]);
+              if (a.fdiag) dsum += a.fdiag[fc] * vol;
+              vsum += vol;
+            }
+          }
+        }
+    a.cflags[c] = obst > 0 ? (uint8_t)(kObst | kNotFree) : (uint8_t)0;
+    if (a.cdiag) a.cdiag[c] = obst > 0 ? 0.0 : (double)(float)(dsum / fmax(vsum, 1.0e-30));
+    if (a.cact) {
+      a.cact[c] = vdn > 0 ? 1 : 0;
+      a.cwraw[c] = vdn > 0 ? vdw : 0.0f;
+      a.cfw[c] = vdn > 0 ? fminf(fmaxf(vdw, 0.0f), 1.0f) : 1.0f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity-side kernels
+// ---------------------------------------------------------------------------
+
+// (_compute_divergence_kernel, fluid.py:3450-3486), float32 arithmetic.
+__global__ void k_divergence(OpGeo<float> g, const float* u, float* div) {
+  HFSI_CELL_LOOP(g.n) {
+    uint32_t i, j, k;
+    decompose(c, g.ny, g.nz, i, j, k);
+    uint8_t f = g.flags[c];
+    float out = 0.0f;
+    if (!(f & kObst)) {
+      const float* uc = u + 3 * (size_t)c;
+      float lx = (f & kXm) ? uc[0] : 0.0f;
+      float rx = (f & kXp) ? u[3 * (size_t)(c + g.sx) + 0] : 0.0f;
+      float ly = (f & kYm) ? uc[1] : 0.0f;
+      float ry = (f & kYp) ? u[3 * (size_t)(c + g.sy) + 1] : 0.0f;
+      float lz = ((g.outlet && k == 0) || (f & kZm)) ? uc[2] : 0.0f;
+      float rz = (f & kZp) ? u[3 * (size_t)(c + 1) + 2] : 0.0f;
+      float dx = (rx - lx) / g.wf[0][i];
+      float dy = (ry - ly) / g.wf[1][j];
+      float dz = (rz - lz) / g.wf[2][k];
+      out = __fadd_rn(__fadd_rn(dx, dy), dz);
+    }
+    div[c] = out;
+  }
+}
+
+// rhs0 = f32(rhs_scale) * (div - source) in float32, minus the interface rhs in
+// float64; b_raw = -rhs0 on non-obstacle rows (fluid.py:5228-5243), with sum V b_raw.
+struct OpRhs {
+  const float* div;
+  const float* src;
+  const double* iface;
+  double* b;
+  float scale;
+  const uint8_t* flags;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[1]) const {
+    // only called for free cells by k_sums; obstacle rows set by OpRhsObst
+    float d = div[c] - (src ? src[c] : 0.0f);
+    double rhs0 = (double)__fmul_rn(scale, d);
+    if (iface) rhs0 -= iface[c];
+    double v = (flags[c] & kObst) ? 0.0 : -rhs0;
+    b[c] = v;
+    acc[0] += v * V;
+  }
+};
+struct OpRhsNonFree {
+  const float* div;
+  const float* src;
+  const double* iface;
+  double* b;
+  float scale;
+  __device__ void operator()(uint32_t c, bool free, uint8_t f) const {
+    if (free) return;
+    if (f & kObst) {
+      b[c] = 0.0;
+      return;
+    }
+    float d = div[c] - (src ? src[c] : 0.0f);
+    double rhs0 = (double)__fmul_rn(scale, d);
+    if (iface) rhs0 -= iface[c];
+    b[c] = -rhs0;
+  }
+};
+
+// Legacy-sign level-0 rhs for the standalone solvers (rhs0 itself).
+struct OpRhsLegacy {
+  const float* div;
+  const float* src;
+  const double* iface;
+  double* out;
+  float scale;
+  __device__ void operator()(uint32_t c, bool, uint8_t f) const {
+    if (f & kObst) {
+      out[c] = 0.0;
+      return;
+    }
+    float d = div[c] - (src ? src[c] : 0.0f);
+    double rhs0 = (double)__fmul_rn(scale, d);
+    if (iface) rhs0 -= iface[c];
+    out[c] = rhs0;
+  }
+};
+
+// (_subtract_pressure_gradient_kernel, fluid.py:5538-5582).
+struct GradArgs {
+  OpGeo<double> g;
+  float* u;
+  const double* p;
+  const float *dx, *dy, *dz;  // centre distances (float metrics)
+  const int32_t* vda;
+  const float* vdw;
+  const float* vdv;
+  float dt_over_rho;
+};
+
+__global__ void k_subtract_gradient(GradArgs a) {
+  const OpGeo<double>& g = a.g;
+  HFSI_CELL_LOOP(g.n) {
+    uint32_t i, j, k;
+    decompose(c, g.ny, g.nz, i, j, k);
+    uint8_t f = g.flags[c];
+    float* uc = a.u + 3 * (size_t)c;
+    if (f & kObst) {
+      uc[0] = 0.0f;
+      uc[1] = 0.0f;
+      uc[2] = 0.0f;
+      continue;
+    }
+    double pc = a.p[c];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (f & kXm) gx = (pc - a.p[c - g.sx]) / (double)a.dx[i];
+    if (f & kYm) gy = (pc - a.p[c - g.sy]) / (double)a.dy[j];
+    if (f & kZm)
+      gz = (pc - a.p[c - 1]) / (double)a.dz[k];
+    else if (g.outlet && k == 0)
+      gz = 2.0 * pc / (double)g.wf[2][k];
+    double w = 1.0;
+    if (a.vda && a.vda[c] != 0) {
+      w = fmin(fmax((double)a.vdw[c], 0.0), 1.0);
+      if (w <= 0.0 && a.vdv) {
+        uc[0] = a.vdv[3 * (size_t)c + 0];
+        uc[1] = a.vdv[3 * (size_t)c + 1];
+        uc[2] = a.vdv[3 * (size_t)c + 2];
+      }
+    }
+    if (w > 0.0) {
+      double s = w * (double)a.dt_over_rho;
+      uc[0] = uc[0] - (float)(s * gx);
+      uc[1] = uc[1] - (float)(s * gy);
+      uc[2] = uc[2] - (float)(s * gz);
+    }
+  }
+}
+
+// (_apply_closed_boundary_no_normal_flow_kernel, fluid.py:5528-5536).
+__global__ void k_wall_clamp(uint32_t nx, uint32_t ny, uint32_t nz, float* u, int outlet) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nyz = ny * nz, nxz = nx * nz, nxy = nx * ny;
+  if (tid < nyz) u[3 * (size_t)tid + 0] = 0.0f;  // i = 0
+  if (tid < nxz) {
+    uint32_t i = tid / nz, k = tid % nz;
+    u[3 * ((size_t)i * ny * nz + k) + 1] = 0.0f;  // j = 0
+  }
+  if (!outlet && tid < nxy) {
+    uint32_t i = tid / ny, j = tid % ny;
+    u[3 * (((size_t)i * ny + j) * nz) + 2] = 0.0f;  // k = 0
+  }
+}
+
+// Float64 true residual r = b - A x on non-obstacle rows (axpby of
+// fluid.py:4518-4531), stored at TR, with {<r,r>, sum V r} in float64.
+template <typename TR, typename Fin>
+__global__ void __launch_bounds__(kBlock) k_true_residual(OpGeo<double> g, const double* x, const double* b, TR* r,
+                                                           double* partials, unsigned int* counter, double* totals,
+                                                           Fin fin) {
+  __shared__ double lds[2 * (kBlock / 64)];
+  double acc[2] = {0.0, 0.0};
+  Ld<double, double> P{x};
+  HFSI_CELL_LOOP(g.n) {
+    uint32_t i, j, k;
+    decompose(c, g.ny, g.nz, i, j, k);
+    uint8_t f = g.flags[c];
+    double v = 0.0;
+    if (!(f & kObst)) v = b[c] - apply_cell(g, c, i, j, k, P);
+    r[c] = (TR)v;
+    if (!(f & kNotFree)) {
+      double V = volume64(g, i, j, k);
+      acc[0] += v * v * V;
+      acc[1] += v * V;
+    }
+  }
+  if (grid_sum<2>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+}
+
+// z = r / diag (_apply_jacobi_preconditioner_kernel, fluid.py:4502-4516).
+template <typename C, typename TR, typename TZ>
+__global__ void __launch_bounds__(kBlock) k_jacobi_precond(OpGeo<C> g, const TR* r, TZ* z) {
+  LdZero<C> P;
+  HFSI_CELL_LOOP(g.n) {
+    uint32_t i, j, k;
+    decompose(c, g.ny, g.nz, i, j, k);
+    uint8_t f = g.flags[c];
+    C v = (C)0;
+    if (!(f & kObst)) {
+      C nb, den;
+      nb_den(g, c, i, j, k, f, P, nb, den);
+      if (g.diag) den += (C)g.diag[c];
+      if (den > (C)0) v = (C)r[c] / den;
+    }
+    z[c] = (TZ)v;
+  }
+}
+
+// Compact uniform Jacobi (_pressure_jacobi_kernel, fluid.py:3996-4053).
+struct LegacyArgs {
+  uint32_t nx, ny, nz;
+  const double* p;
+  double* out;
+  const int32_t* obst;
+  const float* div;
+  const float* src;
+  float rhs_scale, ix2, iy2, iz2;
+  int outlet;
+};
+
+__global__ void k_legacy_jacobi(LegacyArgs a) {
+  const uint32_t n = a.nx * a.ny * a.nz, sx = a.ny * a.nz, sy = a.nz;
+  HFSI_CELL_LOOP(n) {
+    uint32_t i, j, k;
+    decompose(c, a.ny, a.nz, i, j, k);
+    uint32_t im = i > 0 ? i - 1 : 0, ip = i + 1 < a.nx ? i + 1 : a.nx - 1;
+    uint32_t jm = j > 0 ? j - 1 : 0, jp = j + 1 < a.ny ? j + 1 : a.ny - 1;
+    uint32_t km = k > 0 ? k - 1 : 0, kp = k + 1 < a.nz ? k + 1 : a.nz - 1;
+    const double* p = a.p;
+    const int32_t* o = a.obst;
+    uint32_t cxm = im * sx + j * sy + k, cxp = ip * sx + j * sy + k;
+    uint32_t cym = i * sx + jm * sy + k, cyp = i * sx + jp * sy + k;
+    uint32_t czm = i * sx + j * sy + km, czp = i * sx + j * sy + kp;
+    double ix2 = a.ix2, iy2 = a.iy2, iz2 = a.iz2;
+    double nb = ix2 * (p[cxm] + p[cxp]) + iy2 * (p[cym] + p[cyp]) + iz2 * (p[czm] + p[czp]);
+    double den = 2.0 * (double)(a.ix2 + a.iy2 + a.iz2);
+    bool adj = o[cxm] == 1 || o[cxp] == 1 || o[cym] == 1 || o[cyp] == 1 || o[czm] == 1 || o[czp] == 1;
+    if (adj || (a.outlet && k == 0)) {
+      nb = 0.0;
+      den = 0.0;
+      if (i > 0 && o[c - sx] == 0) { nb += ix2 * p[c - sx]; den += ix2; }
+      if (i + 1 < a.nx && o[c + sx] == 0) { nb += ix2 * p[c + sx]; den += ix2; }
+      if (j > 0 && o[c - sy] == 0) { nb += iy2 * p[c - sy]; den += iy2; }
+      if (j + 1 < a.ny && o[c + sy] == 0) { nb += iy2 * p[c + sy]; den += iy2; }
+      if (k > 0 && o[c - 1] == 0) { nb += iz2 * p[c - 1]; den += iz2; }
+      if (k + 1 < a.nz && o[c + 1] == 0) { nb += iz2 * p[c + 1]; den += iz2; }
+      if (a.outlet && k == 0) den += 2.0 * iz2;
+    }
+    float rhs = __fmul_rn(a.rhs_scale, a.div[c] - (a.src ? a.src[c] : 0.0f));
+    a.out[c] = (o[c] == 1 || den <= 0.0) ? 0.0 : (nb - (double)rhs) / den;
+  }
+}
+
+// Divergence report: 18 slots x {max, sum of squares, count}.
+struct ReportArgs {
+  OpGeo<float> g;
+  const float* div;
+  const float* src;
+  const double* idiv;
+  const int32_t* vda;
+  double* partials;
+  unsigned int* counter;
+  double* totals;
+};
+
+__global__ void __launch_bounds__(kBlock) k_div_report(ReportArgs a) {
+  constexpr int NS = 54;
+  __shared__ double lds[NS * (kBlock / 64)];
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  const OpGeo<float>& g = a.g;
+  auto add = [&](int slot, double v) {
+    acc[slot * 3 + 0] = fmax(acc[slot * 3 + 0], v);
+    acc[slot * 3 + 1] += v * v;
+    acc[slot * 3 + 2] += 1.0;
+  };
+  HFSI_CELL_LOOP(g.n) {
+    uint32_t i, j, k;
+    decompose(c, g.ny, g.nz, i, j, k);
+    uint8_t f = g.flags[c];
+    if (f & kNotFree) continue;
+    double total = (double)a.div[c] + (a.idiv ? a.idiv[c] : 0.0);
+    double raw = fabs(total);
+    double res = fabs(total - (a.src ? (double)a.src[c] : 0.0));
+    bool interior = i > 0 && i + 1 < g.nx && j > 0 && j + 1 < g.ny && k > 0 && k + 1 < g.nz;
+    add(0, raw);
+    add(1, res);
+    if (interior) {
+      add(2, raw);
+      add(3, res);
+    }
+    bool uses_vd = false, correctable = false;
+    if (a.vda) {
+      uses_vd = a.vda[c] != 0 || (i + 1 < g.nx && a.vda[c + g.sx] != 0) || (j + 1 < g.ny && a.vda[c + g.sy] != 0) ||
+                (k + 1 < g.nz && a.vda[c + 1] != 0);
+      bool own = a.vda[c] == 0;
+      correctable = ((f & kXm) && own) || ((f & kXp) && a.vda[c + g.sx] == 0) || ((f & kYm) && own) ||
+                    ((f & kYp) && a.vda[c + g.sy] == 0) || ((f & kZm) && own) || ((f & kZp) && a.vda[c + 1] == 0) ||
+                    (g.outlet && k == 0 && own);
+    } else {
+      correctable = (f & (kXm | kXp | kYm | kYp | kZm | kZp)) || (g.outlet && k == 0);
+    }
+    if (uses_vd) {
+      add(4, raw);
+      add(5, res);
+    } else {
+      add(6, raw);
+      add(7, res);
+    }
+    if (correctable) {
+      add(8, raw);
+      add(9, res);
+      if (interior) {
+        add(12, raw);
+        add(13, res);
+      }
+    } else {
+      add(10, raw);
+      add(11, res);
+      if (interior) {
+        add(14, raw);
+        add(15, res);
+      }
+    }
+  }
+  // workgroup reduction: max for slot*3, sums otherwise
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = (s % 3 == 0) ? wave_max(acc[s]) : wave_sum(acc[s]);
+  if (lane == 0)
+    for (int s = 0; s < NS; ++s) lds[s * nw + wave] = acc[s];
+  __syncthreads();
+  __shared__ bool is_last;
+  if (threadIdx.x == 0) {
+    for (int s = 0; s < NS; ++s) {
+      double v = 0.0;
+      for (int w = 0; w < nw; ++w) v = (s % 3 == 0) ? fmax(v, lds[s * nw + w]) : v + lds[s * nw + w];
+      a.partials[(size_t)blockIdx.x * NS + s] = v;
+    }
+    __threadfence();
+    is_last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  if (threadIdx.x < NS) {
+    int s = threadIdx.x;
+    double v = 0.0;
+    for (uint32_t b = 0; b < gridDim.x; ++b) {
+      double pv = __hip_atomic_load(&a.partials[(size_t)b * NS + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v = (s % 3 == 0) ? fmax(v, pv) : v + pv;
+    }
+    a.totals[s] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *a.counter = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup V-cycle tail for the small levels
+// ---------------------------------------------------------------------------
+
+template <typename C, typename TV>
+struct TailLevel {
+  OpGeo<C> g;
+  TV* p;
+  TV* t;
+  TV* rhs;
+};
+
+template <typename C, typename TV>
+struct TailArgs {
+  TailLevel<C, TV> lv[kMaxLevels];
+  int first, last;  // levels [first, last] run in the tail; last is the coarsest
+};
+
+template <typename C, typename P, typename R, typename TOut>
+__device__ inline void tail_sweep(const OpGeo<C>& g, const P& p, const R& rhs, TOut* out, C omega) {
+  for (uint32_t c = threadIdx.x; c < g.n; c += blockDim.x) {
+    uint32_t i, j, k;
+    decompose(c, g.ny, g.nz, i, j, k);
+    out[c] = (TOut)jacobi_cell(g, c, i, j, k, p, rhs, omega);
+  }
+  __syncthreads();
+}
+
+// Writes m = 1..W of a level go to P when (W - m) is even, else to T, so the
+// final smoothed value always ends in P.
+template <typename TV>
+__device__ __host__ inline TV* sweep_target(TV* P, TV* T, int W, int m) {
+  return ((W - m) % 2 == 0) ? P : T;
+}
+
+template <typename C, typename TV>
+__global__ void __launch_bounds__(kTailThreads) k_vcycle_tail(const TailArgs<C, TV>* args, int pre, int coarse,
+                                                              int post, C omega, C scale) {
+  const TailArgs<C, TV>& A = *args;
+  TV* cur[kMaxLevels];
+  // down
+  for (int l = A.first; l < A.last; ++l) {
+    const TailLevel<C, TV>& L = A.lv[l];
+    const int W = pre + post;
+    Ld<TV, C> R{L.rhs};
+    TV* out = sweep_target(L.p, L.t, W, 1);
+    tail_sweep(L.g, LdZero<C>{}, R, out, omega);
+    TV* in = out;
+    for (int m = 2; m <= pre; ++m) {
+      out = sweep_target(L.p, L.t, W, m);
+      tail_sweep(L.g, Ld<TV, C>{in}, R, out, omega);
+      in = out;
+    }
+    cur[l] = in;
+    const TailLevel<C, TV>& Cl = A.lv[l + 1];
+    for (uint32_t c = threadIdx.x; c < Cl.g.n; c += blockDim.x) {
+      uint32_t ci, cj, ck;
+      decompose(c, Cl.g.ny, Cl.g.nz, ci, cj, ck);
+      C v = (C)0;
+      if (!(Cl.g.flags[c] & kObst)) v = restrict_cell<TV>(L.g, ci, cj, ck, Ld<TV, C>{in}, R);
+      Cl.rhs[c] = (TV)v;
+    }
+    __syncthreads();
+  }
+  {  // coarsest
+    const TailLevel<C, TV>& L = A.lv[A.last];
+    Ld<TV, C> R{L.rhs};
+    TV* out = sweep_target(L.p, L.t, coarse, 1);
+    tail_sweep(L.g, LdZero<C>{}, R, out, omega);
+    TV* in = out;
+    for (int m = 2; m <= coarse; ++m) {
+      out = sweep_target(L.p, L.t, coarse, m);
+      tail_sweep(L.g, Ld<TV, C>{in}, R, out, omega);
+      in = out;
+    }
+    cur[A.last] = in;
+  }
+  // up
+  for (int l = A.last - 1; l >= A.first; --l) {
+    const TailLevel<C, TV>& L = A.lv[l];
+    const TailLevel<C, TV>& Cl = A.lv[l + 1];
+    const int W = pre + post;
+    Ld<TV, C> R{L.rhs};
+    LdProlong<TV, TV, C> PP{cur[l], cur[l + 1], L.g.flags, Cl.g.nx, Cl.g.ny, Cl.g.nz, Cl.g.sx, Cl.g.sy, scale};
+    TV* out = sweep_target(L.p, L.t, W, pre + 1);
+    tail_sweep(L.g, PP, R, out, omega);
+    TV* in = out;
+    for (int m = pre + 2; m <= W; ++m) {
+      out = sweep_target(L.p, L.t, W, m);
+      tail_sweep(L.g, Ld<TV, C>{in}, R, out, omega);
+      in = out;
+    }
+    cur[l] = in;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+
+struct LevelMem {
+  uint32_t nx, ny, nz, n;
+  uint8_t* flags = nullptr;
+  uint8_t* vact = nullptr;
+  float* vwraw = nullptr;
+  float* fw = nullptr;
+  double* diag = nullptr;
+  double* cl[3] = {};
+  double* ch[3] = {};
+  float* clf[3] = {};
+  float* chf[3] = {};
+  float* wf[3] = {};
+  double* wd[3] = {};
+  float* df[3] = {};  // centre distances (float)
+  double outlet_add = 0.0;
+  // coarse-level V-cycle storage (levels >= 1), float
+  float* p = nullptr;
+  float* t = nullptr;
+  float* rhs = nullptr;
+};
+
+}  // namespace hfsi
+
+struct hfsi_fluid {
+  int L = 0;
+  int tail_first = 0;
+  int prec_mask = 0;
+  bool uniform = true;
+  bool has_vd = false, has_diag = false;
+  hfsi::LevelMem lv[hfsi::kMaxLevels];
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  // workspaces
+  double* b = nullptr;     // float64 right-hand side
+  double* r64 = nullptr;   // REF vectors
+  double* d64 = nullptr;
+  double* q64 = nullptr;
+  double* z64 = nullptr;
+  double* t64 = nullptr;
+  float* r32 = nullptr;    // FP32 vectors
+  float* d32 = nullptr;
+  float* q32 = nullptr;
+  float* z32 = nullptr;
+  float* t32 = nullptr;
+  double* partials = nullptr;
+  unsigned int* counter = nullptr;
+  double* totals = nullptr;
+  hfsi::Scalars* scal = nullptr;
+  hfsi::Scalars* host_scal = nullptr;  // pinned
+  double* host_totals = nullptr;       // pinned
+  void* tail_args_d = nullptr;  // TailArgs<double,float>
+  void* tail_args_f = nullptr;  // TailArgs<float,float>
+  // live kernel timer: HIP events recorded on the launch stream around every
+  // launch of one selected kernel (HFSI_TIMER_*), summed at flush/read time
+  int timer_id = 0;
+  std::vector<hipEvent_t> timer_events;
+  int timer_used = 0;
+  double timer_ms = 0.0;
+  long timer_count = 0;
+};
+
+namespace hfsi {
+
+template <typename T>
+static int dalloc(hfsi_fluid* h, T** out, size_t count) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  HFSI_CHECK(hipMalloc(&p, bytes));
+  h->allocs.push_back(p);
+  h->bytes += bytes;
+  *out = (T*)p;
+  return HFSI_OK;
+}
+
+#define HFSI_TRY(expr)         \
+  do {                         \
+    int _rc = (expr);          \
+    if (_rc != HFSI_OK) return _rc; \
+  } while (0)
+
+template <typename C>
+static OpGeo<C> geo(const hfsi_fluid* h, int l, int outlet);
+
+template <>
+OpGeo<double> geo<double>(const hfsi_fluid* h, int l, int outlet) {
+  const LevelMem& m = h->lv[l];
+  OpGeo<double> g;
+  g.nx = m.nx; g.ny = m.ny; g.nz = m.nz; g.n = m.n; g.sx = m.ny * m.nz; g.sy = m.nz;
+  g.flags = m.flags;
+  g.fw = h->has_vd ? m.fw : nullptr;
+  g.diag = h->has_diag ? m.diag : nullptr;
+  for (int a = 0; a < 3; ++a) {
+    g.cl[a] = m.cl[a];
+    g.ch[a] = m.ch[a];
+    g.wf[a] = m.wf[a];
+    g.wd[a] = m.wd[a];
+  }
+  g.outlet_add = m.outlet_add;
+  g.outlet = outlet;
+  return g;
+}
+
+template <>
+OpGeo<float> geo<float>(const hfsi_fluid* h, int l, int outlet) {
+  const LevelMem& m = h->lv[l];
+  OpGeo<float> g;
+  g.nx = m.nx; g.ny = m.ny; g.nz = m.nz; g.n = m.n; g.sx = m.ny * m.nz; g.sy = m.nz;
+  g.flags = m.flags;
+  g.fw = h->has_vd ? m.fw : nullptr;
+  g.diag = h->has_diag ? m.diag : nullptr;
+  for (int a = 0; a < 3; ++a) {
+    g.cl[a] = m.clf[a];
+    g.ch[a] = m.chf[a];
+    g.wf[a] = m.wf[a];
+    g.wd[a] = m.wd[a];
+  }
+  g.outlet_add = (float)m.outlet_add;
+  g.outlet = outlet;
+  return g;
+}
+
+static inline uint32_t blocks_for(uint32_t n) { return grid_blocks(n, kMaxGrid); }
+
+template <typename Op>
+static int launch_pointwise(hfsi_fluid* h, int l, Op op, hipStream_t st) {
+  const LevelMem& m = h->lv[l];
+  hipLaunchKernelGGL(k_pointwise<Op>, dim3(blocks_for(m.n)), dim3(kBlock), 0, st, m.n, m.flags, op);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+template <int NS, typename C, typename Op, typename Fin>
+static int launch_sums(hfsi_fluid* h, const OpGeo<C>& g, Op op, Fin fin, hipStream_t st) {
+  hipLaunchKernelGGL((k_sums<NS, C, Op, Fin>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, op, h->partials,
+                     h->counter, h->totals, fin);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+static int sync_scalars(hfsi_fluid* h, hipStream_t st) {
+  HFSI_CHECK(hipMemcpyAsync(h->host_scal, h->scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  return HFSI_OK;
+}
+
+static void timer_flush(hfsi_fluid* h) {
+  if (h->timer_used == 0) return;
+  (void)hipEventSynchronize(h->timer_events[h->timer_used - 1]);
+  for (int i = 0; i + 1 < h->timer_used; i += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, h->timer_events[i], h->timer_events[i + 1]) == hipSuccess) {
+      h->timer_ms += ms;
+      h->timer_count += 1;
+    }
+  }
+  h->timer_used = 0;
+}
+
+static inline void timer_mark(hfsi_fluid* h, int id, hipStream_t st) {
+  if (h->timer_id != id || h->timer_events.empty()) return;
+  if (h->timer_used >= (int)h->timer_events.size()) timer_flush(h);
+  (void)hipEventRecord(h->timer_events[h->timer_used++], st);
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle driver
+// ---------------------------------------------------------------------------
+
+// Level-0 types: T0 storage of p/t at level 0, rhs0 loader R0.  Coarse levels
+// are float storage with compute type C.
+template <typename C, typename T0, typename R0>
+static int run_vcycle(hfsi_fluid* h, T0* P0, T0* T0buf, R0 rhs0, bool zero_init, int pre, int coarse, int post,
+                      int outlet, hipStream_t st) {
+  const int L = h->L;
+  const C omega = (C)0.8f;  // f32 kernel argument in the reference
+  const C scale = (C)(h->uniform ? 1.0f : 0.7f);
+  OpGeo<C> g0 = geo<C>(h, 0, outlet);
+  if (L <= 1) {
+    const int W = std::max(pre + post, coarse);
+    if (!zero_init && (W % 2)) {
+      set_error("level-0 smoothing count must be even for a warm start");
+      return HFSI_ERR_ARG;
+    }
+    T0* in = P0;
+    for (int m = 1; m <= W; ++m) {
+      T0* out = sweep_target(P0, T0buf, W, m);
+      if (m == 1 && zero_init)
+        hipLaunchKernelGGL((k_jacobi<C, LdZero<C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
+                           LdZero<C>{}, rhs0, out, omega);
+      else
+        hipLaunchKernelGGL((k_jacobi<C, Ld<T0, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
+                           Ld<T0, C>{in}, rhs0, out, omega);
+      in = out;
+    }
+    HFSI_LAUNCH_CHECK();
+    return HFSI_OK;
+  }
+  if (!zero_init && ((pre + post) % 2)) {
+    set_error("V-cycle smoothing count must be even for a warm start");
+    return HFSI_ERR_ARG;
+  }
+  const int W = pre + post;
+  const int tail = h->tail_first;  // >= 1
+  // ---- level 0 down
+  T0* in0 = P0;
+  for (int m = 1; m <= pre; ++m) {
+    T0* out = sweep_target(P0, T0buf, W, m);
+    if (m == 1 && zero_init)
+      hipLaunchKernelGGL((k_jacobi<C, LdZero<C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
+                         LdZero<C>{}, rhs0, out, omega);
+    else {
+      timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
+      hipLaunchKernelGGL((k_jacobi<C, Ld<T0, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
+                         Ld<T0, C>{in0}, rhs0, out, omega);
+      timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
+    }
+    in0 = out;
+  }
+  {
+    OpGeo<C> g1 = geo<C>(h, 1, outlet);
+    timer_mark(h, HFSI_TIMER_L0_RESTRICT, st);
+    hipLaunchKernelGGL((k_resid_restrict<T0, C, Ld<T0, C>, R0, float>), dim3(blocks_for(g1.n)), dim3(kBlock), 0, st,
+                       g0, Ld<T0, C>{in0}, rhs0, g1, h->lv[1].rhs);
+    timer_mark(h, HFSI_TIMER_L0_RESTRICT, st);
+  }
+  // ---- grid-launched coarse levels down
+  float* cur[kMaxLevels] = {};
+  for (int l = 1; l < tail; ++l) {
+    LevelMem& m = h->lv[l];
+    OpGeo<C> g = geo<C>(h, l, outlet);
+    Ld<float, C> R{m.rhs};
+    float* in = nullptr;
+    for (int s = 1; s <= pre; ++s) {
+      float* out = sweep_target(m.p, m.t, W, s);
+      if (s == 1)
+        hipLaunchKernelGGL((k_jacobi<C, LdZero<C>, Ld<float, C>, float>), dim3(blocks_for(g.n)), dim3(kBlock), 0,
+                           st, g, LdZero<C>{}, R, out, omega);
+      else
+        hipLaunchKernelGGL((k_jacobi<C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)), dim3(kBlock),
+                           0, st, g, Ld<float, C>{in}, R, out, omega);
+      in = out;
+    }
+    cur[l] = in;
+    OpGeo<C> gc = geo<C>(h, l + 1, outlet);
+    hipLaunchKernelGGL((k_resid_restrict<float, C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(gc.n)),
+                       dim3(kBlock), 0, st, g, Ld<float, C>{in}, R, gc, h->lv[l + 1].rhs);
+  }
+  // ---- tail: levels tail .. L-1 in one workgroup
+  {
+    const void* args = std::is_same<C, double>::value ? h->tail_args_d : h->tail_args_f;
+    hipLaunchKernelGGL((k_vcycle_tail<C, float>), dim3(1), dim3(kTailThreads), 0, st,
+                       (const TailArgs<C, float>*)args, pre, coarse, post, omega, scale);
+    cur[tail] = h->lv[tail].p;  // final smoothed value always lands in P
+  }
+  // ---- up through the grid-launched coarse levels
+  for (int l = tail - 1; l >= 1; --l) {
+    LevelMem& m = h->lv[l];
+    LevelMem& mc = h->lv[l + 1];
+    OpGeo<C> g = geo<C>(h, l, outlet);
+    Ld<float, C> R{m.rhs};
+    LdProlong<float, float, C> PP{cur[l], cur[l + 1], m.flags, mc.nx, mc.ny, mc.nz, mc.ny * mc.nz, mc.nz, scale};
+    float* out = sweep_target(m.p, m.t, W, pre + 1);
+    hipLaunchKernelGGL((k_jacobi<C, LdProlong<float, float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)),
+                       dim3(kBlock), 0, st, g, PP, R, out, omega);
+    float* in = out;
+    for (int s = pre + 2; s <= W; ++s) {
+      out = sweep_target(m.p, m.t, W, s);
+      hipLaunchKernelGGL((k_jacobi<C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)), dim3(kBlock), 0,
+                         st, g, Ld<float, C>{in}, R, out, omega);
+      in = out;
+    }
+    cur[l] = in;
+  }
+  // ---- level 0 up
+  {
+    LevelMem& mc = h->lv[1];
+    LdProlong<T0, float, C> PP{in0, cur[1], h->lv[0].flags, mc.nx, mc.ny, mc.nz, mc.ny * mc.nz, mc.nz, scale};
+    T0* out = sweep_target(P0, T0buf, W, pre + 1);
+    timer_mark(h, HFSI_TIMER_L0_PROLONG_SWEEP, st);
+    hipLaunchKernelGGL((k_jacobi<C, LdProlong<T0, float, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st,
+                       g0, PP, rhs0, out, omega);
+    timer_mark(h, HFSI_TIMER_L0_PROLONG_SWEEP, st);
+    T0* in = out;
+    for (int s = pre + 2; s <= W; ++s) {
+      out = sweep_target(P0, T0buf, W, s);
+      timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
+      hipLaunchKernelGGL((k_jacobi<C, Ld<T0, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
+                         Ld<T0, C>{in}, rhs0, out, omega);
+      timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
+      in = out;
+    }
+  }
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PCG driver
+// ---------------------------------------------------------------------------
+
+template <typename C, typename TV>
+struct Vecs {
+  TV *r, *d, *q, *z, *t;
+};
+
+template <typename C, typename TV>
+static Vecs<C, TV> vecs(hfsi_fluid* h);
+template <>
+Vecs<double, double> vecs<double, double>(hfsi_fluid* h) {
+  return {h->r64, h->d64, h->q64, h->z64, h->t64};
+}
+template <>
+Vecs<float, float> vecs<float, float>(hfsi_fluid* h) {
+  return {h->r32, h->d32, h->q32, h->z32, h->t32};
+}
+
+template <typename C, typename TV>
+static int precondition(hfsi_fluid* h, bool use_mg, int pre, int coarse, int post, int outlet, hipStream_t st) {
+  Vecs<C, TV> v = vecs<C, TV>(h);
+  if (use_mg)
+    return run_vcycle<C, TV, LdNeg<TV, C>>(h, v.z, v.t, LdNeg<TV, C>{v.r}, true, pre, coarse, post, outlet, st);
+  OpGeo<C> g = geo<C>(h, 0, outlet);
+  hipLaunchKernelGGL((k_jacobi_precond<C, TV, TV>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.r, v.z);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+template <typename C, typename TV>
+static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const double* iface, double* x,
+                    const hfsi_cg_options& o, hfsi_cg_stats& S, hipStream_t st) {
+  Vecs<C, TV> v = vecs<C, TV>(h);
+  const int outlet = o.outlet_zmin ? 1 : 0;
+  const int rm = o.remove_mean ? 1 : 0;
+  OpGeo<C> g = geo<C>(h, 0, outlet);
+  OpGeo<double> g64 = geo<double>(h, 0, outlet);
+  Scalars* sc = h->scal;
+  const bool exact_checks = o.true_residual != 0;
+  const int check_every = o.check_interval > 0 ? o.check_interval : 16;
+  const int restart_every = o.restart_interval;
+  const int max_it = std::max(1, o.max_iterations);
+  const double tol = std::max(0.0, o.tolerance);
+  std::memset(&S, 0, sizeof(S));
+  S.initial_relative_residual = INFINITY;
+  S.relative_residual = INFINITY;
+  S.true_relative_residual = -1.0;
+
+  // reset device scalars (keep the free volume of the current masks)
+  {
+    Scalars init{};
+    HFSI_CHECK(hipMemcpyAsync(h->host_scal, h->scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    HFSI_CHECK(hipStreamSynchronize(st));
+    init.free_vol = h->host_scal->free_vol;
+    *h->host_scal = init;
+    HFSI_CHECK(hipMemcpyAsync(h->scal, h->host_scal, sizeof(Scalars), hipMemcpyHostToDevice, st));
+  }
+  // ---- right-hand side b = -(rhs0 - mean)
+  const float scale = (float)o.rhs_scale;
+  HFSI_TRY(launch_pointwise(h, 0, OpRhsNonFree{div, src, iface, h->b, scale}, st));
+  HFSI_TRY(launch_sums<1>(h, g64, OpRhs{div, src, iface, h->b, scale, h->lv[0].flags}, FinBMean{sc, rm}, st));
+  if (rm) S.mean_projection_count += 1;
+  HFSI_TRY(launch_sums<1>(h, g64, OpBFinal{h->b, sc}, FinBNorm{sc}, st));
+  HFSI_TRY(sync_scalars(h, st));
+  const double b_norm = std::sqrt(std::max(h->host_scal->bnorm2, 0.0));
+  S.b_norm = b_norm;
+  if (!(b_norm > 1.0e-30)) {
+    HFSI_TRY(launch_pointwise(h, 0, OpFill<double>{x, 0.0}, st));
+    S.initial_relative_residual = 0.0;
+    S.relative_residual = 0.0;
+    S.true_relative_residual = 0.0;
+    S.converged = 1;
+    return HFSI_OK;
+  }
+
+  auto true_residual = [&](bool count_projection) -> int {
+    hipLaunchKernelGGL((k_true_residual<TV, FinMeanR>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, h->b,
+                       v.r, h->partials, h->counter, h->totals, FinMeanR{sc, rm});
+    HFSI_LAUNCH_CHECK();
+    if (rm) {
+      HFSI_TRY(launch_sums<1>(h, g64, OpSubMeans<TV>{nullptr, v.r, sc}, FinRR{sc}, st));
+      HFSI_TRY(launch_pointwise(h, 0, OpZeroNonFree<TV>{nullptr, v.r}, st));
+      if (count_projection) S.mean_projection_count += 1;
+    }
+    return HFSI_OK;
+  };
+
+  // ---- initial residual
+  HFSI_TRY(true_residual(true));
+  HFSI_TRY(sync_scalars(h, st));
+  S.host_residual_checks += 1;
+  double rel = std::sqrt(std::max(h->host_scal->rr, 0.0)) / b_norm;
+  S.initial_relative_residual = rel;
+  S.relative_residual = rel;
+  if (rel * b_norm <= tol * b_norm) {
+    S.converged = 1;
+    return HFSI_OK;
+  }
+  bool use_mg = h->L > 1 && !o.interface_active &&
+                (o.preconditioner == HFSI_PRECOND_MG || o.preconditioner == HFSI_PRECOND_MG_LIGHT ||
+                 (o.preconditioner == HFSI_PRECOND_AUTO && !h->uniform));
+  const bool light = o.preconditioner == HFSI_PRECOND_MG_LIGHT;
+  const int pre = light ? 1 : 2, coarse = light ? 12 : 24, post = light ? 1 : 2;
+
+  auto precond_and_rz = [&](bool project) -> int {
+    HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st)));
+    if (use_mg) S.vcycles += 1;
+    HFSI_TRY(launch_sums<3>(h, g64, OpDots3<TV, TV>{v.r, v.z}, FinDots3{sc, project ? rm : 0}, st));
+    if (project && rm) S.mean_projection_count += 1;
+    return HFSI_OK;
+  };
+
+  HFSI_TRY(precond_and_rz(true));
+  HFSI_TRY(sync_scalars(h, st));
+  double rz = h->host_scal->rz;
+  if (!std::isfinite(rz) || std::fabs(rz) <= 1.0e-300) {
+    use_mg = false;
+    HFSI_TRY(precond_and_rz(false));
+    HFSI_TRY(sync_scalars(h, st));
+    rz = h->host_scal->rz;
+  }
+  S.used_multigrid = use_mg ? 1 : 0;
+  if (!std::isfinite(rz) || rz <= 0.0) {
+    S.breakdown_kind = HFSI_BREAKDOWN_INITIAL_DOT;
+    return HFSI_OK;
+  }
+  HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 1}, st));
+
+  int it = 0;
+  while (it < max_it) {
+    ++it;
+    timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
+    hipLaunchKernelGGL((k_spmv_dot<C, TV, TV, FinDad>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.d, v.q,
+                       h->partials, h->counter, h->totals, FinDad{sc});
+    timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
+    HFSI_LAUNCH_CHECK();
+    HFSI_TRY(launch_pointwise(h, 0, OpCgUpdate<TV, TV, TV>{x, v.d, v.r, v.q, sc}, st));
+    const bool project = rm && (it % 16 == 0);
+    const bool check = (it % check_every == 0) || it == max_it;
+    if (project) {
+      HFSI_TRY(launch_sums<2>(h, g64, OpSumsXR<TV>{x, v.r}, FinMeansXR{sc}, st));
+      HFSI_TRY(launch_sums<1>(h, g64, OpSubMeans<TV>{x, v.r, sc}, FinRR{sc}, st));
+      HFSI_TRY(launch_pointwise(h, 0, OpZeroNonFree<TV>{x, v.r}, st));
+      S.mean_projection_count += 2;
+    }
+    S.iterations = it;
+    if (check) {
+      if (exact_checks) {
+        HFSI_TRY(true_residual(true));
+        S.true_residual_replacements += 1;
+      } else if (!project) {
+        HFSI_TRY(launch_sums<1>(h, g64, OpDot<TV, TV>{v.r, v.r}, FinRR{sc}, st));
+      }
+      HFSI_TRY(sync_scalars(h, st));
+      S.host_residual_checks += 1;
+      rel = std::sqrt(std::max(h->host_scal->rr, 0.0)) / b_norm;
+      S.relative_residual = rel;
+      if (rel <= tol) {
+        S.converged = 1;
+        break;
+      }
+      const int code = h->host_scal->code;
+      bool restart = false;
+      if (code != 0) {
+        S.breakdown_dAd = h->host_scal->code_dAd;
+        if (it >= max_it) {
+          S.breakdown_kind = HFSI_BREAKDOWN_DEVICE_SCALAR;
+          break;
+        }
+        restart = true;
+      } else if (it < max_it && restart_every > 0 && it % restart_every == 0) {
+        restart = true;
+      }
+      if (restart) {
+        HFSI_TRY(true_residual(true));
+        HFSI_TRY(sync_scalars(h, st));
+        S.host_residual_checks += 1;
+        rel = std::sqrt(std::max(h->host_scal->rr, 0.0)) / b_norm;
+        S.relative_residual = rel;
+        if (rel <= tol) {
+          S.converged = 1;
+          break;
+        }
+        HFSI_TRY(precond_and_rz(true));
+        HFSI_TRY(sync_scalars(h, st));
+        rz = h->host_scal->rz;
+        if (std::isfinite(rz) && rz > 0.0) {
+          HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 1}, st));
+          Scalars fix = *h->host_scal;
+          fix.code = 0;
+          fix.code_dAd = 0.0;
+          *h->host_scal = fix;
+          HFSI_CHECK(hipMemcpyAsync(h->scal, h->host_scal, sizeof(Scalars), hipMemcpyHostToDevice, st));
+          S.restart_count += 1;
+          continue;
+        }
+        S.breakdown_kind = code != 0 ? HFSI_BREAKDOWN_DEVICE_SCALAR : HFSI_BREAKDOWN_RESTART_DOT;
+        break;
+      }
+    }
+    HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st)));
+    if (use_mg) S.vcycles += 1;
+    if (rm) S.mean_projection_count += 1;
+    HFSI_TRY(launch_sums<5>(h, g64, OpDots5<TV, TV, TV>{v.r, v.z, v.q}, FinDots5{sc, rm, use_mg ? 1 : 0}, st));
+    HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 0}, st));
+  }
+  HFSI_TRY(sync_scalars(h, st));
+  S.breakdown_code = h->host_scal->code;
+  if (S.breakdown_code != 0 && S.breakdown_dAd == 0.0) S.breakdown_dAd = h->host_scal->code_dAd;
+  if (o.true_residual_at_exit) {
+    // float64 certificate ||b - A x||_V / ||b||_V (r is overwritten: the solve is over)
+    HFSI_TRY(true_residual(false));
+    HFSI_TRY(sync_scalars(h, st));
+    double rr = h->host_scal->rr;
+    S.true_relative_residual = std::sqrt(std::max(rr, 0.0)) / b_norm;
+  }
+  return HFSI_OK;
+}
+
+template <typename C>
+static int write_tail_args(hfsi_fluid* h, void* dst, hipStream_t st) {
+  TailArgs<C, float> A{};
+  A.first = h->tail_first;
+  A.last = h->L - 1;
+  for (int l = 1; l < h->L; ++l) {
+    A.lv[l].g = geo<C>(h, l, 0);
+    A.lv[l].p = h->lv[l].p;
+    A.lv[l].t = h->lv[l].t;
+    A.lv[l].rhs = h->lv[l].rhs;
+  }
+  // the outlet flag is per solve: the tail reads outlet from lv[].g.outlet,
+  // written below for both values by the solve entry (see set_tail_outlet)
+  HFSI_CHECK(hipMemcpyAsync(dst, &A, sizeof(A), hipMemcpyHostToDevice, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  return HFSI_OK;
+}
+
+// The tail geometry carries the outlet switch; refresh it when it changes.
+static int set_tail_outlet(hfsi_fluid* h, int outlet, hipStream_t st) {
+  using TAd = TailArgs<double, float>;
+  using TLd = TailLevel<double, float>;
+  using TAf = TailArgs<float, float>;
+  using TLf = TailLevel<float, float>;
+  using OGd = OpGeo<double>;
+  using OGf = OpGeo<float>;
+  for (int l = 1; l < h->L; ++l) {
+    size_t off_d = offsetof(TAd, lv) + l * sizeof(TLd) + offsetof(TLd, g) + offsetof(OGd, outlet);
+    size_t off_f = offsetof(TAf, lv) + l * sizeof(TLf) + offsetof(TLf, g) + offsetof(OGf, outlet);
+    HFSI_CHECK(hipMemcpyAsync((char*)h->tail_args_d + off_d, &outlet, sizeof(int), hipMemcpyHostToDevice, st));
+    HFSI_CHECK(hipMemcpyAsync((char*)h->tail_args_f + off_f, &outlet, sizeof(int), hipMemcpyHostToDevice, st));
+  }
+  HFSI_CHECK(hipStreamSynchronize(st));
+  return HFSI_OK;
+}
+
+}  // namespace hfsi
+
+using namespace hfsi;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int hfsi_version(void) { return 1; }
+
+const char* hfsi_last_error(void) { return g_last_error.c_str(); }
+
+int hfsi_device_ok(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+static void axis_metrics(const std::vector<double>& w, double origin, std::vector<float>& wf,
+                         std::vector<float>& df) {
+  // centres by running faces (fluid.py:66-87); float32 storage like the reference's fields
+  const size_t n = w.size();
+  std::vector<double> centers(n);
+  double face = origin;
+  for (size_t i = 0; i < n; ++i) {
+    centers[i] = face + 0.5 * w[i];
+    face += w[i];
+  }
+  wf.resize(n);
+  df.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    wf[i] = (float)w[i];
+    df[i] = (float)(i == 0 ? w[0] : centers[i] - centers[i - 1]);
+  }
+}
+
+int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, const double* widths_x,
+                      const double* widths_y, const double* widths_z, const double* origin3, int32_t precision_mask,
+                      void* stream) {
+  if (!out || nx < 1 || ny < 1 || nz < 1 || !widths_x || !widths_y || !widths_z) {
+    set_error("hfsi_fluid_create: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  if ((uint64_t)nx * ny * nz >= (1ull << 31)) {
+    set_error("hfsi_fluid_create: grid too large for 32-bit cell indices");
+    return HFSI_ERR_ARG;
+  }
+  if (!(precision_mask & (HFSI_PREC_REF64 | HFSI_PREC_FP32))) {
+    set_error("hfsi_fluid_create: precision_mask must include HFSI_PREC_REF64 or HFSI_PREC_FP32");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hfsi_fluid* h = new hfsi_fluid();
+  h->prec_mask = precision_mask;
+  double origin[3] = {0.0, 0.0, 0.0};
+  if (origin3) std::memcpy(origin, origin3, sizeof(origin));
+  // level shapes and widths (fluid.py:692-729)
+  std::vector<std::vector<double>> W[kMaxLevels];
+  std::vector<double> cur[3] = {std::vector<double>(widths_x, widths_x + nx),
+                                std::vector<double>(widths_y, widths_y + ny),
+                                std::vector<double>(widths_z, widths_z + nz)};
+  // uniform test of the level-0 grid (fluid.py:150-162)
+  h->uniform = true;
+  for (int a = 0; a < 3; ++a) {
+    double w0 = cur[a][0];
+    double tolw = std::max(std::fabs(w0), 1.0) * 1.0e-12;
+    for (double v : cur[a])
+      if (std::fabs(v - w0) > tolw) h->uniform = false;
+  }
+  int L = 0;
+  while (true) {
+    W[L] = {cur[0], cur[1], cur[2]};
+    ++L;
+    size_t mn = std::min(cur[0].size(), std::min(cur[1].size(), cur[2].size()));
+    if (mn <= 3 || L >= kMaxLevels) break;
+    std::vector<double> nxt[3];
+    bool same = true;
+    for (int a = 0; a < 3; ++a) {
+      size_t n = cur[a].size();
+      size_t cn = std::max<size_t>(2, (n + 1) / 2);
+      if (cn != n) same = false;
+      for (size_t i = 0; i < n; i += 2) nxt[a].push_back(i + 1 < n ? cur[a][i] + cur[a][i + 1] : cur[a][i]);
+      // coarse count follows max(2, (n+1)//2); pairwise sums give ceil(n/2) which equals it for n > 3
+    }
+    if (same) break;
+    for (int a = 0; a < 3; ++a) cur[a] = nxt[a];
+  }
+  h->L = L;
+  int rc = HFSI_OK;
+  for (int l = 0; l < L && rc == HFSI_OK; ++l) {
+    LevelMem& m = h->lv[l];
+    m.nx = (uint32_t)W[l][0].size();
+    m.ny = (uint32_t)W[l][1].size();
+    m.nz = (uint32_t)W[l][2].size();
+    m.n = m.nx * m.ny * m.nz;
+    rc |= dalloc(h, &m.flags, m.n);
+    rc |= dalloc(h, &m.vact, m.n);
+    rc |= dalloc(h, &m.vwraw, m.n);
+    rc |= dalloc(h, &m.fw, m.n);
+    rc |= dalloc(h, &m.diag, m.n);
+    for (int a = 0; a < 3; ++a) {
+      std::vector<float> wf, df;
+      axis_metrics(W[l][a], origin[a], wf, df);
+      const size_t n = wf.size();
+      std::vector<double> cl(n, 0.0), ch(n, 0.0), wd(n);
+      std::vector<float> clf(n), chf(n);
+      for (size_t i = 0; i < n; ++i) {
+        wd[i] = (double)wf[i];
+        if (i > 0) cl[i] = 1.0 / ((double)wf[i] * (double)df[i]);
+        if (i + 1 < n) ch[i] = 1.0 / ((double)wf[i] * (double)df[i + 1]);
+        clf[i] = (float)cl[i];
+        chf[i] = (float)ch[i];
+      }
+      if (a == 2) m.outlet_add = 2.0 / ((double)wf[0] * (double)wf[0]);
+      rc |= dalloc(h, &m.cl[a], n);
+      rc |= dalloc(h, &m.ch[a], n);
+      rc |= dalloc(h, &m.clf[a], n);
+      rc |= dalloc(h, &m.chf[a], n);
+      rc |= dalloc(h, &m.wf[a], n);
+      rc |= dalloc(h, &m.wd[a], n);
+      rc |= dalloc(h, &m.df[a], n);
+      if (rc != HFSI_OK) break;
+      (void)hipMemcpy(m.cl[a], cl.data(), n * sizeof(double), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.ch[a], ch.data(), n * sizeof(double), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.clf[a], clf.data(), n * sizeof(float), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.chf[a], chf.data(), n * sizeof(float), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.wf[a], wf.data(), n * sizeof(float), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.wd[a], wd.data(), n * sizeof(double), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.df[a], df.data(), n * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (l >= 1 && rc == HFSI_OK) {
+      rc |= dalloc(h, &m.p, m.n);
+      rc |= dalloc(h, &m.t, m.n);
+      rc |= dalloc(h, &m.rhs, m.n);
+    }
+  }
+  // tail start: first level (>= 1) whose size is at most kTailCells; the
+  // coarsest level always runs in the tail
+  h->tail_first = L > 1 ? L - 1 : 1;
+  for (int l = 1; l < L; ++l)
+    if (h->lv[l].n <= kTailCells) {
+      h->tail_first = l;
+      break;
+    }
+  const size_t n0 = h->lv[0].n;
+  if (rc == HFSI_OK) {
+    rc |= dalloc(h, &h->b, n0);
+    rc |= dalloc(h, &h->t64, n0);  // level-0 smoother scratch of the standalone solvers
+    if (precision_mask & HFSI_PREC_REF64) {
+      rc |= dalloc(h, &h->r64, n0);
+      rc |= dalloc(h, &h->d64, n0);
+      rc |= dalloc(h, &h->q64, n0);
+      rc |= dalloc(h, &h->z64, n0);
+    }
+    if (precision_mask & HFSI_PREC_FP32) {
+      rc |= dalloc(h, &h->r32, n0);
+      rc |= dalloc(h, &h->d32, n0);
+      rc |= dalloc(h, &h->q32, n0);
+      rc |= dalloc(h, &h->z32, n0);
+      rc |= dalloc(h, &h->t32, n0);
+    }
+    rc |= dalloc(h, &h->partials, (size_t)kMaxPartials * 64);
+    rc |= dalloc(h, &h->counter, 1);
+    rc |= dalloc(h, &h->totals, 64);
+    rc |= dalloc(h, &h->scal, 1);
+  }
+  if (rc == HFSI_OK) {
+    if (hipHostMalloc((void**)&h->host_scal, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h->host_totals, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      set_error("hipHostMalloc failed");
+      rc = HFSI_ERR_HIP;
+    }
+  }
+  if (rc == HFSI_OK) {
+    (void)hipMemset(h->counter, 0, sizeof(unsigned int));
+    (void)hipMemset(h->scal, 0, sizeof(Scalars));
+    // tail argument blocks (geometry pointers are fixed; the has_vd / has_diag
+    // switches are refreshed by hfsi_fluid_set_masks)
+    rc |= dalloc(h, (char**)&h->tail_args_d, sizeof(TailArgs<double, float>));
+    rc |= dalloc(h, (char**)&h->tail_args_f, sizeof(TailArgs<float, float>));
+  }
+  if (rc == HFSI_OK) {
+    // Valid masks from the start (every cell free), so that no kernel can see
+    // uninitialised open-face bits before the first hfsi_fluid_set_masks.
+    for (int l = 0; l < L; ++l) {
+      LevelMem& m = h->lv[l];
+      (void)hipMemsetAsync(m.flags, 0, m.n, st);
+      hipLaunchKernelGGL(k_open_bits, dim3(blocks_for(m.n)), dim3(kBlock), 0, st, m.nx, m.ny, m.nz, m.flags);
+    }
+    if (hipGetLastError() != hipSuccess) rc = HFSI_ERR_HIP;
+    if (rc == HFSI_OK) rc = write_tail_args<double>(h, h->tail_args_d, st);
+    if (rc == HFSI_OK) rc = write_tail_args<float>(h, h->tail_args_f, st);
+  }
+  if (rc != HFSI_OK) {
+    hfsi_fluid_destroy(h);
+    return rc;
+  }
+  HFSI_CHECK(hipStreamSynchronize(st));
+  *out = h;
+  return HFSI_OK;
+}
+
+int hfsi_fluid_destroy(hfsi_fluid* h) {
+  if (!h) return HFSI_OK;
+  (void)hipDeviceSynchronize();
+  for (hipEvent_t e : h->timer_events) (void)hipEventDestroy(e);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->host_scal) (void)hipHostFree(h->host_scal);
+  if (h->host_totals) (void)hipHostFree(h->host_totals);
+  delete h;
+  return HFSI_OK;
+}
+
+int32_t hfsi_fluid_num_levels(const hfsi_fluid* h) { return h ? h->L : 0; }
+
+int hfsi_fluid_level_shape(const hfsi_fluid* h, int32_t level, int32_t* shape3) {
+  if (!h || !shape3 || level < 0 || level >= h->L) {
+    set_error("hfsi_fluid_level_shape: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  shape3[0] = (int32_t)h->lv[level].nx;
+  shape3[1] = (int32_t)h->lv[level].ny;
+  shape3[2] = (int32_t)h->lv[level].nz;
+  return HFSI_OK;
+}
+
+size_t hfsi_fluid_device_bytes(const hfsi_fluid* h) { return h ? h->bytes : 0; }
+
+int hfsi_fluid_set_masks(hfsi_fluid* h, const int32_t* obstacle, const int32_t* vd_active, const float* vd_weight,
+                         const double* iface_diag, void* stream) {
+  if (!h || !obstacle) {
+    set_error("hfsi_fluid_set_masks: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  h->has_vd = vd_active != nullptr;
+  h->has_diag = iface_diag != nullptr;
+  LevelMem& m0 = h->lv[0];
+  hipLaunchKernelGGL(k_flags0, dim3(blocks_for(m0.n)), dim3(kBlock), 0, st, m0.n, obstacle, m0.flags);
+  hipLaunchKernelGGL(k_open_bits, dim3(blocks_for(m0.n)), dim3(kBlock), 0, st, m0.nx, m0.ny, m0.nz, m0.flags);
+  if (h->has_vd)
+    hipLaunchKernelGGL(k_vd0, dim3(blocks_for(m0.n)), dim3(kBlock), 0, st, m0.n, vd_active, vd_weight, m0.vact,
+                       m0.vwraw, m0.fw);
+  if (h->has_diag) HFSI_CHECK(hipMemcpyAsync(m0.diag, iface_diag, m0.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  for (int l = 1; l < h->L; ++l) {
+    LevelMem& f = h->lv[l - 1];
+    LevelMem& c = h->lv[l];
+    CoarseMaskArgs a;
+    a.fnx = f.nx; a.fny = f.ny; a.fnz = f.nz;
+    a.cnx = c.nx; a.cny = c.ny; a.cnz = c.nz;
+    a.fflags = f.flags;
+    a.cflags = c.flags;
+    a.fact = h->has_vd ? f.vact : nullptr;
+    a.fwraw = f.vwraw;
+    a.cact = h->has_vd ? c.vact : nullptr;
+    a.cwraw = c.vwraw;
+    a.cfw = c.fw;
+    a.fdiag = h->has_diag ? f.diag : nullptr;
+    a.cdiag = h->has_diag ? c.diag : nullptr;
+    a.fwx = f.wf[0]; a.fwy = f.wf[1]; a.fwz = f.wf[2];
+    hipLaunchKernelGGL(k_coarse_masks, dim3(blocks_for(c.n)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_open_bits, dim3(blocks_for(c.n)), dim3(kBlock), 0, st, c.nx, c.ny, c.nz, c.flags);
+  }
+  HFSI_LAUNCH_CHECK();
+  // free volume (V over free cells) into the device scalars
+  OpGeo<double> g = geo<double>(h, 0, 0);
+  HFSI_TRY(launch_sums<1>(h, g, OpVolume{}, FinFreeVol{h->scal}, st));
+  HFSI_TRY(write_tail_args<double>(h, h->tail_args_d, st));
+  HFSI_TRY(write_tail_args<float>(h, h->tail_args_f, st));
+  return HFSI_OK;
+}
+
+int hfsi_fluid_divergence(hfsi_fluid* h, const float* velocity, float* divergence, int32_t outlet_zmin,
+                          void* stream) {
+  if (!h || !velocity || !divergence) {
+    set_error("hfsi_fluid_divergence: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  OpGeo<float> g = geo<float>(h, 0, outlet_zmin ? 1 : 0);
+  hipLaunchKernelGGL(k_divergence, dim3(blocks_for(g.n)), dim3(kBlock), 0, (hipStream_t)stream, g, velocity,
+                     divergence);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+int hfsi_fluid_apply(hfsi_fluid* h, const double* p, double* q, int32_t outlet_zmin, void* stream) {
+  if (!h || !p || !q) {
+    set_error("hfsi_fluid_apply: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  OpGeo<double> g = geo<double>(h, 0, outlet_zmin ? 1 : 0);
+  hipLaunchKernelGGL((k_apply<double, double, double>), dim3(blocks_for(g.n)), dim3(kBlock), 0, (hipStream_t)stream,
+                     g, p, q);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+int hfsi_fluid_diagonal(hfsi_fluid* h, double* diag, int32_t outlet_zmin, void* stream) {
+  if (!h || !diag) {
+    set_error("hfsi_fluid_diagonal: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  OpGeo<double> g = geo<double>(h, 0, outlet_zmin ? 1 : 0);
+  hipLaunchKernelGGL(k_diagonal<double>, dim3(blocks_for(g.n)), dim3(kBlock), 0, (hipStream_t)stream, g, diag);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+int hfsi_fluid_weighted_dot(hfsi_fluid* h, const double* a, const double* b, double* out, void* stream) {
+  if (!h || !a || !b || !out) {
+    set_error("hfsi_fluid_weighted_dot: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  OpGeo<double> g = geo<double>(h, 0, 0);
+  HFSI_TRY(launch_sums<1>(h, g, OpDot<double, double>{a, b}, FinNone{}, st));
+  HFSI_CHECK(hipMemcpyAsync(h->host_totals, h->totals, sizeof(double), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  *out = h->host_totals[0];
+  return HFSI_OK;
+}
+
+int hfsi_fluid_vcycle(hfsi_fluid* h, const double* r, double* z, int32_t precision, int32_t pre, int32_t coarse,
+                      int32_t post, int32_t outlet_zmin, void* stream) {
+  if (!h || !r || !z || pre < 1 || coarse < 1 || post < 1) {
+    set_error("hfsi_fluid_vcycle: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  if (!(h->prec_mask & precision)) {
+    set_error("hfsi_fluid_vcycle: precision not enabled on this handle");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int outlet = outlet_zmin ? 1 : 0;
+  HFSI_TRY(set_tail_outlet(h, outlet, st));
+  if (precision == HFSI_PREC_REF64) {
+    HFSI_TRY((run_vcycle<double, double, LdNeg<double, double>>(h, z, h->t64, LdNeg<double, double>{r}, true, pre,
+                                                                 coarse, post, outlet, st)));
+  } else if (precision == HFSI_PREC_FP32) {
+    HFSI_TRY(launch_pointwise(h, 0, OpCopy<double, float>{r, h->r32}, st));
+    HFSI_TRY((run_vcycle<float, float, LdNeg<float, float>>(h, h->z32, h->t32, LdNeg<float, float>{h->r32}, true,
+                                                             pre, coarse, post, outlet, st)));
+    HFSI_TRY(launch_pointwise(h, 0, OpCopy<float, double>{h->z32, z}, st));
+  } else {
+    set_error("hfsi_fluid_vcycle: unsupported precision");
+    return HFSI_ERR_ARG;
+  }
+  return HFSI_OK;
+}
+
+int hfsi_fluid_solve_cg(hfsi_fluid* h, const float* divergence, const float* source, const double* iface_rhs,
+                        double* pressure, const hfsi_cg_options* opt, hfsi_cg_stats* stats, void* stream) {
+  if (!h || !divergence || !pressure || !opt) {
+    set_error("hfsi_fluid_solve_cg: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  if (!(h->prec_mask & opt->precision)) {
+    set_error("hfsi_fluid_solve_cg: precision not enabled on this handle");
+    return HFSI_ERR_ARG;
+  }
+  if (opt->preconditioner < 0 || opt->preconditioner > 3) {
+    set_error("hfsi_fluid_solve_cg: unsupported preconditioner");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  HFSI_TRY(set_tail_outlet(h, opt->outlet_zmin ? 1 : 0, st));
+  hfsi_cg_stats local;
+  hfsi_cg_stats& S = stats ? *stats : local;
+  if (opt->precision == HFSI_PREC_REF64)
+    return solve_cg<double, double>(h, divergence, source, iface_rhs, pressure, *opt, S, st);
+  if (opt->precision == HFSI_PREC_FP32)
+    return solve_cg<float, float>(h, divergence, source, iface_rhs, pressure, *opt, S, st);
+  set_error("hfsi_fluid_solve_cg: unsupported precision");
+  return HFSI_ERR_ARG;
+}
+
+int hfsi_fluid_solve_fv_jacobi(hfsi_fluid* h, const float* divergence, const float* source, const double* iface_rhs,
+                               double* pressure, double rhs_scale, int32_t iterations, int32_t outlet_zmin,
+                               void* stream) {
+  if (!h || !divergence || !pressure || iterations < 0) {
+    set_error("hfsi_fluid_solve_fv_jacobi: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int outlet = outlet_zmin ? 1 : 0;
+  HFSI_TRY(launch_pointwise(h, 0, OpRhsLegacy{divergence, source, iface_rhs, h->b, (float)rhs_scale}, st));
+  OpGeo<double> g = geo<double>(h, 0, outlet);
+  Ld<double, double> R{h->b};
+  double* in = pressure;
+  for (int m = 0; m < iterations; ++m) {
+    double* out = (in == pressure) ? h->t64 : pressure;
+    hipLaunchKernelGGL((k_jacobi<double, Ld<double, double>, Ld<double, double>, double>), dim3(blocks_for(g.n)),
+                       dim3(kBlock), 0, st, g, Ld<double, double>{in}, R, out, (double)0.8f);
+    in = out;
+  }
+  HFSI_LAUNCH_CHECK();
+  if (in != pressure) HFSI_CHECK(hipMemcpyAsync(pressure, in, g.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  return HFSI_OK;
+}
+
+int hfsi_fluid_solve_fv_multigrid(hfsi_fluid* h, const float* divergence, const float* source,
+                                  const double* iface_rhs, double* pressure, double rhs_scale, int32_t iterations,
+                                  int32_t cycles, int32_t outlet_zmin, void* stream) {
+  if (!h || !divergence || !pressure) {
+    set_error("hfsi_fluid_solve_fv_multigrid: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int outlet = outlet_zmin ? 1 : 0;
+  HFSI_TRY(set_tail_outlet(h, outlet, st));
+  HFSI_TRY(launch_pointwise(h, 0, OpRhsLegacy{divergence, source, iface_rhs, h->b, (float)rhs_scale}, st));
+  Ld<double, double> R{h->b};
+  if (h->L <= 1) {
+    int its = std::max(1, iterations);
+    return hfsi_fluid_solve_fv_jacobi(h, divergence, source, iface_rhs, pressure, rhs_scale, its, outlet_zmin, stream);
+  }
+  int ncyc = cycles > 0 ? cycles : (h->uniform ? 12 : 24);
+  for (int c = 0; c < ncyc; ++c)
+    HFSI_TRY((run_vcycle<double, double, Ld<double, double>>(h, pressure, h->t64, R, false, 3, 36, 3, outlet, st)));
+  if (!h->uniform)
+    return hfsi_fluid_solve_fv_jacobi(h, divergence, source, iface_rhs, pressure, rhs_scale, std::max(1, iterations),
+                                      outlet_zmin, stream);
+  return HFSI_OK;
+}
+
+int hfsi_fluid_solve_legacy_jacobi(hfsi_fluid* h, const float* divergence, const float* source,
+                                   const int32_t* obstacle, double* pressure, double rhs_scale, double inv_dx2,
+                                   double inv_dy2, double inv_dz2, int32_t iterations, int32_t outlet_zmin,
+                                   void* stream) {
+  if (!h || !divergence || !obstacle || !pressure || iterations < 0) {
+    set_error("hfsi_fluid_solve_legacy_jacobi: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  LevelMem& m = h->lv[0];
+  LegacyArgs a;
+  a.nx = m.nx; a.ny = m.ny; a.nz = m.nz;
+  a.obst = obstacle;
+  a.div = divergence;
+  a.src = source;
+  a.rhs_scale = (float)rhs_scale;
+  a.ix2 = (float)inv_dx2;
+  a.iy2 = (float)inv_dy2;
+  a.iz2 = (float)inv_dz2;
+  a.outlet = outlet_zmin ? 1 : 0;
+  double* in = pressure;
+  for (int it = 0; it < iterations; ++it) {
+    double* out = (in == pressure) ? h->t64 : pressure;
+    a.p = in;
+    a.out = out;
+    hipLaunchKernelGGL(k_legacy_jacobi, dim3(blocks_for(m.n)), dim3(kBlock), 0, st, a);
+    in = out;
+  }
+  HFSI_LAUNCH_CHECK();
+  if (in != pressure) HFSI_CHECK(hipMemcpyAsync(pressure, in, m.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  return HFSI_OK;
+}
+
+int hfsi_fluid_subtract_gradient(hfsi_fluid* h, float* velocity, const double* pressure, const int32_t* vd_active,
+                                 const float* vd_weight, const float* vd_value, float dt_over_rho,
+                                 int32_t outlet_zmin, void* stream) {
+  if (!h || !velocity || !pressure || (vd_active && !vd_weight)) {
+    set_error("hfsi_fluid_subtract_gradient: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  GradArgs a;
+  a.g = geo<double>(h, 0, outlet_zmin ? 1 : 0);
+  a.u = velocity;
+  a.p = pressure;
+  a.dx = h->lv[0].df[0];
+  a.dy = h->lv[0].df[1];
+  a.dz = h->lv[0].df[2];
+  a.vda = vd_active;
+  a.vdw = vd_weight;
+  a.vdv = vd_value;
+  a.dt_over_rho = dt_over_rho;
+  hipLaunchKernelGGL(k_subtract_gradient, dim3(blocks_for(a.g.n)), dim3(kBlock), 0, (hipStream_t)stream, a);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+int hfsi_fluid_wall_clamp(hfsi_fluid* h, float* velocity, int32_t outlet_zmin, void* stream) {
+  if (!h || !velocity) {
+    set_error("hfsi_fluid_wall_clamp: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  LevelMem& m = h->lv[0];
+  uint32_t most = std::max(m.ny * m.nz, std::max(m.nx * m.nz, m.nx * m.ny));
+  hipLaunchKernelGGL(k_wall_clamp, dim3((most + 255) / 256), dim3(256), 0, (hipStream_t)stream, m.nx, m.ny, m.nz,
+                     velocity, outlet_zmin ? 1 : 0);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+int hfsi_fluid_kernel_timer(hfsi_fluid* h, int32_t kernel_id) {
+  if (!h || kernel_id < 0 || kernel_id > HFSI_TIMER_L0_PROLONG_SWEEP) {
+    set_error("hfsi_fluid_kernel_timer: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  timer_flush(h);
+  h->timer_id = kernel_id;
+  h->timer_ms = 0.0;
+  h->timer_count = 0;
+  if (kernel_id != 0 && h->timer_events.empty()) {
+    h->timer_events.resize(4096);
+    for (hipEvent_t& e : h->timer_events) HFSI_CHECK(hipEventCreate(&e));
+  }
+  return HFSI_OK;
+}
+
+int hfsi_fluid_kernel_timer_read(hfsi_fluid* h, int64_t* launches, double* total_ms) {
+  if (!h || !launches || !total_ms) {
+    set_error("hfsi_fluid_kernel_timer_read: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  timer_flush(h);
+  *launches = h->timer_count;
+  *total_ms = h->timer_ms;
+  return HFSI_OK;
+}
+
+int hfsi_fluid_divergence_report(hfsi_fluid* h, const float* divergence, const float* source,
+                                 const double* iface_div, const int32_t* vd_active, int32_t outlet_zmin,
+                                 double* out54, void* stream) {
+  if (!h || !divergence || !out54) {
+    set_error("hfsi_fluid_divergence_report: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  ReportArgs a;
+  a.g = geo<float>(h, 0, outlet_zmin ? 1 : 0);
+  a.div = divergence;
+  a.src = source;
+  a.idiv = iface_div;
+  a.vda = vd_active;
+  a.partials = h->partials;
+  a.counter = h->counter;
+  a.totals = h->totals;
+  hipLaunchKernelGGL(k_div_report, dim3(blocks_for(a.g.n)), dim3(kBlock), 0, st, a);
+  HFSI_LAUNCH_CHECK();
+  HFSI_CHECK(hipMemcpyAsync(h->host_totals, h->totals, 54 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  std::memcpy(out54, h->host_totals, 54 * sizeof(double));
+  return HFSI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,21 @@
+"""Drop-in replacements for EasyFsi's ``simulation_core`` hot-path classes.
+
+Import the classes from the submodules (``fluid``, ``neo_hookean_mpm``); the
+submodules load libhfsi and need a visible gfx950 GPU when a solver object is
+constructed (grid geometry is host-only and works anywhere).
+"""
+
+from .grid import CartesianGrid, FluidDomainSpec, GradedGridSpec, RefinementRegion, build_graded_grid  # noqa: F401
+from .runtime import RuntimeConfig, TaichiRuntimeConfig, init_runtime, init_taichi  # noqa: F401
+
+
+def __getattr__(name):
+    if name == "CartesianFluidSolver":
+        from .fluid import CartesianFluidSolver
+
+        return CartesianFluidSolver
+    if name in ("NeoHookeanMpmState", "NeoHookeanMpmReport"):
+        from . import neo_hookean_mpm
+
+        return getattr(neo_hookean_mpm, name)
+    raise AttributeError(name)

@@ -1,0 +1,813 @@
+"""CartesianFluidSolver on libhfsi (MI355X).
+
+Drop-in for ``simulation_core/fluid.py::CartesianFluidSolver`` on the plain
+projection path: same constructor, method names, keyword arguments, argument
+validation messages, report keys and ``last_cg_*`` host mirrors.  Fields are
+torch tensors on the GPU (``easyfsi_amd.simulation_core.fields.Field``); all
+per-cell work of the projection runs in the hand-written HIP kernels of
+libhfsi (include/hfsi.h).
+
+Out of this build (raise ``NotImplementedError``): the HIBM pressure-interface
+row list and coupling slots (fluid.py:4341-4456) and their BiCGSTAB fallback,
+``predict`` (the semi-Lagrangian predictor, fluid.py:2447-2527), velocity
+constraints and the local divergence cleanup.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .fields import Field, scalar_field, vector_field
+from .grid import (
+    CartesianGrid,
+    FluidDomainSpec,
+    GradedGridSpec,
+    RefinementRegion,
+    axis_center_distances,
+    axis_centers,
+    build_graded_grid,
+    coarsen_axis_widths,
+    multigrid_axis_widths,
+    multigrid_shapes,
+)
+from .runtime import RuntimeConfig, TaichiRuntimeConfig, init_runtime
+
+__all__ = [
+    "CG_PRECONDITIONER_CHOICES",
+    "CartesianFluidSolver",
+    "CartesianGrid",
+    "FluidDomainSpec",
+    "GradedGridSpec",
+    "RefinementRegion",
+    "build_graded_grid",
+    "VelocityDirichletBoundaryReport",
+]
+
+CG_PRECONDITIONER_CHOICES = ("auto", "jacobi", "fv_multigrid", "fv_multigrid_light")
+PRESSURE_SOLVERS = ("jacobi", "compact_jacobi", "fv_jacobi", "fv_multigrid", "fv_cg")
+SOLVER_PRECISIONS = {"ref64": _lib.HFSI_PREC_REF64, "fp32": _lib.HFSI_PREC_FP32}
+
+
+class VelocityDirichletBoundaryReport:
+    def __init__(self, active_cells=0, max_delta_mps=0.0, mean_delta_mps=0.0, momentum_delta_n_s=(0.0, 0.0, 0.0)):
+        self.active_cells = active_cells
+        self.max_delta_mps = max_delta_mps
+        self.mean_delta_mps = mean_delta_mps
+        self.momentum_delta_n_s = momentum_delta_n_s
+
+
+class CartesianFluidSolver:
+    """Cell-centred 3D Cartesian fluid state whose projection runs on libhfsi."""
+
+    DEFAULT_MULTIGRID_CYCLES = 12
+    DEFAULT_NONUNIFORM_MULTIGRID_CYCLES = 24
+
+    _build_multigrid_shapes = staticmethod(multigrid_shapes)
+    _coarsen_axis_widths = staticmethod(coarsen_axis_widths)
+    _build_multigrid_axis_widths = staticmethod(multigrid_axis_widths)
+
+    def default_multigrid_cycles(self) -> int:
+        """fluid.py:731-734."""
+        return self.DEFAULT_MULTIGRID_CYCLES if self.grid.is_uniform else self.DEFAULT_NONUNIFORM_MULTIGRID_CYCLES
+
+    def __init__(self, spec: FluidDomainSpec, runtime: RuntimeConfig | None = None, *,
+                 solver_precision: str = "ref64"):
+        init_runtime(runtime)
+        self._lib = _lib.lib()
+        if solver_precision not in SOLVER_PRECISIONS:
+            raise ValueError(f"unsupported solver_precision: {solver_precision!r}")
+        self.solver_precision = solver_precision
+        self.spec = spec
+        self.grid: CartesianGrid = spec.cartesian_grid
+        self.nx, self.ny, self.nz = self.grid.grid_nodes
+        if self.grid.is_uniform:
+            self.dx, self.dy, self.dz = self.grid.uniform_spacing_m
+        else:
+            self.dx, self.dy, self.dz = (min(self.grid.cell_widths_x_m), min(self.grid.cell_widths_y_m),
+                                         min(self.grid.cell_widths_z_m))
+        self.bounds_min = tuple(float(v) for v in self.grid.bounds_min_m)
+        self.bounds_max = tuple(float(v) for v in self.grid.bounds_max_m)
+        self.rho = float(spec.density_kgm3)
+        self.mu = float(spec.viscosity_pa_s)
+        self.dt = float(spec.dt_s)
+        shape = self.grid.grid_nodes
+        f32, f64, i32 = torch.float32, torch.float64, torch.int32
+
+        # state fields (fluid.py:756-1024, plain-path subset)
+        self.velocity = vector_field(3, shape, f32)
+        self.saved_velocity = vector_field(3, shape, f32)
+        self.pressure = scalar_field(shape, f64)
+        self.saved_pressure = scalar_field(shape, f64)
+        self.pressure_tmp = scalar_field(shape, f64)
+        self.pressure_accum = scalar_field(shape, f64)
+        self.divergence = scalar_field(shape, f32)
+        self.pressure_interface_projection_divergence_s = scalar_field(shape, f64)
+        self.volume_source_s = scalar_field(shape, f32)
+        self.pressure_interface_matrix_diagonal = scalar_field(shape, f64)
+        self.pressure_interface_matrix_rhs = scalar_field(shape, f64)
+        self.obstacle = scalar_field(shape, i32)
+        self.velocity_dirichlet_boundary_active = scalar_field(shape, i32)
+        self.velocity_dirichlet_boundary_value_mps = vector_field(3, shape, f32)
+        self.velocity_dirichlet_boundary_projection_weight = scalar_field(shape, f32)
+        self.velocity_dirichlet_boundary_marker_region_id = scalar_field(shape, i32)
+        self.force = vector_field(3, shape, f32)
+        self.fv_diag = scalar_field(shape, f64)
+        self.cg_r = scalar_field(shape, f64)
+        self.cg_Ad = scalar_field(shape, f64)
+        self.cg_z = scalar_field(shape, f64)
+        self.cg_d = scalar_field(shape, f64)
+
+        # 1D metric fields, float32 like the reference (fluid.py:1013-1024)
+        g = self.grid
+        self.cell_width_x_m = self._axis_field(g.cell_widths_x_m)
+        self.cell_width_y_m = self._axis_field(g.cell_widths_y_m)
+        self.cell_width_z_m = self._axis_field(g.cell_widths_z_m)
+        self.cell_face_x_m = self._axis_field(g.cell_faces_x_m)
+        self.cell_face_y_m = self._axis_field(g.cell_faces_y_m)
+        self.cell_face_z_m = self._axis_field(g.cell_faces_z_m)
+        self.cell_center_x_m = self._axis_field(g.cell_centers_x_m)
+        self.cell_center_y_m = self._axis_field(g.cell_centers_y_m)
+        self.cell_center_z_m = self._axis_field(g.cell_centers_z_m)
+        self.center_distance_x_m = self._axis_field(g.center_distances_x_m)
+        self.center_distance_y_m = self._axis_field(g.center_distances_y_m)
+        self.center_distance_z_m = self._axis_field(g.center_distances_z_m)
+
+        self._mg_shapes = multigrid_shapes(shape)
+        self._mg_axis_widths_m = multigrid_axis_widths(self.grid, self._mg_shapes)
+        self._mg_cell_width_x_m, self._mg_cell_width_y_m, self._mg_cell_width_z_m = [], [], []
+        self._mg_center_distance_x_m, self._mg_center_distance_y_m, self._mg_center_distance_z_m = [], [], []
+        for level, (wx, wy, wz) in enumerate(self._mg_axis_widths_m):
+            self._mg_cell_width_x_m.append(self.cell_width_x_m if level == 0 else self._axis_field(wx))
+            self._mg_cell_width_y_m.append(self.cell_width_y_m if level == 0 else self._axis_field(wy))
+            self._mg_cell_width_z_m.append(self.cell_width_z_m if level == 0 else self._axis_field(wz))
+            for lst, w, o in ((self._mg_center_distance_x_m, wx, self.bounds_min[0]),
+                              (self._mg_center_distance_y_m, wy, self.bounds_min[1]),
+                              (self._mg_center_distance_z_m, wz, self.bounds_min[2])):
+                lst.append(self._axis_field(axis_center_distances(axis_centers(o, w), w)))
+
+        # libhfsi handle
+        handle = ctypes.c_void_p()
+        wx = np.ascontiguousarray(g.cell_widths_x_m, dtype=np.float64)
+        wy = np.ascontiguousarray(g.cell_widths_y_m, dtype=np.float64)
+        wz = np.ascontiguousarray(g.cell_widths_z_m, dtype=np.float64)
+        origin = np.ascontiguousarray(self.bounds_min, dtype=np.float64)
+        mask = SOLVER_PRECISIONS[solver_precision] | _lib.HFSI_PREC_REF64
+        _lib.check(self._lib.hfsi_fluid_create(ctypes.byref(handle), self.nx, self.ny, self.nz,
+                                               wx.ctypes.data, wy.ctypes.data, wz.ctypes.data, origin.ctypes.data,
+                                               mask, _lib.stream_handle()), "hfsi_fluid_create")
+        self._handle = handle
+        if self._lib.hfsi_fluid_num_levels(handle) != len(self._mg_shapes):
+            raise RuntimeError("libhfsi multigrid hierarchy does not match the host hierarchy")
+
+        self._reset_cg_mirrors()
+        self._reset_project_cg_mirrors()
+        self.last_divergence_report_host_reads = 0
+        self.last_pressure_outlet_report_host_reads = 0
+        self._hibm_pressure_unreached_count = 0
+        self.clear()
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.hfsi_fluid_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    @staticmethod
+    def _axis_field(values) -> Field:
+        f = scalar_field((len(values),), torch.float32)
+        f.from_numpy(np.asarray(values, dtype=np.float32))
+        return f
+
+    # ------------------------------------------------------------------
+    # host mirrors (fluid.py:1079-1157)
+    # ------------------------------------------------------------------
+    def _reset_cg_mirrors(self) -> None:
+        self.last_cg_iterations = 0
+        self.last_cg_initial_relative_residual = math.inf
+        self.last_cg_relative_residual = math.inf
+        self.last_cg_converged = False
+        self.last_cg_breakdown = ""
+        self.last_cg_host_residual_checks = 0
+        self.last_cg_mean_host_reads = 0
+        self.last_cg_mean_projection_count = 0
+        self.last_cg_unreached_set_mean_projection_count = 0
+        self.last_cg_restart_count = 0
+        self.last_cg_restart_count_measured = False
+        self.last_cg_restart_policy = ""
+        self.last_cg_breakdown_dAd = 0.0
+        self.last_cg_breakdown_code = 0
+        self.last_cg_true_relative_residual = -1.0
+        self.last_cg_used_multigrid = False
+
+    def _reset_project_cg_mirrors(self) -> None:
+        self.last_project_cg_project_calls = 0
+        self.last_project_cg_iterations_total = 0
+        self.last_project_cg_iterations_max = 0
+        self.last_project_cg_host_residual_checks = 0
+        self.last_project_cg_mean_host_reads = 0
+        self.last_project_cg_mean_projection_count = 0
+        self.last_project_cg_unreached_set_mean_projection_count = 0
+        self.last_project_cg_restart_count = 0
+        self.last_project_cg_restart_count_measured = False
+        self.last_project_cg_restart_policy = ""
+        self.last_project_cg_initial_relative_residual_max = 0.0
+        self.last_project_cg_relative_residual_max = 0.0
+        self.last_project_cg_converged_all = True
+        self.last_project_cg_breakdown_count = 0
+        self.last_project_cg_breakdown_code = 0
+        self.last_project_cg_breakdown_dAd = 0.0
+        self.last_project_cg_breakdown = ""
+
+    # ------------------------------------------------------------------
+    # state helpers
+    # ------------------------------------------------------------------
+    def clear(self) -> None:
+        """Zero every field (fluid.py:1201-1325)."""
+        for f in (self.velocity, self.saved_velocity, self.pressure, self.saved_pressure, self.pressure_tmp,
+                  self.pressure_accum, self.divergence, self.pressure_interface_projection_divergence_s,
+                  self.volume_source_s, self.pressure_interface_matrix_diagonal, self.pressure_interface_matrix_rhs,
+                  self.obstacle, self.velocity_dirichlet_boundary_active, self.velocity_dirichlet_boundary_value_mps,
+                  self.velocity_dirichlet_boundary_projection_weight, self.force, self.fv_diag, self.cg_r,
+                  self.cg_Ad, self.cg_z, self.cg_d):
+            f.fill(0)
+        self.velocity_dirichlet_boundary_marker_region_id.fill(-1)
+
+    def clear_pressure(self) -> None:
+        """fluid.py:4055-4073."""
+        self.pressure.fill(0)
+        self.pressure_tmp.fill(0)
+        self.pressure_accum.fill(0)
+
+    def save_state(self) -> None:
+        """Snapshot velocity and pressure (fluid.py:2266-2281)."""
+        self.saved_velocity.copy_from(self.velocity)
+        self.saved_pressure.copy_from(self.pressure)
+
+    def restore_state(self) -> None:
+        """fluid.py:2283-2296."""
+        self.velocity.copy_from(self.saved_velocity)
+        self.pressure.copy_from(self.saved_pressure)
+
+    def set_uniform_velocity(self, velocity_mps) -> None:
+        """fluid.py:2778-2790."""
+        v = torch.tensor([float(x) for x in velocity_mps], dtype=torch.float32, device=self.velocity.tensor.device)
+        self.velocity.tensor[:] = v
+
+    def set_sinusoidal_divergent_velocity(self, amplitude_mps: float) -> None:
+        """Cell-index sinusoids (fluid.py:2792-2812)."""
+        if amplitude_mps <= 0.0:
+            raise ValueError("amplitude_mps must be positive")
+        a = np.float32(amplitude_mps)
+        out = np.zeros(self.grid.grid_nodes + (3,), dtype=np.float32)
+        two_pi = np.float32(2.0 * math.pi)
+        for axis, n, s in ((0, self.nx, np.float32(1.0)), (1, self.ny, np.float32(0.5)), (2, self.nz, np.float32(-0.25))):
+            c = (np.arange(n, dtype=np.float32) + np.float32(0.5)) / np.float32(n)
+            prof = (s * a * np.sin(two_pi * c)).astype(np.float32)
+            shape = [1, 1, 1]
+            shape[axis] = n
+            out[..., axis] = prof.reshape(shape)
+        self.velocity.from_numpy(out)
+
+    def set_simple_shear_velocity(self, shear_rate_s: float, center_y_m: float = 0.0) -> None:
+        """fluid.py:2814-2826."""
+        y = self.cell_center_y_m.tensor
+        self.velocity.tensor.zero_()
+        self.velocity.tensor[..., 0] = (np.float32(shear_rate_s) * (y - np.float32(center_y_m)))[None, :, None]
+
+    def mark_sphere_obstacle(self, center_m, radius_m: float) -> None:
+        """Cell-centre-in-sphere mask (fluid.py:2828-2858)."""
+        if radius_m <= 0.0:
+            raise ValueError("radius_m must be positive")
+        c = [np.float32(v) for v in center_m]
+        r2 = np.float32(radius_m) * np.float32(radius_m)
+        x = self.cell_center_x_m.tensor - c[0]
+        y = self.cell_center_y_m.tensor - c[1]
+        z = self.cell_center_z_m.tensor - c[2]
+        d2 = x[:, None, None] * x[:, None, None] + y[None, :, None] * y[None, :, None] + z[None, None, :] * z[None, None, :]
+        self.obstacle.tensor.copy_((d2 <= r2).to(torch.int32))
+
+    # ------------------------------------------------------------------
+    # library plumbing
+    # ------------------------------------------------------------------
+    @property
+    def _stream(self) -> int:
+        return _lib.stream_handle()
+
+    def _vd_active_any(self) -> bool:
+        return bool(torch.any(self.velocity_dirichlet_boundary_active.tensor != 0).item())
+
+    def _set_masks(self) -> None:
+        has_vd = self._vd_active_any()
+        diag = self.pressure_interface_matrix_diagonal.tensor
+        has_diag = bool(torch.any(diag != 0).item())
+        _lib.check(self._lib.hfsi_fluid_set_masks(
+            self._handle, self.obstacle.ptr,
+            self.velocity_dirichlet_boundary_active.ptr if has_vd else None,
+            self.velocity_dirichlet_boundary_projection_weight.ptr if has_vd else None,
+            diag.data_ptr() if has_diag else None, self._stream), "hfsi_fluid_set_masks")
+        self._masks_have_vd = has_vd
+
+    # ------------------------------------------------------------------
+    # divergence and its statistics
+    # ------------------------------------------------------------------
+    def compute_divergence(self, *, pressure_outlet_zmin: bool = False) -> None:
+        """fluid.py:3488-3491."""
+        self._set_masks()
+        self._divergence_only(pressure_outlet_zmin)
+
+    def _divergence_only(self, outlet: bool) -> None:
+        _lib.check(self._lib.hfsi_fluid_divergence(self._handle, self.velocity.ptr, self.divergence.ptr,
+                                                   1 if outlet else 0, self._stream), "hfsi_fluid_divergence")
+
+    def _report_slots(self, outlet: bool = False) -> np.ndarray:
+        out = np.zeros(54, dtype=np.float64)
+        vd = self.velocity_dirichlet_boundary_active.ptr if self._vd_active_any() else None
+        _lib.check(self._lib.hfsi_fluid_divergence_report(
+            self._handle, self.divergence.ptr, self.volume_source_s.ptr,
+            self.pressure_interface_projection_divergence_s.ptr, vd, 1 if outlet else 0,
+            out.ctypes.data, self._stream), "hfsi_fluid_divergence_report")
+        self.last_divergence_report_host_reads += 1
+        return out.reshape(18, 3)
+
+    @staticmethod
+    def _decode(slot_row, with_count: bool = False) -> dict:
+        count = int(slot_row[2])
+        out = {"max_abs": float(slot_row[0]), "l2": math.sqrt(float(slot_row[1]) / max(1, count))}
+        if with_count:
+            out["count"] = count
+        return out
+
+    def divergence_stats(self, *, interior_only: bool = False) -> dict[str, float]:
+        """fluid.py:3827-3836 (raw divergence + interface projection divergence)."""
+        slots = self._report_slots()
+        return self._decode(slots[2 if interior_only else 0])
+
+    def divergence_residual_stats(self, *, interior_only: bool = False) -> dict[str, float]:
+        """fluid.py:3838-3848 (minus the explicit volume source)."""
+        slots = self._report_slots()
+        return self._decode(slots[3 if interior_only else 1])
+
+    def final_and_dirichlet_partition_report_stats(self, *, pressure_outlet_zmin: bool = False):
+        """The 16 decoded partitions of the final report (fluid.py:3860-3994)."""
+        s = self._report_slots(pressure_outlet_zmin)
+        order = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)
+        return tuple(self._decode(s[i], with_count=i >= 8) for i in order)
+
+    # ------------------------------------------------------------------
+    # pressure solvers
+    # ------------------------------------------------------------------
+    def _solve_pressure_poisson_with_solver(self, *, iterations, rhs_scale, inv_dx2, inv_dy2, inv_dz2,
+                                            pressure_outlet_zmin, pressure_solver, multigrid_cycles, cg_tolerance,
+                                            cg_preconditioner="auto", remove_nullspace_mean=True,
+                                            pressure_interface_matrix_active=False) -> None:
+        """fluid.py:6813-6867."""
+        outlet = 1 if pressure_outlet_zmin else 0
+        st = self._stream
+        iface_rhs = self.pressure_interface_matrix_rhs.tensor
+        iface_ptr = iface_rhs.data_ptr() if bool(torch.any(iface_rhs != 0).item()) else None
+        if pressure_solver in ("jacobi", "compact_jacobi"):
+            _lib.check(self._lib.hfsi_fluid_solve_legacy_jacobi(
+                self._handle, self.divergence.ptr, self.volume_source_s.ptr, self.obstacle.ptr, self.pressure.ptr,
+                float(rhs_scale), float(inv_dx2), float(inv_dy2), float(inv_dz2), int(iterations), outlet, st),
+                "hfsi_fluid_solve_legacy_jacobi")
+            return
+        if pressure_solver == "fv_jacobi":
+            _lib.check(self._lib.hfsi_fluid_solve_fv_jacobi(
+                self._handle, self.divergence.ptr, self.volume_source_s.ptr, iface_ptr, self.pressure.ptr,
+                float(rhs_scale), int(iterations), outlet, st), "hfsi_fluid_solve_fv_jacobi")
+            return
+        if pressure_solver == "fv_multigrid":
+            cycles = int(multigrid_cycles) if multigrid_cycles is not None else self.default_multigrid_cycles()
+            if cycles <= 0:
+                raise ValueError("multigrid_cycles must be positive")
+            _lib.check(self._lib.hfsi_fluid_solve_fv_multigrid(
+                self._handle, self.divergence.ptr, self.volume_source_s.ptr, iface_ptr, self.pressure.ptr,
+                float(rhs_scale), int(iterations), cycles, outlet, st), "hfsi_fluid_solve_fv_multigrid")
+            return
+        if pressure_solver == "fv_cg":
+            self._solve_pressure_poisson_fv_cg(
+                iterations=iterations, rhs_scale=rhs_scale, pressure_outlet_zmin=pressure_outlet_zmin,
+                tolerance=cg_tolerance, preconditioner=cg_preconditioner,
+                remove_nullspace_mean=remove_nullspace_mean,
+                pressure_interface_matrix_active=pressure_interface_matrix_active, iface_rhs_ptr=iface_ptr)
+            return
+        raise ValueError(f"unsupported pressure_solver: {pressure_solver!r}")
+
+    def _solve_pressure_poisson_fv_cg(self, *, iterations, rhs_scale, pressure_outlet_zmin, tolerance,
+                                      preconditioner="auto", remove_nullspace_mean=True,
+                                      pressure_interface_matrix_active=False, iface_rhs_ptr=None,
+                                      precision: str | None = None, true_residual_at_exit: bool = False) -> None:
+        """Flexible PCG (fluid.py:6535-6811) on libhfsi."""
+        name = str(preconditioner)
+        if name not in CG_PRECONDITIONER_CHOICES:
+            raise ValueError(f"unsupported cg_preconditioner: {preconditioner!r}")
+        prec = precision or self.solver_precision
+        opt = _lib.CgOptions()
+        opt.precision = SOLVER_PRECISIONS[prec]
+        opt.preconditioner = _lib.PRECONDITIONERS[name]
+        opt.max_iterations = max(1, int(iterations))
+        opt.remove_mean = 1 if remove_nullspace_mean else 0
+        opt.outlet_zmin = 1 if pressure_outlet_zmin else 0
+        opt.interface_active = 1 if pressure_interface_matrix_active else 0
+        opt.check_interval = 16
+        opt.restart_interval = 256 if pressure_interface_matrix_active else 512
+        opt.true_residual = 1 if prec != "ref64" else 0
+        opt.true_residual_at_exit = 1 if true_residual_at_exit else 0
+        opt.tolerance = max(0.0, float(tolerance))
+        opt.rhs_scale = float(rhs_scale)
+        stats = _lib.CgStats()
+        self._reset_cg_mirrors()
+        _lib.check(self._lib.hfsi_fluid_solve_cg(self._handle, self.divergence.ptr, self.volume_source_s.ptr,
+                                                 iface_rhs_ptr, self.pressure.ptr, ctypes.byref(opt),
+                                                 ctypes.byref(stats), self._stream), "hfsi_fluid_solve_cg")
+        self.last_cg_iterations = int(stats.iterations)
+        self.last_cg_initial_relative_residual = float(stats.initial_relative_residual)
+        self.last_cg_relative_residual = float(stats.relative_residual)
+        self.last_cg_converged = bool(stats.converged)
+        self.last_cg_breakdown = _lib.BREAKDOWN_MESSAGES.get(int(stats.breakdown_kind), "")
+        self.last_cg_host_residual_checks = int(stats.host_residual_checks)
+        self.last_cg_mean_host_reads = 0
+        self.last_cg_mean_projection_count = int(stats.mean_projection_count)
+        self.last_cg_restart_count = int(stats.restart_count)
+        self.last_cg_restart_count_measured = True
+        self.last_cg_restart_policy = "periodic_exact_residual"
+        self.last_cg_breakdown_dAd = float(stats.breakdown_dAd)
+        self.last_cg_breakdown_code = int(stats.breakdown_code)
+        self.last_cg_true_relative_residual = float(stats.true_relative_residual)
+        self.last_cg_used_multigrid = bool(stats.used_multigrid)
+        self.last_cg_stats = stats.as_dict()
+
+    # ------------------------------------------------------------------
+    # projection
+    # ------------------------------------------------------------------
+    def _subtract_pressure_gradient(self, dt_over_rho: float, outlet: bool) -> None:
+        has_vd = self._vd_active_any()
+        _lib.check(self._lib.hfsi_fluid_subtract_gradient(
+            self._handle, self.velocity.ptr, self.pressure.ptr,
+            self.velocity_dirichlet_boundary_active.ptr if has_vd else None,
+            self.velocity_dirichlet_boundary_projection_weight.ptr if has_vd else None,
+            self.velocity_dirichlet_boundary_value_mps.ptr if has_vd else None,
+            float(np.float32(dt_over_rho)), 1 if outlet else 0, self._stream), "hfsi_fluid_subtract_gradient")
+
+    def _apply_closed_boundary_no_normal_flow_kernel(self, pressure_outlet_zmin: int) -> None:
+        _lib.check(self._lib.hfsi_fluid_wall_clamp(self._handle, self.velocity.ptr, int(pressure_outlet_zmin),
+                                                   self._stream), "hfsi_fluid_wall_clamp")
+
+    def _apply_zmin_no_backflow_kernel(self) -> None:
+        """fluid.py:5761-5765."""
+        vz = self.velocity.tensor[:, :, 0, 2]
+        free = self.obstacle.tensor[:, :, 0] == 0
+        vz.masked_fill_(free & (vz > 0.0), 0.0)
+
+    def apply_velocity_dirichlet_boundary_rows(self, *, read_report: bool = True):
+        """Copy prescribed velocities into active free rows (fluid.py:2548-2626)."""
+        act = (self.velocity_dirichlet_boundary_active.tensor != 0) & (self.obstacle.tensor == 0)
+        n = int(act.sum().item())
+        if n == 0:
+            return VelocityDirichletBoundaryReport() if read_report else None
+        old = self.velocity.tensor[act]
+        new = self.velocity_dirichlet_boundary_value_mps.tensor[act]
+        self.velocity.tensor[act] = new
+        if not read_report:
+            return None
+        delta = (new - old).double()
+        norms = torch.linalg.vector_norm(delta, dim=1)
+        vol = (self.cell_width_x_m.tensor.double()[:, None, None] * self.cell_width_y_m.tensor.double()[None, :, None]
+               * self.cell_width_z_m.tensor.double()[None, None, :])[act]
+        mom = (self.rho * delta * vol[:, None]).sum(dim=0)
+        return VelocityDirichletBoundaryReport(active_cells=n, max_delta_mps=float(norms.max().item()),
+                                               mean_delta_mps=float(norms.sum().item()) / n,
+                                               momentum_delta_n_s=tuple(float(v) for v in mom.tolist()))
+
+    def pressure_interface_matrix_terms_report(self) -> dict:
+        """Cell-field part of fluid.py:3080-3115 (this build keeps no row list)."""
+        d = self.pressure_interface_matrix_diagonal.tensor
+        r = self.pressure_interface_matrix_rhs.tensor
+        vol = (self.cell_width_x_m.tensor.double()[:, None, None] * self.cell_width_y_m.tensor.double()[None, :, None]
+               * self.cell_width_z_m.tensor.double()[None, None, :])
+        return {
+            "diagonal_integral": float((d * vol).sum().item()),
+            "rhs_integral": float((r * vol).sum().item()),
+            "max_abs_diagonal": float(d.abs().max().item()),
+            "active_cells": int(((d.abs() > 0) | (r.abs() > 0)).sum().item()),
+            "row_count": 0, "row_active_count": 0, "row_invalid_count": 0, "row_overflow_count": 0,
+            "row_diagonal_integral": 0.0, "row_diagonal_integral_abs_mismatch": 0.0,
+            "row_diagonal_max_abs_density_mismatch": 0.0, "row_max_transmissibility": 0.0,
+        }
+
+    def project(self, iterations: int = 40, pressure_outlet_zmin: bool = False, dt_s: float | None = None,
+                preserve_velocity_constraints: bool = False, velocity_constraint_blend: float = 1.0,
+                velocity_constraint_solid_mobility_ratio: float = 0.0, divergence_cleanup_iterations: int = 0,
+                divergence_cleanup_relaxation: float = 0.7, reset_pressure: bool = False,
+                pressure_solver: str = "jacobi", multigrid_cycles: int | None = None, cg_tolerance: float = 1.0e-6,
+                cg_preconditioner: str = "auto", pressure_solve_failure_policy: str = "report",
+                read_report: bool = True) -> dict:
+        """Pressure projection (fluid.py:6889-7613), plain path."""
+        if iterations <= 0:
+            raise ValueError("iterations must be positive")
+        step_dt_s = self.dt if dt_s is None else float(dt_s)
+        if step_dt_s <= 0.0:
+            raise ValueError("dt_s must be positive")
+        if not 0.0 <= float(velocity_constraint_blend) <= 1.0:
+            raise ValueError("velocity_constraint_blend must be in [0, 1]")
+        mob = float(velocity_constraint_solid_mobility_ratio)
+        if not math.isfinite(mob) or mob < 0.0:
+            raise ValueError("velocity_constraint_solid_mobility_ratio must be a finite non-negative number")
+        if not 0.0 <= float(divergence_cleanup_relaxation) <= 1.0:
+            raise ValueError("divergence_cleanup_relaxation must be in [0, 1]")
+        tol = float(cg_tolerance)
+        if not math.isfinite(tol) or tol < 0.0:
+            raise ValueError("cg_tolerance must be a finite non-negative number")
+        if str(cg_preconditioner) not in CG_PRECONDITIONER_CHOICES:
+            raise ValueError(f"unsupported cg_preconditioner: {cg_preconditioner!r}")
+        policy = str(pressure_solve_failure_policy)
+        if policy not in ("report", "raise"):
+            raise ValueError("pressure_solve_failure_policy must be 'report' or 'raise'")
+        requested = str(pressure_solver)
+        solver = requested
+        if solver not in PRESSURE_SOLVERS:
+            raise ValueError(f"unsupported pressure_solver: {pressure_solver!r}")
+        if not self.grid.is_uniform:
+            if solver not in ("fv_jacobi", "fv_multigrid", "fv_cg"):
+                raise ValueError("non-uniform CartesianGrid projection requires an FV pressure solver")
+            if divergence_cleanup_iterations > 0:
+                raise ValueError(
+                    "non-uniform CartesianGrid divergence cleanup requires non-uniform cleanup operators")
+        if multigrid_cycles is not None and int(multigrid_cycles) <= 0:
+            raise ValueError("multigrid_cycles must be positive")
+        if preserve_velocity_constraints:
+            raise NotImplementedError("velocity constraints are outside this build's projection path")
+        if divergence_cleanup_iterations > 0:
+            raise NotImplementedError("the local divergence cleanup is outside this build's projection path")
+
+        report = bool(read_report)
+        outlet = bool(pressure_outlet_zmin)
+        self.last_divergence_report_host_reads = 0
+        empty = {"max_abs": math.nan, "l2": math.nan}
+        iface = self.pressure_interface_matrix_terms_report()
+        iface_active = (iface["active_cells"] > 0 or iface["row_count"] > 0 or iface["row_active_count"] > 0
+                        or iface["max_abs_diagonal"] > 0.0)
+        forced, force_reason = False, ""
+        if iface_active and solver == "fv_multigrid":
+            solver, forced, force_reason = "fv_cg", True, "pressure_interface_matrix_requires_rowlist_fv_cg"
+        anchored = (not outlet) and iface["max_abs_diagonal"] > 0.0
+        zero_mean = (not outlet) and solver == "fv_cg" and not anchored
+        if outlet:
+            ns_policy, ns_measured = "pressure_outlet_dirichlet", True
+        elif anchored:
+            ns_policy, ns_measured = "interface_matrix_anchored", solver == "fv_cg"
+        elif solver == "fv_cg":
+            ns_policy, ns_measured = "closed_neumann_fv_cg_zero_mean", True
+        else:
+            ns_policy, ns_measured = "closed_neumann_non_cg_unmeasured", False
+        self._reset_project_cg_mirrors()
+        self.last_project_cg_restart_count_measured = solver == "fv_cg"
+        self.last_project_cg_restart_policy = "periodic_exact_residual" if solver == "fv_cg" else "not_applicable_non_cg"
+        failed = False
+        failure_action = "none"
+
+        def record_cg_stats() -> None:
+            nonlocal failed
+            if solver != "fv_cg":
+                return
+            self.last_project_cg_project_calls += 1
+            self.last_project_cg_iterations_total += int(self.last_cg_iterations)
+            self.last_project_cg_iterations_max = max(self.last_project_cg_iterations_max, int(self.last_cg_iterations))
+            self.last_project_cg_host_residual_checks += int(self.last_cg_host_residual_checks)
+            self.last_project_cg_mean_projection_count += int(self.last_cg_mean_projection_count)
+            self.last_project_cg_restart_count += int(self.last_cg_restart_count)
+            self.last_project_cg_restart_count_measured = bool(self.last_cg_restart_count_measured)
+            self.last_project_cg_restart_policy = str(self.last_cg_restart_policy)
+            if math.isfinite(self.last_cg_initial_relative_residual):
+                self.last_project_cg_initial_relative_residual_max = max(
+                    self.last_project_cg_initial_relative_residual_max, self.last_cg_initial_relative_residual)
+            if math.isfinite(self.last_cg_relative_residual):
+                self.last_project_cg_relative_residual_max = max(self.last_project_cg_relative_residual_max,
+                                                                 self.last_cg_relative_residual)
+            if not self.last_cg_converged:
+                failed = True
+                self.last_project_cg_converged_all = False
+                self.last_project_cg_breakdown_count += 1
+                self.last_project_cg_breakdown_code = max(self.last_project_cg_breakdown_code,
+                                                          int(self.last_cg_breakdown_code))
+                if math.isfinite(self.last_cg_breakdown_dAd):
+                    self.last_project_cg_breakdown_dAd = float(self.last_cg_breakdown_dAd)
+                if self.last_cg_breakdown:
+                    self.last_project_cg_breakdown = str(self.last_cg_breakdown)
+
+        def handle_failure() -> None:
+            nonlocal failure_action
+            if solver != "fv_cg" or self.last_cg_converged:
+                return
+            if policy != "raise":
+                failure_action = "reported_cleared_pressure_correction"
+                self.pressure.fill(0)
+                self.pressure_tmp.fill(0)
+                self.pressure_accum.fill(0)
+                return
+            raise RuntimeError(
+                "FV-CG pressure solve did not converge before pressure-gradient velocity correction "
+                f"(iterations={int(self.last_cg_iterations)}, "
+                f"relative_residual={float(self.last_cg_relative_residual):.6g}, "
+                f"breakdown={self.last_cg_breakdown!r})")
+
+        vd_calls = 0
+        vd_cells_total = 0
+        vd_cells_max = 0
+        vd_delta_sum = 0.0
+        vd_delta_max = 0.0
+        vd_momentum = (0.0, 0.0, 0.0)
+
+        def apply_bcs() -> None:
+            nonlocal vd_calls, vd_cells_total, vd_cells_max, vd_delta_sum, vd_delta_max, vd_momentum
+            rep = self.apply_velocity_dirichlet_boundary_rows(read_report=report)
+            if report:
+                vd_calls += 1
+                vd_cells_total += rep.active_cells
+                vd_cells_max = max(vd_cells_max, rep.active_cells)
+                vd_delta_sum += rep.mean_delta_mps * rep.active_cells
+                vd_delta_max = max(vd_delta_max, rep.max_delta_mps)
+                vd_momentum = tuple(a + b for a, b in zip(vd_momentum, rep.momentum_delta_n_s))
+            if outlet:
+                self._apply_zmin_no_backflow_kernel()
+                self._apply_closed_boundary_no_normal_flow_kernel(1)
+
+        self.pressure_interface_projection_divergence_s.fill(0)
+        self._set_masks()
+        apply_bcs()
+        self._divergence_only(outlet)
+        if report:
+            pre_raw, pre = self.divergence_stats(), self.divergence_residual_stats()
+        else:
+            pre_raw, pre = empty, empty
+        if reset_pressure:
+            self.clear_pressure()
+        rhs_scale = self.rho / step_dt_s
+        solve_kwargs = dict(iterations=int(iterations), rhs_scale=rhs_scale, inv_dx2=1.0 / (self.dx * self.dx),
+                            inv_dy2=1.0 / (self.dy * self.dy), inv_dz2=1.0 / (self.dz * self.dz),
+                            pressure_outlet_zmin=outlet, pressure_solver=solver, multigrid_cycles=multigrid_cycles,
+                            cg_tolerance=tol, cg_preconditioner=str(cg_preconditioner),
+                            remove_nullspace_mean=zero_mean, pressure_interface_matrix_active=iface_active)
+        self._solve_pressure_poisson_with_solver(**solve_kwargs)
+        record_cg_stats()
+        handle_failure()
+        self._subtract_pressure_gradient(step_dt_s / self.rho, outlet)
+        apply_bcs()
+        self._divergence_only(outlet)
+        target_l2_sq = 0.0
+        if report:
+            proj_raw, proj = self.divergence_stats(), self.divergence_residual_stats()
+        else:
+            proj_raw, proj = empty, empty
+            if outlet:  # cleanup target from the post-projection residual (fluid.py:7246-7256)
+                target_l2_sq = max(self.divergence_residual_stats()["l2"] ** 2 * 1.05 * 1.05, 1.0e-16)
+        post_raw, post = proj_raw, proj
+        if outlet:
+            self.pressure_accum.copy_from(self.pressure)
+            apply_bcs()
+            self._divergence_only(outlet)
+            if report:
+                post_raw, post = self.divergence_stats(), self.divergence_residual_stats()
+                target = max(float(proj["l2"]) * 1.05, 1.0e-8)
+                needed = post["l2"] > target
+            else:
+                needed = self.divergence_residual_stats()["l2"] ** 2 > target_l2_sq
+            cleanup_its = max(1, min(int(iterations), 256))
+            for _ in range(3):
+                if not needed:
+                    break
+                self.pressure.fill(0)
+                self.pressure_tmp.fill(0)
+                self.pressure_interface_matrix_rhs.fill(0)
+                self._solve_pressure_poisson_with_solver(**{**solve_kwargs, "iterations": cleanup_its})
+                record_cg_stats()
+                handle_failure()
+                self._subtract_pressure_gradient(step_dt_s / self.rho, outlet)
+                self.pressure_accum.tensor.add_(self.pressure.tensor)
+                self.pressure.copy_from(self.pressure_accum)
+                self.pressure_tmp.copy_from(self.pressure_accum)
+                apply_bcs()
+                self._divergence_only(outlet)
+                if report:
+                    post_raw, post = self.divergence_stats(), self.divergence_residual_stats()
+                    needed = post["l2"] > target
+                else:
+                    needed = self.divergence_residual_stats()["l2"] ** 2 > target_l2_sq
+        if not outlet:
+            self._apply_closed_boundary_no_normal_flow_kernel(0)
+        if not report:
+            return {}
+        vd_mean = vd_delta_sum / vd_cells_total if vd_cells_total > 0 else 0.0
+        self._divergence_only(outlet)
+        (final_raw, final, int_raw, interior, near_raw, near, far_raw, far, corr_raw, corr, fixed_raw, fixed,
+         icorr_raw, icorr, ifixed_raw, ifixed) = self.final_and_dirichlet_partition_report_stats(
+            pressure_outlet_zmin=outlet)
+        return {
+            "pressure_solver_requested": requested,
+            "pressure_solver": solver,
+            "pressure_solver_forced_to_fv_cg": forced,
+            "pressure_solver_force_reason": force_reason,
+            "l2": final["l2"], "max_abs": final["max_abs"],
+            "raw_l2": final_raw["l2"], "raw_max_abs": final_raw["max_abs"],
+            "interior_l2": interior["l2"], "interior_max_abs": interior["max_abs"],
+            "interior_raw_l2": int_raw["l2"], "interior_raw_max_abs": int_raw["max_abs"],
+            "pre_projection_l2": pre["l2"], "pre_projection_max_abs": pre["max_abs"],
+            "pre_projection_raw_l2": pre_raw["l2"], "pre_projection_raw_max_abs": pre_raw["max_abs"],
+            "projection_l2": proj["l2"], "projection_max_abs": proj["max_abs"],
+            "projection_raw_l2": proj_raw["l2"], "projection_raw_max_abs": proj_raw["max_abs"],
+            "post_boundary_l2": post["l2"], "post_boundary_max_abs": post["max_abs"],
+            "post_boundary_raw_l2": post_raw["l2"], "post_boundary_raw_max_abs": post_raw["max_abs"],
+            "post_constraint_l2": final["l2"], "post_constraint_max_abs": final["max_abs"],
+            "post_constraint_raw_l2": final_raw["l2"], "post_constraint_raw_max_abs": final_raw["max_abs"],
+            "velocity_dirichlet_near_l2": near["l2"], "velocity_dirichlet_near_max_abs": near["max_abs"],
+            "velocity_dirichlet_near_raw_l2": near_raw["l2"], "velocity_dirichlet_near_raw_max_abs": near_raw["max_abs"],
+            "velocity_dirichlet_far_l2": far["l2"], "velocity_dirichlet_far_max_abs": far["max_abs"],
+            "velocity_dirichlet_far_raw_l2": far_raw["l2"], "velocity_dirichlet_far_raw_max_abs": far_raw["max_abs"],
+            "velocity_dirichlet_boundary_apply_calls": int(vd_calls),
+            "velocity_dirichlet_boundary_active_cells_total": int(vd_cells_total),
+            "velocity_dirichlet_boundary_active_cells_max": int(vd_cells_max),
+            "velocity_dirichlet_boundary_max_delta_mps": float(vd_delta_max),
+            "velocity_dirichlet_boundary_mean_delta_mps": float(vd_mean),
+            "velocity_dirichlet_boundary_momentum_delta_n_s": vd_momentum,
+            "pressure_correctable_l2": corr["l2"], "pressure_correctable_max_abs": corr["max_abs"],
+            "pressure_correctable_cell_count": corr["count"],
+            "pressure_correctable_raw_l2": corr_raw["l2"], "pressure_correctable_raw_max_abs": corr_raw["max_abs"],
+            "pressure_fixed_l2": fixed["l2"], "pressure_fixed_max_abs": fixed["max_abs"],
+            "pressure_fixed_cell_count": fixed["count"],
+            "pressure_fixed_raw_l2": fixed_raw["l2"], "pressure_fixed_raw_max_abs": fixed_raw["max_abs"],
+            "interior_pressure_correctable_l2": icorr["l2"],
+            "interior_pressure_correctable_max_abs": icorr["max_abs"],
+            "interior_pressure_correctable_cell_count": icorr["count"],
+            "interior_pressure_correctable_raw_l2": icorr_raw["l2"],
+            "interior_pressure_correctable_raw_max_abs": icorr_raw["max_abs"],
+            "interior_pressure_fixed_l2": ifixed["l2"], "interior_pressure_fixed_max_abs": ifixed["max_abs"],
+            "interior_pressure_fixed_cell_count": ifixed["count"],
+            "interior_pressure_fixed_raw_l2": ifixed_raw["l2"],
+            "interior_pressure_fixed_raw_max_abs": ifixed_raw["max_abs"],
+            "pressure_interface_matrix_diagonal_integral": iface["diagonal_integral"],
+            "pressure_interface_matrix_rhs_integral": iface["rhs_integral"],
+            "pressure_interface_matrix_max_abs_diagonal": iface["max_abs_diagonal"],
+            "pressure_interface_matrix_active_cells": iface["active_cells"],
+            "pressure_interface_matrix_active": bool(iface_active),
+            "pressure_interface_matrix_row_count": 0,
+            "pressure_interface_matrix_row_active_count": 0,
+            "pressure_interface_matrix_row_invalid_count": 0,
+            "pressure_interface_matrix_row_overflow_count": 0,
+            "pressure_interface_matrix_row_diagonal_integral": 0.0,
+            "pressure_interface_matrix_row_diagonal_integral_abs_mismatch": 0.0,
+            "pressure_interface_matrix_row_diagonal_max_abs_density_mismatch": 0.0,
+            "pressure_interface_matrix_row_max_transmissibility": 0.0,
+            "pressure_nullspace_policy": ns_policy,
+            "pressure_nullspace_compatibility_measured": bool(ns_measured),
+            "pressure_nullspace_zero_mean_projection_applied": bool(zero_mean),
+            "pressure_system_anchored_by_interface_matrix": bool(anchored),
+            "pressure_solve_failure_policy": policy,
+            "pressure_solve_failed": bool(failed),
+            "pressure_solve_failure_action": failure_action,
+            "pressure_projection_physical_failure": False,
+            "pressure_projection_physical_failure_reason": "",
+            "pressure_projection_physical_failure_action": "none",
+            "hibm_unreached_incompatible_component_count": 0,
+            "hibm_unreached_component_rhs_mean_max_abs": 0.0,
+            "hibm_unreached_component_rhs_integral_max_abs": 0.0,
+            "cg_project_calls": int(self.last_project_cg_project_calls),
+            "cg_iterations_total": int(self.last_project_cg_iterations_total),
+            "cg_iterations_max": int(self.last_project_cg_iterations_max),
+            "cg_host_residual_checks": int(self.last_project_cg_host_residual_checks),
+            "cg_mean_host_reads": 0,
+            "cg_mean_projection_count": int(self.last_project_cg_mean_projection_count),
+            "cg_unreached_set_mean_projection_count": 0,
+            "hibm_pressure_unreached_cell_count": 0,
+            "hibm_pressure_reachability_converged": True,
+            "cg_unreached_component_count": 0,
+            "cg_unreached_component_overflow": False,
+            "unreached_cells_with_interface_diagonal": 0,
+            "unreached_cells_with_interface_coupling": 0,
+            "unreached_components_with_interface_hits": 0,
+            "hibm_pressure_component_labels_converged": True,
+            "hibm_solid_band_last_marked_increment": 0,
+            "hibm_solid_band_interior_cells": -1,
+            "hibm_solid_band_enclosed_water_cells": -1,
+            "hibm_pressure_filled_cell_count": -1,
+            "cg_restart_count": int(self.last_project_cg_restart_count),
+            "cg_restart_count_measured": bool(self.last_project_cg_restart_count_measured),
+            "cg_restart_policy": str(self.last_project_cg_restart_policy),
+            "cg_initial_relative_residual_max": float(self.last_project_cg_initial_relative_residual_max),
+            "cg_relative_residual_max": float(self.last_project_cg_relative_residual_max),
+            "cg_converged_all": bool(self.last_project_cg_converged_all),
+            "cg_breakdown_count": int(self.last_project_cg_breakdown_count),
+            "cg_breakdown_code": int(self.last_project_cg_breakdown_code),
+            "cg_breakdown_dAd": float(self.last_project_cg_breakdown_dAd),
+            "cg_breakdown": str(self.last_project_cg_breakdown),
+        }
+
+    def predict(self, dt_s: float | None = None, *, advection_scheme: str = "euler") -> None:
+        raise NotImplementedError("the semi-Lagrangian predictor (fluid.py:2447-2527) is not in this build yet")

@@ -1,0 +1,206 @@
+/*
+ * hfsi.h -- C ABI of the MI355X (gfx950) pressure-projection and MPM library
+ * (libhfsi.so).
+ *
+ * Conventions
+ *   - Device buffers are owned by the caller and passed as plain pointers.
+ *     Cell arrays are row-major [i][j][k] with k fastest (nx*ny*nz elements);
+ *     velocity is [i][j][k][3] float32 where component d is the normal flux on
+ *     the LOW d-face of the cell.
+ *   - Every call takes the HIP stream (hipStream_t, passed as void*) it
+ *     enqueues on and returns an int status: 0 ok, < 0 usage/runtime error
+ *     (message from hfsi_last_error()).
+ *   - A handle may be used from one host thread / one stream at a time.
+ *     Calls are asynchronous unless they say they synchronise.
+ *
+ * The Python facade (easyfsi_amd.simulation_core) maps these calls onto the
+ * reference's CartesianFluidSolver / NeoHookeanMpmState methods; each entry
+ * point names the reference function it replaces.
+ */
+#ifndef HFSI_H
+#define HFSI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFSI_OK 0
+#define HFSI_ERR_ARG (-1)
+#define HFSI_ERR_HIP (-2)
+#define HFSI_ERR_STATE (-3)
+
+/* Solve precision modes (bit values; hfsi_fluid_create takes a mask). */
+#define HFSI_PREC_REF64 1 /* float64 CG vectors and level-0 V-cycle, float32 coarse levels */
+#define HFSI_PREC_FP32 2  /* float32 CG vectors and V-cycle, float64 solution and dots      */
+#define HFSI_PREC_MIXED 4 /* fp16-stored V-cycle, float32 CG, float64 solution and dots     */
+
+/* Preconditioners (simulation_core/fluid.py:16 CG_PRECONDITIONER_CHOICES). */
+#define HFSI_PRECOND_AUTO 0
+#define HFSI_PRECOND_JACOBI 1
+#define HFSI_PRECOND_MG 2       /* "fv_multigrid": 2 pre / 24 coarse / 2 post          */
+#define HFSI_PRECOND_MG_LIGHT 3 /* "fv_multigrid_light": 1 / 12 / 1                     */
+
+/* Breakdown kinds reported in hfsi_cg_stats.breakdown_kind
+ * (messages of simulation_core/fluid.py:6668-6797). */
+#define HFSI_BREAKDOWN_NONE 0
+#define HFSI_BREAKDOWN_INITIAL_DOT 1   /* "initial residual/preconditioner dot is not positive finite"   */
+#define HFSI_BREAKDOWN_DEVICE_SCALAR 2 /* "device-side CG scalar update failed"                          */
+#define HFSI_BREAKDOWN_RESTART_DOT 3   /* "restarted residual/preconditioner dot is not positive finite" */
+
+typedef struct hfsi_fluid hfsi_fluid;
+
+/* Per-solve statistics (the reference's last_cg_* host mirrors,
+ * simulation_core/fluid.py:1079-1091). */
+typedef struct hfsi_cg_stats {
+  int32_t iterations;
+  int32_t converged;
+  int32_t breakdown_code; /* device code: 1 non-positive dAd, 2 beta breakdown */
+  int32_t breakdown_kind; /* HFSI_BREAKDOWN_* */
+  int32_t host_residual_checks;
+  int32_t mean_projection_count;
+  int32_t restart_count;
+  int32_t used_multigrid;
+  int32_t true_residual_replacements;
+  int32_t vcycles;
+  double b_norm;
+  double initial_relative_residual;
+  double relative_residual;      /* of the residual the convergence test used      */
+  double breakdown_dAd;
+  double true_relative_residual; /* ||b - A x||_V / ||b||_V in float64 at exit, or -1 */
+} hfsi_cg_stats;
+
+/* Solve options for hfsi_fluid_solve_cg. */
+typedef struct hfsi_cg_options {
+  int32_t precision;          /* one HFSI_PREC_* bit the handle was created with          */
+  int32_t preconditioner;     /* HFSI_PRECOND_*                                           */
+  int32_t max_iterations;     /* >= 1                                                     */
+  int32_t remove_mean;        /* project the closed-domain null space (V-weighted mean)   */
+  int32_t outlet_zmin;        /* Dirichlet p = 0 ghost below k = 0                        */
+  int32_t interface_active;   /* pressure-interface matrix terms present: no MG          */
+  int32_t check_interval;     /* residual check interval (reference: 16)                  */
+  int32_t restart_interval;   /* exact-residual restart interval (reference: 512)         */
+  int32_t true_residual;      /* 1: every check recomputes r = b - A x in float64        */
+  int32_t true_residual_at_exit;
+  double tolerance;           /* relative, on ||r||_V / ||b||_V                            */
+  double rhs_scale;           /* rho / dt                                                 */
+} hfsi_cg_options;
+
+/* ---- library ------------------------------------------------------------ */
+
+int hfsi_version(void);
+const char* hfsi_last_error(void);
+/* 1 if a gfx950 device is visible and the code objects load, else 0 (synchronises). */
+int hfsi_device_ok(void);
+
+/* ---- fluid grid handle ---------------------------------------------------- */
+
+/* Grid + multigrid hierarchy + solver workspaces for the requested precision
+ * modes.  Replaces CartesianFluidSolver.__init__'s field allocation and metric
+ * upload (simulation_core/fluid.py:736-1199): level shapes halve while every
+ * axis is above 3 (fluid.py:692-701), coarse widths are pairwise sums
+ * (fluid.py:704-708), metrics are stored as float32 like the reference's
+ * fields.  Allocates device memory (the only call that does). */
+int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, const double* widths_x,
+                      const double* widths_y, const double* widths_z, const double* origin3,
+                      int32_t precision_mask, void* stream);
+int hfsi_fluid_destroy(hfsi_fluid* h);
+int32_t hfsi_fluid_num_levels(const hfsi_fluid* h);
+/* Level shape (3 ints) of level `level`. */
+int hfsi_fluid_level_shape(const hfsi_fluid* h, int32_t level, int32_t* shape3);
+/* Device bytes held by the handle. */
+size_t hfsi_fluid_device_bytes(const hfsi_fluid* h);
+
+/* Derive the per-level open-face masks, coarse obstacle / velocity-Dirichlet
+ * flags and V-averaged interface diagonals from the caller's fields
+ * (restriction rules of _mg_restrict_residual_kernel, fluid.py:5309-5388).
+ * vd_active / vd_weight / iface_diag may be NULL (all zero). */
+int hfsi_fluid_set_masks(hfsi_fluid* h, const int32_t* obstacle, const int32_t* vd_active,
+                         const float* vd_weight, const double* iface_diag, void* stream);
+
+/* div = sum_d (u_d[c+e_d] - u_d[c]) / w_d[c] over open faces
+ * (_compute_divergence_kernel, fluid.py:3450-3486).  Uses the masks of the
+ * last hfsi_fluid_set_masks. */
+int hfsi_fluid_divergence(hfsi_fluid* h, const float* velocity, float* divergence, int32_t outlet_zmin,
+                          void* stream);
+
+/* q = A p (_fv_laplacian_apply_kernel, fluid.py:4311-4340, plain path). */
+int hfsi_fluid_apply(hfsi_fluid* h, const double* p, double* q, int32_t outlet_zmin, void* stream);
+
+/* Jacobi diagonal (_fv_diagonal_kernel, fluid.py:4280-4309). */
+int hfsi_fluid_diagonal(hfsi_fluid* h, double* diag, int32_t outlet_zmin, void* stream);
+
+/* sum over free cells of V a b (_weighted_dot_kernel, fluid.py:4533-4544);
+ * synchronises and writes the result to *out. */
+int hfsi_fluid_weighted_dot(hfsi_fluid* h, const double* a, const double* b, double* out, void* stream);
+
+/* One V-cycle z = M^-1 r (_apply_fv_multigrid_preconditioner, fluid.py:6085-6262)
+ * at the given precision; r and z are float64 level-0 arrays. */
+int hfsi_fluid_vcycle(hfsi_fluid* h, const double* r, double* z, int32_t precision, int32_t pre,
+                      int32_t coarse, int32_t post, int32_t outlet_zmin, void* stream);
+
+/* Flexible PCG on A p = -(rhs0 - mean) with rhs0 = f32(rhs_scale)*(div - source)
+ * - iface_rhs (_solve_pressure_poisson_fv_cg, fluid.py:6535-6811).  pressure is
+ * the float64 warm start and result.  source / iface_rhs may be NULL.
+ * Synchronises at residual checks; stats may be NULL. */
+int hfsi_fluid_solve_cg(hfsi_fluid* h, const float* divergence, const float* source, const double* iface_rhs,
+                        double* pressure, const hfsi_cg_options* opt, hfsi_cg_stats* stats, void* stream);
+
+/* Standalone solvers on the same operator: `iterations` damped Jacobi sweeps
+ * (fv_jacobi, fluid.py:5977-5990) and Jacobi V-cycles 3/36/3 (+ final sweeps on
+ * graded grids) (fv_multigrid, fluid.py:5992-6083).  pressure is in/out. */
+int hfsi_fluid_solve_fv_jacobi(hfsi_fluid* h, const float* divergence, const float* source,
+                               const double* iface_rhs, double* pressure, double rhs_scale, int32_t iterations,
+                               int32_t outlet_zmin, void* stream);
+int hfsi_fluid_solve_fv_multigrid(hfsi_fluid* h, const float* divergence, const float* source,
+                                  const double* iface_rhs, double* pressure, double rhs_scale, int32_t iterations,
+                                  int32_t cycles, int32_t outlet_zmin, void* stream);
+
+/* Compact uniform-grid Jacobi, the reference project() default
+ * (_pressure_jacobi_kernel + copy, fluid.py:3996-4053, 5842-5860). */
+int hfsi_fluid_solve_legacy_jacobi(hfsi_fluid* h, const float* divergence, const float* source,
+                                   const int32_t* obstacle, double* pressure, double rhs_scale, double inv_dx2,
+                                   double inv_dy2, double inv_dz2, int32_t iterations, int32_t outlet_zmin,
+                                   void* stream);
+
+/* u_d -= (dt/rho) w (p_c - p_{c-e_d}) / d_d[c] (_subtract_pressure_gradient_kernel,
+ * fluid.py:5538-5582).  vd_* may be NULL. */
+int hfsi_fluid_subtract_gradient(hfsi_fluid* h, float* velocity, const double* pressure, const int32_t* vd_active,
+                                 const float* vd_weight, const float* vd_value, float dt_over_rho,
+                                 int32_t outlet_zmin, void* stream);
+
+/* Zero the normal flux of the low wall faces
+ * (_apply_closed_boundary_no_normal_flow_kernel, fluid.py:5528-5536). */
+int hfsi_fluid_wall_clamp(hfsi_fluid* h, float* velocity, int32_t outlet_zmin, void* stream);
+
+/* Divergence report reductions (_divergence_stats_kernel,
+ * _divergence_residual_stats_kernel, _divergence_final_report_kernel,
+ * fluid.py:3493-3663): 18 slots x (max, sum of squares, count) written to
+ * out54 (float64, host memory); synchronises.  Slots 0/1 all free cells
+ * (raw/residual), 2/3 interior, 4/5 stencil touching a velocity-Dirichlet
+ * row, 6/7 the rest, 8/9 pressure-correctable, 10/11 fixed, 12-15 their
+ * interior parts, 16/17 unused here (HIBM unreached set).  vd_active and
+ * source and iface_div may be NULL. */
+int hfsi_fluid_divergence_report(hfsi_fluid* h, const float* divergence, const float* source,
+                                 const double* iface_div, const int32_t* vd_active, int32_t outlet_zmin,
+                                 double* out54, void* stream);
+
+/* Live kernel timing (no reference counterpart; measurement support):
+ * while enabled, HIP events are recorded on the launch stream around every
+ * launch of the selected level-0 kernel and their elapsed times summed.
+ * kernel_id 0 disables. */
+#define HFSI_TIMER_OFF 0
+#define HFSI_TIMER_L0_SWEEP 1         /* level-0 weighted-Jacobi sweep (not the first, not the prolongating one) */
+#define HFSI_TIMER_SPMV_DOT 2         /* q = A d with <d, q>_V                                                   */
+#define HFSI_TIMER_L0_RESTRICT 3      /* level-0 residual + restriction                                          */
+#define HFSI_TIMER_L0_PROLONG_SWEEP 4 /* level-0 prolongation fused into the first post-smoothing sweep          */
+int hfsi_fluid_kernel_timer(hfsi_fluid* h, int32_t kernel_id);
+/* Synchronises on the recorded events; totals since the last enable. */
+int hfsi_fluid_kernel_timer_read(hfsi_fluid* h, int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HFSI_H */

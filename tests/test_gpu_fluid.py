@@ -1,0 +1,333 @@
+"""GPU parity of the pressure-projection path (libhfsi through its C ABI)
+against the NumPy oracle, on seeded inputs small enough for the oracle.
+
+Tolerances (float64 oracle vs device):
+  divergence (float32 arithmetic in both)      : <= 1 ulp
+  operator, float64                             : rtol 1e-12
+  V-cycle, float64 level 0 + float32 levels     : 1e-6 of max |z|
+  PCG to 1e-10 (ref64)                          : 1e-7 of max |p|, same iteration count +-1
+  PCG fp32 vectors to 1e-8 (true residual)      : 1e-5 of max |p|
+"""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fluid as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _solver(widths, precision="ref64", rho=1000.0, dt=1e-3):
+    from easyfsi_amd.simulation_core.fluid import CartesianFluidSolver
+    from easyfsi_amd.simulation_core.grid import CartesianGrid, FluidDomainSpec
+
+    g = CartesianGrid(bounds_min_m=(0.0, 0.0, 0.0), cell_widths_x_m=tuple(widths[0]),
+                      cell_widths_y_m=tuple(widths[1]), cell_widths_z_m=tuple(widths[2]))
+    spec = FluidDomainSpec(bounds_min_m=g.bounds_min_m, bounds_max_m=g.bounds_max_m, grid_nodes=None,
+                           density_kgm3=rho, viscosity_pa_s=1e-3, dt_s=dt, cartesian_grid=g)
+    return CartesianFluidSolver(spec, solver_precision=precision)
+
+
+def _graded(shape, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.uniform(0.6, 1.4, size=n) / n for n in shape]
+
+
+def _obstacle(shape, seed, frac=0.08):
+    rng = np.random.default_rng(seed + 100)
+    o = (rng.random(shape) < frac).astype(np.int32)
+    return o
+
+
+def _t(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ok(s, rc):
+    assert rc == 0, s._lib.hfsi_last_error().decode()
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+def test_divergence_matches_oracle(outlet):
+    _say(f"divergence outlet={outlet}")
+    shape = (12, 10, 9)
+    w = _graded(shape, 1)
+    obst = _obstacle(shape, 1)
+    rng = np.random.default_rng(2)
+    u = rng.normal(size=shape + (3,)).astype(np.float32)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s.velocity.from_numpy(u)
+    s.compute_divergence(pressure_outlet_zmin=outlet)
+    got = s.divergence.to_numpy()
+    want = F.divergence(u, obst, w, outlet)
+    np.testing.assert_array_max_ulp(got, want, maxulp=1)
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+def test_operator_matches_oracle_with_dirichlet_rows_and_interface_diagonal(outlet):
+    _say(f"operator outlet={outlet}")
+    shape = (11, 9, 13)
+    w = _graded(shape, 3)
+    obst = _obstacle(shape, 3)
+    rng = np.random.default_rng(4)
+    vda = (rng.random(shape) < 0.2).astype(np.int32)
+    vdw = rng.uniform(-0.2, 1.2, size=shape).astype(np.float32)
+    diag = np.where(rng.random(shape) < 0.1, rng.uniform(0, 50, size=shape), 0.0)
+    p = rng.normal(size=shape)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s.velocity_dirichlet_boundary_active.from_numpy(vda)
+    s.velocity_dirichlet_boundary_projection_weight.from_numpy(vdw)
+    s.pressure_interface_matrix_diagonal.from_numpy(diag)
+    s._set_masks()
+    pt = _t(p, torch.float64)
+    q = torch.zeros_like(pt)
+    _ok(s, s._lib.hfsi_fluid_apply(s._handle, pt.data_ptr(), q.data_ptr(), int(outlet), _stream()))
+    lvl = F.make_level(w, obst, vd_active=vda, vd_weight=vdw, iface_diag=diag)
+    want = F.fv_apply(p, lvl, outlet)
+    np.testing.assert_allclose(q.cpu().numpy(), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    d = torch.zeros_like(pt)
+    _ok(s, s._lib.hfsi_fluid_diagonal(s._handle, d.data_ptr(), int(outlet), _stream()))
+    np.testing.assert_allclose(d.cpu().numpy(), F.fv_diagonal(lvl, outlet), rtol=1e-12)
+    out = ctypes.c_double()
+    _ok(s, s._lib.hfsi_fluid_weighted_dot(s._handle, pt.data_ptr(), q.data_ptr(), ctypes.byref(out), _stream()))
+    assert out.value == pytest.approx(F.weighted_dot(p, want, lvl), rel=1e-12)
+
+
+@pytest.mark.parametrize("shape,graded,outlet", [((20, 18, 16), True, False), ((16, 16, 16), False, True),
+                                                  ((40, 36, 33), True, True)])
+def test_vcycle_ref64_matches_oracle(shape, graded, outlet):
+    _say(f"vcycle shape={shape} graded={graded} outlet={outlet}")
+    w = _graded(shape, 5) if graded else [np.full(n, 1.0 / n) for n in shape]
+    obst = _obstacle(shape, 5, 0.05)
+    rng = np.random.default_rng(6)
+    r = np.where(obst == 0, rng.normal(size=shape), 0.0)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    rt = _t(r, torch.float64)
+    z = torch.zeros_like(rt)
+    _ok(s, s._lib.hfsi_fluid_vcycle(s._handle, rt.data_ptr(), z.data_ptr(), 1, 2, 24, 2, int(outlet), _stream()))
+    h = F.build_hierarchy(w, obst)
+    want = F.vcycle(r, h, outlet)
+    got = z.cpu().numpy()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-6 * np.abs(want).max())
+
+
+def _cg(s, div, precision, tol, max_it, outlet, remove_mean, precond=2, true_exit=0, check=16, true_res=0):
+    from easyfsi_amd import _lib
+
+    opt = _lib.CgOptions()
+    opt.precision = precision
+    opt.preconditioner = precond
+    opt.max_iterations = max_it
+    opt.remove_mean = int(remove_mean)
+    opt.outlet_zmin = int(outlet)
+    opt.interface_active = 0
+    opt.check_interval = check
+    opt.restart_interval = 512
+    opt.true_residual = true_res
+    opt.true_residual_at_exit = true_exit
+    opt.tolerance = tol
+    opt.rhs_scale = s.rho / s.dt
+    st = _lib.CgStats()
+    s.divergence.from_numpy(div)
+    _ok(s, s._lib.hfsi_fluid_solve_cg(s._handle, s.divergence.ptr, s.volume_source_s.ptr, None, s.pressure.ptr,
+                                      ctypes.byref(opt), ctypes.byref(st), _stream()))
+    return st
+
+
+def _case(shape, seed, outlet):
+    w = _graded(shape, seed)
+    obst = _obstacle(shape, seed, 0.05)
+    rng = np.random.default_rng(seed + 7)
+    div = np.where(obst == 0, rng.normal(size=shape), 0.0).astype(np.float32)
+    return w, obst, div
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+def test_pcg_ref64_matches_oracle(outlet):
+    _say(f"pcg ref64 outlet={outlet}")
+    shape = (24, 20, 18)
+    w, obst, div = _case(shape, 11, outlet)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    st = _cg(s, div, 1, 1e-10, 300, outlet, not outlet, true_exit=1)
+    assert st.converged == 1, st.as_dict()
+    h = F.build_hierarchy(w, obst)
+    rhs0 = F.level0_rhs(div, np.zeros(shape, np.float32), None, h.levels[0], s.rho / s.dt)
+    ref = F.pcg(rhs0, np.zeros(shape), h, outlet=outlet, tolerance=1e-10, max_iters=300,
+                preconditioner="fv_multigrid", remove_mean=not outlet)
+    assert ref.converged
+    assert abs(st.iterations - ref.iterations) <= 1, (st.iterations, ref.iterations)
+    got = s.pressure.to_numpy()
+    np.testing.assert_allclose(got, ref.pressure, rtol=0, atol=1e-7 * np.abs(ref.pressure).max())
+    assert st.true_relative_residual <= 2e-10
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+def test_pcg_fp32_reaches_true_residual(outlet):
+    _say(f"pcg fp32 outlet={outlet}")
+    shape = (32, 28, 24)
+    w, obst, div = _case(shape, 13, outlet)
+    s = _solver(w, precision="fp32")
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    st = _cg(s, div, 2, 1e-8, 400, outlet, not outlet, true_exit=1, true_res=1)
+    assert st.converged == 1, st.as_dict()
+    assert st.true_relative_residual <= 1e-8, st.as_dict()
+    h = F.build_hierarchy(w, obst)
+    rhs0 = F.level0_rhs(div, np.zeros(shape, np.float32), None, h.levels[0], s.rho / s.dt)
+    ref = F.pcg(rhs0, np.zeros(shape), h, outlet=outlet, tolerance=1e-11, max_iters=400,
+                preconditioner="fv_multigrid", remove_mean=not outlet)
+    np.testing.assert_allclose(s.pressure.to_numpy(), ref.pressure, rtol=0, atol=1e-5 * np.abs(ref.pressure).max())
+
+
+def test_pcg_jacobi_preconditioner_and_zero_rhs():
+    _say("pcg jacobi + zero rhs")
+    shape = (10, 12, 8)
+    w, obst, div = _case(shape, 17, False)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    st = _cg(s, div, 1, 1e-9, 500, False, True, precond=1)
+    h = F.build_hierarchy(w, obst)
+    rhs0 = F.level0_rhs(div, np.zeros(shape, np.float32), None, h.levels[0], s.rho / s.dt)
+    ref = F.pcg(rhs0, np.zeros(shape), h, outlet=False, tolerance=1e-9, max_iters=500, preconditioner="jacobi")
+    assert st.converged and ref.converged
+    assert abs(st.iterations - ref.iterations) <= 1
+    np.testing.assert_allclose(s.pressure.to_numpy(), ref.pressure, atol=1e-6 * np.abs(ref.pressure).max())
+    s.pressure.fill(3.0)
+    st = _cg(s, np.zeros(shape, np.float32), 1, 1e-6, 8, False, True)
+    assert st.converged and st.initial_relative_residual == 0.0
+    assert np.abs(s.pressure.to_numpy()).max() == 0.0
+
+
+def test_gradient_and_wall_clamp_match_oracle():
+    _say("gradient + clamp")
+    shape = (9, 8, 10)
+    w = _graded(shape, 19)
+    obst = _obstacle(shape, 19)
+    rng = np.random.default_rng(20)
+    p = rng.normal(size=shape)
+    u = rng.normal(size=shape + (3,)).astype(np.float32)
+    for outlet in (False, True):
+        s = _solver(w)
+        s.obstacle.from_numpy(obst)
+        s.pressure.from_numpy(p)
+        s.velocity.from_numpy(u)
+        s._set_masks()
+        s._subtract_pressure_gradient(0.37, outlet)
+        want = F.subtract_pressure_gradient(u, p, obst, w, 0.37, outlet)
+        np.testing.assert_allclose(s.velocity.to_numpy(), want, rtol=1e-6, atol=1e-6)
+        s._apply_closed_boundary_no_normal_flow_kernel(int(outlet))
+        np.testing.assert_allclose(s.velocity.to_numpy(), F.closed_wall_clamp(want, outlet), rtol=1e-6, atol=1e-6)
+
+
+def test_standalone_solvers_match_oracle():
+    _say("fv_jacobi / fv_multigrid / legacy jacobi")
+    shape = (16, 14, 12)
+    w, obst, div = _case(shape, 23, True)
+    h = F.build_hierarchy(w, obst)
+    rhs0 = F.level0_rhs(div, np.zeros(shape, np.float32), None, h.levels[0], 1000.0 / 1e-3)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s.divergence.from_numpy(div)
+    s._set_masks()
+    p0 = np.random.default_rng(24).normal(size=shape) * 10
+    s.pressure.from_numpy(p0)
+    s._solve_pressure_poisson_with_solver(iterations=7, rhs_scale=1e6, inv_dx2=1, inv_dy2=1, inv_dz2=1,
+                                          pressure_outlet_zmin=True, pressure_solver="fv_jacobi",
+                                          multigrid_cycles=None, cg_tolerance=1e-6)
+    want = F.fv_jacobi_solve(rhs0, p0, h, outlet=True, iterations=7)
+    np.testing.assert_allclose(s.pressure.to_numpy(), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    s.pressure.from_numpy(p0)
+    s._solve_pressure_poisson_with_solver(iterations=4, rhs_scale=1e6, inv_dx2=1, inv_dy2=1, inv_dz2=1,
+                                          pressure_outlet_zmin=True, pressure_solver="fv_multigrid",
+                                          multigrid_cycles=3, cg_tolerance=1e-6)
+    want = F.fv_multigrid_solve(rhs0, p0, h, outlet=True, iterations=4, cycles=3)
+    np.testing.assert_allclose(s.pressure.to_numpy(), want, rtol=0, atol=1e-6 * np.abs(want).max())
+    # legacy compact Jacobi on a uniform grid
+    n = (10, 10, 10)
+    s2 = _solver([np.full(10, 0.1)] * 3)
+    o2 = _obstacle(n, 25)
+    d2 = np.random.default_rng(26).normal(size=n).astype(np.float32)
+    s2.obstacle.from_numpy(o2)
+    s2.divergence.from_numpy(d2)
+    s2._set_masks()
+    s2._solve_pressure_poisson_with_solver(iterations=5, rhs_scale=1e6, inv_dx2=100.0, inv_dy2=100.0,
+                                           inv_dz2=100.0, pressure_outlet_zmin=True, pressure_solver="jacobi",
+                                           multigrid_cycles=None, cg_tolerance=1e-6)
+    want = F.legacy_jacobi_solve(np.zeros(n), d2, np.zeros(n, np.float32), o2, rhs_scale=1e6,
+                                 spacing=(0.1, 0.1, 0.1), outlet=True, iterations=5)
+    np.testing.assert_allclose(s2.pressure.to_numpy(), want, rtol=1e-6, atol=1e-9 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+def test_project_end_to_end_matches_oracle(outlet):
+    _say(f"project outlet={outlet}")
+    shape = (20, 16, 18)
+    w = _graded(shape, 29)
+    obst = _obstacle(shape, 29, 0.04)
+    rng = np.random.default_rng(30)
+    u = rng.normal(size=shape + (3,)).astype(np.float32) * 0.1
+    s = _solver(w, dt=1e-3)
+    s.obstacle.from_numpy(obst)
+    s.velocity.from_numpy(u)
+    rep = s.project(iterations=300, pressure_outlet_zmin=outlet, pressure_solver="fv_cg",
+                    cg_tolerance=1e-10, cg_preconditioner="fv_multigrid", reset_pressure=True)
+    u_ref, p_ref, div_ref, info = F.project_plain(u, np.zeros(shape), obst, w, rho=1000.0, dt=1e-3, solver="fv_cg",
+                                                  iterations=300, tolerance=1e-10, preconditioner="fv_multigrid",
+                                                  outlet=outlet, reset_pressure=True)
+    assert rep["cg_converged_all"] and info["cg"].converged
+    if not outlet:  # outlet runs may take the cleanup loop; compare the first solve only when closed
+        np.testing.assert_allclose(s.pressure.to_numpy(), p_ref, atol=1e-7 * np.abs(p_ref).max())
+        np.testing.assert_allclose(s.velocity.to_numpy(), u_ref, rtol=1e-5, atol=1e-6 * np.abs(u_ref).max())
+        assert rep["pre_projection_l2"] == pytest.approx(info["pre"]["l2"], rel=1e-5)
+        assert rep["l2"] == pytest.approx(info["final"]["l2"], rel=1e-2, abs=1e-9)
+        assert rep["l2"] < 1e-4 * rep["pre_projection_l2"]
+    else:
+        # the outlet's no-backflow clamp leaves divergence on k = 0 cells by design
+        # (fluid.py:5761-5765); away from the outlet the field is projected
+        s.compute_divergence(pressure_outlet_zmin=True)
+        d = s.divergence.to_numpy()[:, :, 2:]
+        free = obst[:, :, 2:] == 0
+        assert np.abs(d[free]).max() < 1e-4 * rep["pre_projection_max_abs"]
+
+
+def test_report_matches_oracle_stats():
+    _say("divergence report")
+    shape = (9, 10, 11)
+    w = _graded(shape, 31)
+    obst = _obstacle(shape, 31)
+    rng = np.random.default_rng(32)
+    div = rng.normal(size=shape).astype(np.float32)
+    src = rng.normal(size=shape).astype(np.float32) * 0.1
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s.divergence.from_numpy(div)
+    s.volume_source_s.from_numpy(src)
+    s._set_masks()
+    for interior in (False, True):
+        got = s.divergence_stats(interior_only=interior)
+        want = F.divergence_l2_stats(div, src, obst, interior, residual=False)
+        assert got["l2"] == pytest.approx(want["l2"], rel=1e-12)
+        assert got["max_abs"] == pytest.approx(want["max_abs"], rel=1e-12)
+        got = s.divergence_residual_stats(interior_only=interior)
+        want = F.divergence_l2_stats(div, src, obst, interior, residual=True)
+        assert got["l2"] == pytest.approx(want["l2"], rel=1e-12)
