@@ -1,0 +1,305 @@
+#!/usr/bin/env python3
+"""Projection-solve throughput on one MI355X.
+
+Workload (GPU scale): 256^3 graded MAC grid on the
+unit cube -- a core of 128 equal cells of width h and 64 cells on each side of
+width h*1.04^k (k = 1..64), h = 1/(128 + 2*sum 1.04^k) -- with a sphere
+obstacle of radius 0.06 at the centre; right-hand sides are the divergence of
+seeded random Fourier velocity fields (|k| <= 16 in cell-index space, one
+field per step).  A step is one pressure solve from x0 = 0 to a float64 TRUE
+volume-weighted relative residual ||b - A x||_V / ||b||_V <= 1e-8.
+
+Variants (all on the GPU, same grid and right-hand sides):
+  plain_fp32 : the reference's V-cycle (Jacobi 2/24/2, x0.7 prolongation)
+               in float32 inside flexible CG with float32 vectors, float64
+               solution, dots and true-residual replacement every 16 its.
+  ref64      : the same algorithm at the reference's precision (float64
+               vectors, float32 coarse levels), recursive-residual test.
+
+Prints one JSON line (rank 0).  --gpus above 1 is refused: this build
+measures one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "projection solves/s at 256³ to resid 1e-8: learned+mixed vs plain MGPCG"
+HBM_PEAK_GBS = 8000.0
+
+
+def graded_axis(n_core: int, n_side: int, growth: float) -> np.ndarray:
+    from easyfsi_amd.simulation_core.grid import explicit_graded_axis
+
+    raw = np.asarray(explicit_graded_axis(n_core, 1.0, n_side, growth))
+    return raw / raw.sum()
+
+
+def make_solver(size: int, precision: str):
+    from easyfsi_amd.simulation_core.fluid import CartesianFluidSolver
+    from easyfsi_amd.simulation_core.grid import CartesianGrid, FluidDomainSpec
+
+    w = graded_axis(size // 2, size // 4, 1.04)
+    g = CartesianGrid(bounds_min_m=(0.0, 0.0, 0.0), cell_widths_x_m=tuple(w), cell_widths_y_m=tuple(w),
+                      cell_widths_z_m=tuple(w))
+    spec = FluidDomainSpec(bounds_min_m=g.bounds_min_m, bounds_max_m=g.bounds_max_m, grid_nodes=None,
+                           density_kgm3=1000.0, viscosity_pa_s=1e-3, dt_s=1e-3, cartesian_grid=g)
+    s = CartesianFluidSolver(spec, solver_precision=precision)
+    s.mark_sphere_obstacle((0.5, 0.5, 0.5), 0.06)
+    s._set_masks()
+    return s, w
+
+
+def fourier_velocity(shape, kmax: int, seed: int):
+    """Velocity with N(0,1) complex coefficients on 0 < |k| <= kmax (index space), unit RMS per component."""
+    import torch
+
+    nx, ny, nz = shape
+    dev = torch.device("cuda")
+    kx = torch.fft.fftfreq(nx, d=1.0 / nx, device=dev)
+    ky = torch.fft.fftfreq(ny, d=1.0 / ny, device=dev)
+    kz = torch.fft.rfftfreq(nz, d=1.0 / nz, device=dev)
+    k2 = kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2
+    mask = ((k2 > 0) & (k2 <= kmax * kmax)).to(torch.float32)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    out = torch.empty(shape + (3,), dtype=torch.float32, device=dev)
+    for d in range(3):
+        re = torch.randn(mask.shape, generator=gen, device=dev)
+        im = torch.randn(mask.shape, generator=gen, device=dev)
+        f = torch.fft.irfftn(torch.complex(re * mask, im * mask), s=shape)
+        out[..., d] = (f / f.pow(2).mean().sqrt()).to(torch.float32)
+    return out
+
+
+def make_rhs(solver, count: int, seed0: int):
+    """Divergence fields (float32) of `count` seeded velocity fields, via libhfsi."""
+    import torch
+
+    divs = []
+    for s in range(count):
+        solver.velocity.tensor.copy_(fourier_velocity(solver.grid.grid_nodes, 16, seed0 + s))
+        solver._divergence_only(False)
+        divs.append(solver.divergence.tensor.clone())
+    torch.cuda.synchronize()
+    return divs
+
+
+VARIANTS = {
+    "plain_fp32": dict(precision="fp32", true_residual=1, dtype="f32"),
+    "ref64": dict(precision="ref64", true_residual=0, dtype="f64"),
+}
+
+
+def run_variant(name, solver, divs, steps, warmup, timer_kernel="l0_sweep"):
+    import ctypes
+
+    import torch
+
+    from easyfsi_amd import _lib
+
+    cfg = VARIANTS[name]
+    L = solver._lib
+    opt = _lib.CgOptions()
+    opt.precision = {"fp32": _lib.HFSI_PREC_FP32, "ref64": _lib.HFSI_PREC_REF64}[cfg["precision"]]
+    opt.preconditioner = _lib.PRECONDITIONERS["fv_multigrid"]
+    opt.max_iterations = 500
+    opt.remove_mean = 1
+    opt.outlet_zmin = 0
+    opt.interface_active = 0
+    opt.check_interval = 16
+    opt.restart_interval = 512
+    opt.true_residual = cfg["true_residual"]
+    opt.true_residual_at_exit = 1
+    opt.tolerance = 1e-8
+    opt.rhs_scale = solver.rho / solver.dt
+    stream = torch.cuda.current_stream().cuda_stream
+    stats = _lib.CgStats()
+    src = solver.volume_source_s.ptr
+    p = solver.pressure
+
+    def solve(div):
+        p.tensor.zero_()
+        _lib.check(L.hfsi_fluid_solve_cg(solver._handle, div.data_ptr(), src, None, p.ptr, ctypes.byref(opt),
+                                         ctypes.byref(stats), stream), "hfsi_fluid_solve_cg")
+        return stats.as_dict()
+
+    for s in range(warmup):
+        solve(divs[s % len(divs)])
+    torch.cuda.synchronize()
+    _lib.check(L.hfsi_fluid_kernel_timer(solver._handle, _lib.TIMER_KERNELS[timer_kernel]), "timer")
+    per_solve, records = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        a = time.perf_counter()
+        rec = solve(divs[s % len(divs)])
+        torch.cuda.synchronize()
+        per_solve.append(time.perf_counter() - a)
+        records.append(rec)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    launches = ctypes.c_int64()
+    ms = ctypes.c_double()
+    _lib.check(L.hfsi_fluid_kernel_timer_read(solver._handle, ctypes.byref(launches), ctypes.byref(ms)), "timer")
+    _lib.check(L.hfsi_fluid_kernel_timer(solver._handle, 0), "timer")
+    its = [r["iterations"] for r in records]
+    true_res = [r["true_relative_residual"] for r in records]
+    return {
+        "total_s": total,
+        "solves_per_s": steps / total,
+        "median_solve_ms": 1e3 * statistics.median(per_solve),
+        "iterations_median": statistics.median(its),
+        "iterations_max": max(its),
+        "true_rel_residual_max": max(true_res),
+        "all_converged": all(r["converged"] for r in records) and max(true_res) <= 1e-8,
+        "vcycles_total": sum(r["vcycles"] for r in records),
+        "timer_launches": int(launches.value),
+        "timer_ms_total": float(ms.value),
+        "dtype": cfg["dtype"],
+    }
+
+
+def cpu_baseline(solver, w, div, iterations_needed: int, budget_s: float = 15.0):
+    """C + OpenMP oracle of the same plain algorithm (float64, reference precision) on the
+    host cores: times setup + 1 iteration and setup + 1 + m iterations on the same grid,
+    obstacle and right-hand side, and extrapolates to `iterations_needed`."""
+    from oracle import c_oracle
+
+    obst = solver.obstacle.to_numpy()
+    scale = np.float32(solver.rho / solver.dt)
+    d = div.detach().cpu().numpy()
+    rhs0 = np.where(obst == 1, 0.0, (scale * d).astype(np.float32).astype(np.float64))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    t = time.perf_counter()
+    c_oracle.pcg([w, w, w], obst, rhs0, max_iters=1, tolerance=0.0)
+    t1 = time.perf_counter() - t
+    m = 2
+    t = time.perf_counter()
+    c_oracle.pcg([w, w, w], obst, rhs0, max_iters=1 + m, tolerance=0.0)
+    t2 = time.perf_counter() - t
+    per_it = max((t2 - t1) / m, 1e-9)
+    # grow the sample towards the budget
+    if t1 + t2 < budget_s / 2:
+        m = int(max(2, min(40, (budget_s - t1 - t2) / per_it)))
+        t = time.perf_counter()
+        c_oracle.pcg([w, w, w], obst, rhs0, max_iters=1 + m, tolerance=0.0)
+        t2 = time.perf_counter() - t
+        per_it = max((t2 - t1) / m, 1e-9)
+    setup = max(t1 - per_it, 0.0)
+    solve_s = setup + per_it * iterations_needed
+    return {
+        "value": 1.0 / solve_s,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"C+OpenMP oracle (oracle/c/mgpcg.c), float64 reference-precision MGPCG on the same 256^3 "
+                   f"graded grid and first RHS: setup+1 iteration ({t1:.2f} s) and setup+{1 + m} iterations "
+                   f"({t2:.2f} s); {per_it:.3f} s/iteration x {iterations_needed} iterations + {setup:.2f} s setup"),
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--variants", default="plain_fp32,ref64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding)",
+              file=sys.stderr)
+        return 2
+    if args.size % 4 or args.size < 16:
+        print("--size must be a multiple of 4 and >= 16", file=sys.stderr)
+        return 2
+    import torch
+
+    variants = [v for v in args.variants.split(",") if v]
+    for v in variants:
+        if v not in VARIANTS:
+            print(f"unknown variant {v!r}; choose from {sorted(VARIANTS)}", file=sys.stderr)
+            return 2
+    print(f"[bench] building {args.size}^3 graded grid", file=sys.stderr, flush=True)
+    solver, w = make_solver(args.size, "fp32")
+    n = solver.nx * solver.ny * solver.nz
+    divs = make_rhs(solver, max(args.steps, 1), seed0=1000)
+    results = {}
+    for v in variants:
+        print(f"[bench] variant {v}: {args.warmup} warmup + {args.steps} timed solves", file=sys.stderr, flush=True)
+        results[v] = run_variant(v, solver, divs, args.steps, args.warmup)
+        print(f"[bench] {v}: {results[v]['solves_per_s']:.2f} solves/s, "
+              f"{results[v]['iterations_median']} its, true res {results[v]['true_rel_residual_max']:.2e}",
+              file=sys.stderr, flush=True)
+    head_name = "plain_fp32" if "plain_fp32" in results else variants[0]
+    head = results[head_name]
+    # roofline of the level-0 smoother sweep of the headline variant
+    bytes_per_cell = 13.0 if head["dtype"] == "f32" else 25.0  # p + rhs(-r) read, out write, 1 B mask
+    avg_s = head["timer_ms_total"] / max(head["timer_launches"], 1) / 1e3
+    achieved = bytes_per_cell * n / avg_s / 1e9 if head["timer_launches"] else 0.0
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline (C oracle)", file=sys.stderr, flush=True)
+        its_needed = int(results.get("ref64", head)["iterations_median"])
+        cpu = cpu_baseline(solver, w, divs[0], its_needed, args.cpu_budget)
+    line = {
+        "metric": METRIC,
+        "value": round(head["solves_per_s"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * head["total_s"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head["dtype"],
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.size}^3 graded MAC grid ({args.size // 2}^3 core, {args.size // 4} cells/side at "
+                         f"growth 1.04), sphere obstacle r=0.06, RHS = divergence of seeded Fourier velocity "
+                         f"|k|<=16; {head_name} MGPCG from x0=0 to float64 true relative residual 1e-8"),
+            "cells": n,
+            "variant": head_name,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "level-0 weighted-Jacobi sweep (k_jacobi)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_cell": bytes_per_cell,
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "launches": head["timer_launches"],
+        },
+        "cpu_baseline": cpu,
+        "variants": {k: {kk: (float(f"{vv:.6g}") if isinstance(vv, float) else vv) for kk, vv in r.items()}
+                     for k, r in results.items()},
+    }
+    if "ref64" in results and "plain_fp32" in results:
+        line["speedup_plain_fp32_vs_ref64"] = round(results["plain_fp32"]["solves_per_s"]
+                                                    / results["ref64"]["solves_per_s"], 3)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

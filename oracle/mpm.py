@@ -1,0 +1,264 @@
+"""NumPy restatement of the neo-Hookean MLS/APIC MPM step (test oracle).
+
+TEST INFRASTRUCTURE ONLY -- never imported by the product package.
+
+Follows simulation_core/neo_hookean_mpm.py:
+  lattice init        _initialize_box_kernel                 144-229
+  grid coordinate     _particle_grid_coordinate               652-660
+  stencil bounds      _particle_grid_stencil_out_of_bounds    662-671
+  surface geometry    _update_surface_geometry_from_deformation 673-685
+  B-spline weights    _weights                                687-693
+  step                _step_kernel                            695-959
+  report              report                                  1016-1066
+
+Everything is float64 (the device state is float32; inputs are taken from it).
+The SVD is numpy.linalg.svd put into the rotation convention the clamp logic
+expects (U, V proper rotations, the sign of det F on the smallest singular
+value), which is the convention of Taichi's 3x3 ti.svd.  For degenerate or
+inverted F with repeated singular values that convention does not fix U and
+V uniquely: parity there is unpinned beyond the invariants the reference's
+tests check (clamp count, det > 0, singular values in [1e-2, 1e2], |J| = 1).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+MIN_SIGMA = 1.0e-2
+MAX_SIGMA = 1.0e2
+
+
+@dataclass
+class MpmState:
+    x: np.ndarray          # (P, 3)
+    v: np.ndarray          # (P, 3)
+    C: np.ndarray          # (P, 3, 3)
+    F: np.ndarray          # (P, 3, 3)
+    mass: np.ndarray       # (P,)
+    volume: np.ndarray     # (P,)
+    rest_x: np.ndarray     # (P, 3)
+    fixed: np.ndarray      # (P,) int
+    region: np.ndarray     # (P,) int
+    f_ext: np.ndarray      # (P, 3)
+    normal: np.ndarray     # (P, 3)
+    rest_normal: np.ndarray
+    area: np.ndarray       # (P,)
+    rest_area: np.ndarray
+
+    def copy(self) -> "MpmState":
+        return MpmState(**{k: np.array(v, copy=True) for k, v in self.__dict__.items()})
+
+
+def initialize_box(counts, box_min, box_max, density) -> MpmState:
+    """Cell-centred particle lattice, V = box/count, m = rho V (neo_hookean_mpm.py:144-229), float32 positions."""
+    nxp, nyp, nzp = (int(c) for c in counts)
+    P = nxp * nyp * nzp
+    f = np.float32
+    lo = np.asarray(box_min, f)
+    hi = np.asarray(box_max, f)
+    vol = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2])
+    pv = f(vol / max(f(P), f(1.0)))
+    p = np.arange(P)
+    ix, iy, iz = p % nxp, (p // nxp) % nyp, p // (nxp * nyp)
+    x = np.stack([lo[0] + (ix.astype(f) + f(0.5)) * (hi[0] - lo[0]) / f(nxp),
+                  lo[1] + (iy.astype(f) + f(0.5)) * (hi[1] - lo[1]) / f(nyp),
+                  lo[2] + (iz.astype(f) + f(0.5)) * (hi[2] - lo[2]) / f(nzp)], axis=1).astype(f)
+    eye = np.broadcast_to(np.eye(3), (P, 3, 3)).copy()
+    z3 = np.zeros((P, 3))
+    return MpmState(x=x.astype(np.float64), v=z3.copy(), C=np.zeros((P, 3, 3)), F=eye,
+                    mass=np.full(P, float(f(density) * pv)), volume=np.full(P, float(pv)),
+                    rest_x=x.astype(np.float64), fixed=np.zeros(P, np.int32), region=np.zeros(P, np.int32),
+                    f_ext=z3.copy(), normal=z3.copy(), rest_normal=z3.copy(), area=np.zeros(P), rest_area=np.zeros(P))
+
+
+def weights(fx: np.ndarray) -> np.ndarray:
+    """Quadratic B-spline weights at offsets 0, 1, 2 (neo_hookean_mpm.py:687-693); (3, ...)."""
+    return np.stack([0.5 * (1.5 - fx) ** 2, 0.75 - (fx - 1.0) ** 2, 0.5 * (fx - 0.5) ** 2])
+
+
+def svd_rotations(F: np.ndarray):
+    """F = U diag(s) V^T with U, V in SO(3), s sorted by magnitude, sign(det F) on s[2]."""
+    U, s, Vt = np.linalg.svd(F)
+    V = np.swapaxes(Vt, -1, -2).copy()
+    U = U.copy()
+    s = s.copy()
+    du = np.linalg.det(U) < 0
+    U[du, :, 2] *= -1.0
+    s[du, 2] *= -1.0
+    dv = np.linalg.det(V) < 0
+    V[dv, :, 2] *= -1.0
+    s[dv, 2] *= -1.0
+    return U, s, V
+
+
+def clamp_deformation(F: np.ndarray):
+    """SVD clamp and reflection fix of the P2G loop (neo_hookean_mpm.py:720-746).
+
+    Returns (Fp, clamped) with clamped a bool array."""
+    raw_J = np.linalg.det(F)
+    U, s, V = svd_rotations(F)
+    sc = np.clip(np.abs(s), MIN_SIGMA, MAX_SIGMA)
+    clamped = np.any(np.abs(sc - s) > 1.0e-6, axis=1)
+    refl = np.linalg.det(U @ np.swapaxes(V, -1, -2)) < 0.0
+    U[refl, :, 2] *= -1.0
+    clamped |= refl
+    Fp = U @ (sc[:, :, None] * np.swapaxes(V, -1, -2))
+    neg = np.linalg.det(Fp) <= 0.0
+    Fp[neg, :, 2] *= -1.0
+    clamped |= neg
+    clamped |= raw_J <= 0.0
+    return Fp, clamped
+
+
+def _update_surface(state: MpmState, idx: np.ndarray) -> None:
+    """Normal and area from F (neo_hookean_mpm.py:673-685)."""
+    ra = state.rest_area[idx]
+    rn = state.rest_normal[idx]
+    ok = (ra > 0.0) & (np.linalg.norm(rn, axis=1) > 1.0e-12)
+    if not np.any(ok):
+        return
+    sel = idx[ok]
+    F = state.F[sel]
+    J = np.linalg.det(F)
+    good = J > 1.0e-12
+    sel, F, J = sel[good], F[good], J[good]
+    if sel.size == 0:
+        return
+    finv_t = np.swapaxes(np.linalg.inv(F), -1, -2)
+    av = J[:, None] * np.einsum("pij,pj->pi", finv_t, state.rest_normal[sel])
+    s = np.linalg.norm(av, axis=1)
+    good = s > 1.0e-12
+    state.normal[sel[good]] = av[good] / s[good, None]
+    state.area[sel[good]] = state.rest_area[sel[good]] * s[good]
+
+
+def _out_of_bounds(Xp: np.ndarray, n) -> np.ndarray:
+    n = np.asarray(n, np.float64)
+    return np.any((Xp < 0.5) | (Xp >= n - 1.5), axis=1)
+
+
+@dataclass
+class StepReport:
+    values: dict
+    grid_mass: np.ndarray
+    grid_velocity: np.ndarray
+    grid_force: np.ndarray
+
+
+def step(state: MpmState, *, bounds_min, bounds_max, grid_nodes, dt, mu, lam, damping=1.0,
+         primary_region_id=1, secondary_region_id=2) -> StepReport:
+    """One P2G / grid update / G2P step in float64 (neo_hookean_mpm.py:695-959).  Mutates ``state``."""
+    n = np.asarray(grid_nodes)
+    bmin = np.asarray(bounds_min, np.float64)
+    dx = (np.asarray(bounds_max, np.float64) - bmin) / n
+    P = state.x.shape[0]
+    gm = np.zeros(tuple(n))
+    gv = np.zeros(tuple(n) + (3,))
+    gf = np.zeros(tuple(n) + (3,))
+    rep = {}
+
+    # ---- P2G (712-801)
+    Xp = (state.x - bmin) / dx
+    base = np.floor(Xp - 0.5).astype(np.int64)
+    fx = Xp - base
+    w = weights(fx)  # (3, P, 3)
+    Fp, clamped = clamp_deformation(state.F)
+    state.F[clamped] = Fp[clamped]
+    J = np.linalg.det(Fp)
+    finv_t = np.swapaxes(np.linalg.inv(Fp), -1, -2)
+    Pk = mu * (Fp - finv_t) + lam * np.log(J)[:, None, None] * finv_t
+    inv_dx2 = 4.0 / (dx * dx)
+    stress = -dt * state.volume[:, None, None] * ((Pk @ np.swapaxes(Fp, -1, -2)) * inv_dx2[None, None, :])
+    affine = stress + state.mass[:, None, None] * state.C
+    mom = state.mass[:, None] * state.v
+    fext = state.f_ext.copy()
+    fixed = state.fixed != 0
+    mom[fixed] = 0.0
+    affine[fixed] = 0.0
+    fext[fixed] = 0.0
+    rep["particle_momentum"] = mom.sum(axis=0)
+    rep["external_force"] = fext.sum(axis=0)
+    rep["total_mass"] = float(state.mass.sum())
+    rep["total_volume"] = float(state.volume.sum())
+    rep["momentum_square_sum"] = float(np.sum(mom * mom))
+    rep["max_abs_j"] = float(np.abs(J).max()) if P else 0.0
+    rep["clamp_count"] = int(clamped.sum())
+    oob = _out_of_bounds(Xp, n)
+    rep["oob_count"] = int(oob.sum())
+    ins = ~oob
+    for ox in range(3):
+        for oy in range(3):
+            for oz in range(3):
+                o = np.array([ox, oy, oz])
+                wt = w[ox, :, 0] * w[oy, :, 1] * w[oz, :, 2]
+                node = base + o
+                dpos = (o - fx) * dx
+                m_ = wt[:, None] * (mom + np.einsum("pij,pj->pi", affine, dpos))
+                f_ = wt[:, None] * fext
+                idx = tuple(node[ins].T)
+                np.add.at(gm, idx, (wt * state.mass)[ins])
+                np.add.at(gv, idx, m_[ins])
+                np.add.at(gf, idx, f_[ins])
+
+    # ---- grid update (803-817)
+    active = gm > 1.0e-20
+    safe = np.where(active, gm, 1.0)
+    vel = gv / safe[..., None] + dt * gf / safe[..., None]
+    transfer = np.where(active[..., None], vel, 0.0)
+    vel = np.where(active[..., None], vel * damping, gv)
+    rep["transfer_grid_momentum"] = (gm[..., None] * transfer).sum(axis=(0, 1, 2))
+    rep["grid_momentum"] = np.where(active[..., None], gm[..., None] * vel, 0.0).sum(axis=(0, 1, 2))
+    rep["active_grid_nodes"] = int(active.sum())
+    rep["max_speed"] = float(np.linalg.norm(np.where(active[..., None], vel, 0.0), axis=-1).max())
+    grid_v = vel
+
+    # ---- G2P (819-874)
+    new_v = np.zeros((P, 3))
+    new_C = np.zeros((P, 3, 3))
+    for ox in range(3):
+        for oy in range(3):
+            for oz in range(3):
+                o = np.array([ox, oy, oz])
+                wt = w[ox, :, 0] * w[oy, :, 1] * w[oz, :, 2]
+                node = np.clip(base + o, 0, n - 1)
+                dpos = (o - fx) * dx
+                g = grid_v[node[:, 0], node[:, 1], node[:, 2]]
+                new_v += wt[:, None] * g
+                new_C += wt[:, None, None] * (g[:, :, None] * (4.0 * dpos / (dx * dx))[:, None, :])
+    upd = ins & ~fixed
+    state.v[upd] = new_v[upd]
+    state.C[upd] = new_C[upd]
+    state.x[upd] = state.x[upd] + dt * new_v[upd]
+    state.F[upd] = (np.eye(3) + dt * new_C[upd]) @ state.F[upd]
+    state.v[fixed] = 0.0
+    state.C[fixed] = 0.0
+    state.F[fixed] = np.eye(3)
+    _update_surface(state, np.arange(P))
+    Xn = (state.x - bmin) / dx
+    ins2 = ~_out_of_bounds(Xn, n)
+    prim = ins2 & (state.region == primary_region_id)
+    sec = ins2 & (state.region == secondary_region_id) & ~prim
+    disp = state.x - state.rest_x
+    rep["primary_displacement_sum"] = disp[prim].sum(axis=0)
+    rep["primary_velocity_sum"] = state.v[prim].sum(axis=0)
+    rep["secondary_displacement_sum"] = disp[sec].sum(axis=0)
+    rep["secondary_velocity_sum"] = state.v[sec].sum(axis=0)
+    rep["primary_count"] = int(prim.sum())
+    rep["secondary_count"] = int(sec.sum())
+
+    # ---- radial stretch diagnostics (876-907)
+    cnt = max(int(ins2.sum()), 1)
+    cur_c = state.x[ins2].sum(axis=0) / cnt
+    rest_c = state.rest_x[ins2].sum(axis=0) / cnt
+    rr = np.linalg.norm(state.rest_x - rest_c, axis=1)
+    use = ins2 & (rr > 1.0e-12)
+    stretch = np.linalg.norm(state.x - cur_c, axis=1)[use] / rr[use]
+    rep["radial_stretch_sum"] = float(stretch.sum())
+    rep["radial_stretch_count"] = int(use.sum())
+    rep["max_radial_stretch_error"] = float(np.abs(stretch - 1.0).max()) if stretch.size else 0.0
+    expected = rep["particle_momentum"] + dt * rep["external_force"]
+    den = max(float(np.linalg.norm(expected)), float(np.sqrt(rep["momentum_square_sum"])), 1.0e-20)
+    rep["transfer_relative_error"] = float(np.linalg.norm(rep["transfer_grid_momentum"] - expected) / den)
+    return StepReport(values=rep, grid_mass=gm, grid_velocity=grid_v, grid_force=gf)
