@@ -73,6 +73,20 @@ class CgOptions(ctypes.Structure):
     ]
 
 
+class MpmParticles(ctypes.Structure):
+    """hfsi_mpm_particles: device pointers of the particle arrays."""
+
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "x", "v", "C", "F", "mass", "volume", "rest_x", "fixed", "region", "external_force", "normal",
+        "rest_normal", "area", "rest_area")]
+
+
+class MpmGrid(ctypes.Structure):
+    """hfsi_mpm_grid: device pointers of the background-grid arrays."""
+
+    _fields_ = [("mass", ctypes.c_void_p), ("velocity", ctypes.c_void_p), ("force", ctypes.c_void_p)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 _D = ctypes.c_double
@@ -103,6 +117,10 @@ SIGNATURES = {
     "hfsi_fluid_divergence_report": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P]),
     "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
     "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),
+    "hfsi_mpm_create": (ctypes.c_int, [ctypes.POINTER(_P), _I, _P, _P, _P, _P]),
+    "hfsi_mpm_destroy": (ctypes.c_int, [_P]),
+    "hfsi_mpm_step": (ctypes.c_int, [_P, ctypes.POINTER(MpmParticles), _I, ctypes.POINTER(MpmGrid), _F, _F, _F,
+                                     _F, _I, _I, _P, _P]),
 }
 
 TIMER_KERNELS = {"l0_sweep": 1, "spmv_dot": 2, "l0_restrict": 3, "l0_prolong_sweep": 4}

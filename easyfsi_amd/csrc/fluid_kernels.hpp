@@ -41,6 +41,8 @@ struct OpGeo {
 
 template <typename T, typename C>
 struct Ld {
+  using Storage = T;
+  static constexpr int kSign = 1;
   const T* p;
   __device__ C operator()(uint32_t c, uint32_t, uint32_t, uint32_t) const { return (C)p[c]; }
   __device__ C at(uint32_t c) const { return (C)p[c]; }
@@ -48,6 +50,8 @@ struct Ld {
 
 template <typename T, typename C>
 struct LdNeg {
+  using Storage = T;
+  static constexpr int kSign = -1;
   const T* p;
   __device__ C at(uint32_t c) const { return -(C)p[c]; }
 };
@@ -208,11 +212,6 @@ __device__ inline C restrict_cell(const OpGeo<C>& fine, uint32_t ci, uint32_t cj
 // Full-grid kernels (XCD-aware chunk loop)
 // ---------------------------------------------------------------------------
 
-#define HFSI_CELL_LOOP(n)                                                         \
-  const ChunkRange _cr = xcd_chunks(((n) + blockDim.x - 1) / blockDim.x);        \
-  for (uint32_t _ch = _cr.first; _ch < _cr.end; _ch += _cr.step)                 \
-    for (uint32_t c = _ch * blockDim.x + threadIdx.x, _once = 1; _once; _once = 0) \
-      if (c < (n))
 
 template <typename C, typename P, typename R, typename TOut>
 __global__ void __launch_bounds__(kBlock) k_jacobi(OpGeo<C> g, P p, R rhs, TOut* out, C omega) {

@@ -64,6 +64,14 @@ __device__ inline ChunkRange xcd_chunks(uint32_t nchunks) {
   return ChunkRange{lo + s, hi, per};
 }
 
+// Loop over the cells [0, n) owned by this thread under the XCD-aware chunk
+// decomposition; `c` is the cell index; `continue` moves to the next chunk.
+#define HFSI_CELL_LOOP(n)                                                           \
+  const ChunkRange _cr = xcd_chunks(((n) + blockDim.x - 1) / blockDim.x);          \
+  for (uint32_t _ch = _cr.first; _ch < _cr.end; _ch += _cr.step)                   \
+    for (uint32_t c = _ch * blockDim.x + threadIdx.x, _once = 1; _once; _once = 0) \
+      if (c < (n))
+
 // Number of workgroups for a full-grid launch over n cells.
 inline uint32_t grid_blocks(uint64_t n, uint32_t max_blocks) {
   uint64_t chunks = (n + kBlock - 1) / kBlock;

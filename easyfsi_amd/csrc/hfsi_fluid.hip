@@ -23,7 +23,7 @@
 #include <vector>
 
 #include "../../include/hfsi.h"
-#include "fluid_kernels.hpp"
+#include "fluid_march.hpp"
 
 namespace hfsi {
 
@@ -34,6 +34,7 @@ constexpr int kMaxLevels = 16;
 constexpr uint32_t kMaxGrid = 2048;      // workgroups of full-grid launches (8 per CU)
 constexpr uint32_t kTailCells = 4096;    // levels at most this big run in the single-workgroup tail
 constexpr int kTailThreads = 1024;
+constexpr uint32_t kTailLdsMax = 152 * 1024;  // dynamic LDS budget of the tail workgroup (160 KB per CU)
 
 // ---------------------------------------------------------------------------
 // Device-resident CG scalars
@@ -745,10 +746,19 @@ struct TailLevel {
   TV* rhs;
 };
 
+// LDS layout of the tail levels (byte offsets into dynamic shared memory).
+struct TailLdsPlan {
+  uint32_t bytes;  // 0: the tail does not fit, run it from global memory
+  uint32_t off_lv;
+  uint32_t off_p[kMaxLevels], off_t[kMaxLevels], off_rhs[kMaxLevels], off_flags[kMaxLevels];
+  uint32_t off_cl[kMaxLevels][3], off_ch[kMaxLevels][3], off_wf[kMaxLevels][3];
+};
+
 template <typename C, typename TV>
 struct TailArgs {
   TailLevel<C, TV> lv[kMaxLevels];
   int first, last;  // levels [first, last] run in the tail; last is the coarsest
+  TailLdsPlan plan;
 };
 
 template <typename C, typename P, typename R, typename TOut>
@@ -768,14 +778,14 @@ __device__ __host__ inline TV* sweep_target(TV* P, TV* T, int W, int m) {
   return ((W - m) % 2 == 0) ? P : T;
 }
 
+// V-cycle over levels [first, last] from zero initial guesses, rhs[first]
+// given; the result of level `first` ends in lv[first].p.
 template <typename C, typename TV>
-__global__ void __launch_bounds__(kTailThreads) k_vcycle_tail(const TailArgs<C, TV>* args, int pre, int coarse,
-                                                              int post, C omega, C scale) {
-  const TailArgs<C, TV>& A = *args;
+__device__ inline void tail_body(const TailLevel<C, TV>* lv, int first, int last, int pre, int coarse, int post,
+                                 C omega, C scale) {
   TV* cur[kMaxLevels];
-  // down
-  for (int l = A.first; l < A.last; ++l) {
-    const TailLevel<C, TV>& L = A.lv[l];
+  for (int l = first; l < last; ++l) {
+    const TailLevel<C, TV>& L = lv[l];
     const int W = pre + post;
     Ld<TV, C> R{L.rhs};
     TV* out = sweep_target(L.p, L.t, W, 1);
@@ -787,7 +797,7 @@ __global__ void __launch_bounds__(kTailThreads) k_vcycle_tail(const TailArgs<C, 
       in = out;
     }
     cur[l] = in;
-    const TailLevel<C, TV>& Cl = A.lv[l + 1];
+    const TailLevel<C, TV>& Cl = lv[l + 1];
     for (uint32_t c = threadIdx.x; c < Cl.g.n; c += blockDim.x) {
       uint32_t ci, cj, ck;
       decompose(c, Cl.g.ny, Cl.g.nz, ci, cj, ck);
@@ -798,7 +808,7 @@ __global__ void __launch_bounds__(kTailThreads) k_vcycle_tail(const TailArgs<C, 
     __syncthreads();
   }
   {  // coarsest
-    const TailLevel<C, TV>& L = A.lv[A.last];
+    const TailLevel<C, TV>& L = lv[last];
     Ld<TV, C> R{L.rhs};
     TV* out = sweep_target(L.p, L.t, coarse, 1);
     tail_sweep(L.g, LdZero<C>{}, R, out, omega);
@@ -808,12 +818,11 @@ __global__ void __launch_bounds__(kTailThreads) k_vcycle_tail(const TailArgs<C, 
       tail_sweep(L.g, Ld<TV, C>{in}, R, out, omega);
       in = out;
     }
-    cur[A.last] = in;
+    cur[last] = in;
   }
-  // up
-  for (int l = A.last - 1; l >= A.first; --l) {
-    const TailLevel<C, TV>& L = A.lv[l];
-    const TailLevel<C, TV>& Cl = A.lv[l + 1];
+  for (int l = last - 1; l >= first; --l) {
+    const TailLevel<C, TV>& L = lv[l];
+    const TailLevel<C, TV>& Cl = lv[l + 1];
     const int W = pre + post;
     Ld<TV, C> R{L.rhs};
     LdProlong<TV, TV, C> PP{cur[l], cur[l + 1], L.g.flags, Cl.g.nx, Cl.g.ny, Cl.g.nz, Cl.g.sx, Cl.g.sy, scale};
@@ -827,6 +836,56 @@ __global__ void __launch_bounds__(kTailThreads) k_vcycle_tail(const TailArgs<C, 
     }
     cur[l] = in;
   }
+}
+
+template <typename C, typename TV>
+__global__ void __launch_bounds__(kTailThreads) k_vcycle_tail(const TailArgs<C, TV>* args, int pre, int coarse,
+                                                              int post, C omega, C scale) {
+  tail_body<C, TV>(args->lv, args->first, args->last, pre, coarse, post, omega, scale);
+}
+
+template <typename T>
+__device__ inline void block_copy(T* dst, const T* src, uint32_t n) {
+  for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) dst[c] = src[c];
+}
+
+// Same V-cycle tail with every level's iterates, masks and coefficients in LDS.
+template <typename C>
+__global__ void __launch_bounds__(kTailThreads) k_vcycle_tail_lds(const TailArgs<C, float>* args, int pre,
+                                                                  int coarse, int post, C omega, C scale) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const TailArgs<C, float>& A = *args;
+  const TailLdsPlan& P = A.plan;
+  TailLevel<C, float>* lv = reinterpret_cast<TailLevel<C, float>*>(smem + P.off_lv);
+  const int first = A.first, last = A.last;
+  if ((int)threadIdx.x >= first && (int)threadIdx.x <= last) {
+    const int l = threadIdx.x;
+    TailLevel<C, float> L = A.lv[l];
+    L.g.flags = smem + P.off_flags[l];
+    for (int a = 0; a < 3; ++a) {
+      L.g.cl[a] = reinterpret_cast<const C*>(smem + P.off_cl[l][a]);
+      L.g.ch[a] = reinterpret_cast<const C*>(smem + P.off_ch[l][a]);
+      L.g.wf[a] = reinterpret_cast<const float*>(smem + P.off_wf[l][a]);
+    }
+    L.p = reinterpret_cast<float*>(smem + P.off_p[l]);
+    L.t = reinterpret_cast<float*>(smem + P.off_t[l]);
+    L.rhs = reinterpret_cast<float*>(smem + P.off_rhs[l]);
+    lv[l] = L;
+  }
+  for (int l = first; l <= last; ++l) {
+    const OpGeo<C>& g = A.lv[l].g;
+    block_copy(smem + P.off_flags[l], g.flags, g.n);
+    const uint32_t len[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a) {
+      block_copy(reinterpret_cast<C*>(smem + P.off_cl[l][a]), g.cl[a], len[a]);
+      block_copy(reinterpret_cast<C*>(smem + P.off_ch[l][a]), g.ch[a], len[a]);
+      block_copy(reinterpret_cast<float*>(smem + P.off_wf[l][a]), g.wf[a], len[a]);
+    }
+  }
+  block_copy(reinterpret_cast<float*>(smem + P.off_rhs[first]), A.lv[first].rhs, A.lv[first].g.n);
+  __syncthreads();
+  tail_body<C, float>(lv, first, last, pre, coarse, post, omega, scale);
+  block_copy(A.lv[first].p, reinterpret_cast<const float*>(smem + P.off_p[first]), A.lv[first].g.n);
 }
 
 // ---------------------------------------------------------------------------
@@ -885,6 +944,7 @@ struct hfsi_fluid {
   double* host_totals = nullptr;       // pinned
   void* tail_args_d = nullptr;  // TailArgs<double,float>
   void* tail_args_f = nullptr;  // TailArgs<float,float>
+  uint32_t tail_lds_bytes_d = 0, tail_lds_bytes_f = 0;
   // live kernel timer: HIP events recorded on the launch stream around every
   // launch of one selected kernel (HFSI_TIMER_*), summed at flush/read time
   int timer_id = 0;
@@ -1001,15 +1061,107 @@ static inline void timer_mark(hfsi_fluid* h, int id, hipStream_t st) {
 // V-cycle driver
 // ---------------------------------------------------------------------------
 
-// Level-0 types: T0 storage of p/t at level 0, rhs0 loader R0.  Coarse levels
-// are float storage with compute type C.
+static inline bool march_ok(const hfsi_fluid* h, int l) {
+  const LevelMem& m = h->lv[l];
+  return !h->has_vd && m.nz >= 4 && (m.nz % 4) == 0;
+}
+
+static inline uint32_t pick_ichunk(uint32_t nx, uint32_t per_plane_blocks) {
+  uint32_t c = (nx * per_plane_blocks) / 2048u;
+  c = c < 2 ? 2 : (c > 16 ? 16 : c);
+  return c > nx ? nx : c;
+}
+
+static inline MarchGeo level_march_geo(const LevelMem& m) {
+  MarchGeo g0 = march_geo(m.nx, m.ny, m.nz, 1);
+  return march_geo(m.nx, m.ny, m.nz, pick_ichunk(m.nx, g0.kblocks * g0.jblocks));
+}
+
+// Marching sweep / apply on level l.  pc: coarse iterate of level l + 1 (prolongation).
+template <typename C, typename TP, typename TR, typename TO, typename TQ, int MODE, bool DOTS, typename Fin>
+static void launch_march(hfsi_fluid* h, int l, int outlet, const TP* p, const TR* rhs, C sign, TO* out, C omega,
+                         const float* pc, C scale, const TQ* q, Fin fin, hipStream_t st) {
+  const LevelMem& m = h->lv[l];
+  MarchArgs<C, TP, TR, TO, TQ> a;
+  a.m = level_march_geo(m);
+  OpGeo<C> g = geo<C>(h, l, outlet);
+  a.flags = m.flags;
+  a.diag = g.diag;
+  for (int ax = 0; ax < 3; ++ax) {
+    a.cl[ax] = g.cl[ax];
+    a.ch[ax] = g.ch[ax];
+    a.wd[ax] = m.wd[ax];
+  }
+  a.outlet_add = g.outlet_add;
+  a.outlet = outlet;
+  a.omega = omega;
+  a.rhs_sign = sign;
+  a.p = p;
+  a.rhs = rhs;
+  a.out = out;
+  a.pc = pc;
+  if (l + 1 < h->L) {
+    const LevelMem& c = h->lv[l + 1];
+    a.cnx = c.nx; a.cny = c.ny; a.cnz = c.nz; a.csx = c.ny * c.nz; a.csy = c.nz;
+  } else {
+    a.cnx = a.cny = a.cnz = 1; a.csx = a.csy = 1;
+  }
+  a.scale = scale;
+  a.q = q;
+  a.partials = h->partials;
+  a.counter = h->counter;
+  a.totals = h->totals;
+  hipLaunchKernelGGL((k_march<C, TP, TR, TO, TQ, MODE, DOTS, Fin>), dim3(march_grid(a.m)), dim3(256), 0, st, a, fin);
+}
+
+template <typename C, typename TP, typename TR, typename TRes>
+static void launch_march_restrict(hfsi_fluid* h, int l, int outlet, const TP* p, const TR* rhs, C sign,
+                                  hipStream_t st) {
+  const LevelMem& f = h->lv[l];
+  const LevelMem& c = h->lv[l + 1];
+  RestrictArgs<C, TP, TR, TRes> a;
+  MarchGeo g0 = march_geo(c.nx, c.ny, f.nz, 1);
+  a.m = march_geo(c.nx, c.ny, f.nz, pick_ichunk(c.nx, g0.kblocks * g0.jblocks));
+  a.fnx = f.nx; a.fny = f.ny; a.fnz = f.nz; a.fsx = f.ny * f.nz; a.fsy = f.nz;
+  a.cnx = c.nx; a.cny = c.ny; a.cnz = c.nz; a.csx = c.ny * c.nz; a.csy = c.nz;
+  OpGeo<C> g = geo<C>(h, l, outlet);
+  a.flags = f.flags;
+  a.cflags = c.flags;
+  a.diag = g.diag;
+  for (int ax = 0; ax < 3; ++ax) {
+    a.cl[ax] = g.cl[ax];
+    a.ch[ax] = g.ch[ax];
+    a.wf[ax] = f.wf[ax];
+  }
+  a.outlet_add = g.outlet_add;
+  a.outlet = outlet;
+  a.rhs_sign = sign;
+  a.p = p;
+  a.rhs = rhs;
+  a.out = c.rhs;
+  hipLaunchKernelGGL((k_march_restrict<C, TP, TR, TRes>), dim3(march_grid(a.m)), dim3(256), 0, st, a);
+}
+
+// Optional fusion of the PCG sums into the last level-0 post-smoothing sweep.
+struct FusedDots {
+  const void* q;
+  Scalars* sc;
+  int remove_mean, use_mg;
+  bool done;
+};
+
+// Level-0 types: T0 storage of p/t at level 0, rhs0 loader R0 (Ld or LdNeg).
+// Coarse levels are float storage with compute type C.
 template <typename C, typename T0, typename R0>
 static int run_vcycle(hfsi_fluid* h, T0* P0, T0* T0buf, R0 rhs0, bool zero_init, int pre, int coarse, int post,
-                      int outlet, hipStream_t st) {
+                      int outlet, hipStream_t st, FusedDots* dots = nullptr) {
+  using TR0 = typename R0::Storage;
   const int L = h->L;
   const C omega = (C)0.8f;  // f32 kernel argument in the reference
   const C scale = (C)(h->uniform ? 1.0f : 0.7f);
+  const C sign0 = (C)R0::kSign;
   OpGeo<C> g0 = geo<C>(h, 0, outlet);
+  if (dots) dots->done = false;
   if (L <= 1) {
     const int W = std::max(pre + post, coarse);
     if (!zero_init && (W % 2)) {
@@ -1036,14 +1188,27 @@ static int run_vcycle(hfsi_fluid* h, T0* P0, T0* T0buf, R0 rhs0, bool zero_init,
   }
   const int W = pre + post;
   const int tail = h->tail_first;  // >= 1
+  const bool m0 = march_ok(h, 0);
   // ---- level 0 down
   T0* in0 = P0;
   for (int m = 1; m <= pre; ++m) {
     T0* out = sweep_target(P0, T0buf, W, m);
-    if (m == 1 && zero_init)
+    const bool from_zero = m == 1 && zero_init;
+    if (m0) {
+      if (from_zero)
+        launch_march<C, T0, TR0, T0, T0, kModeFromZero, false>(h, 0, outlet, (const T0*)nullptr, rhs0.p, sign0, out,
+                                                               omega, nullptr, scale, (const T0*)nullptr, FinNone{},
+                                                               st);
+      else {
+        timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
+        launch_march<C, T0, TR0, T0, T0, kModeNormal, false>(h, 0, outlet, in0, rhs0.p, sign0, out, omega, nullptr,
+                                                             scale, (const T0*)nullptr, FinNone{}, st);
+        timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
+      }
+    } else if (from_zero) {
       hipLaunchKernelGGL((k_jacobi<C, LdZero<C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
                          LdZero<C>{}, rhs0, out, omega);
-    else {
+    } else {
       timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
       hipLaunchKernelGGL((k_jacobi<C, Ld<T0, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
                          Ld<T0, C>{in0}, rhs0, out, omega);
@@ -1051,40 +1216,63 @@ static int run_vcycle(hfsi_fluid* h, T0* P0, T0* T0buf, R0 rhs0, bool zero_init,
     }
     in0 = out;
   }
-  {
+  timer_mark(h, HFSI_TIMER_L0_RESTRICT, st);
+  if (m0) {
+    launch_march_restrict<C, T0, TR0, T0>(h, 0, outlet, in0, rhs0.p, sign0, st);
+  } else {
     OpGeo<C> g1 = geo<C>(h, 1, outlet);
-    timer_mark(h, HFSI_TIMER_L0_RESTRICT, st);
     hipLaunchKernelGGL((k_resid_restrict<T0, C, Ld<T0, C>, R0, float>), dim3(blocks_for(g1.n)), dim3(kBlock), 0, st,
                        g0, Ld<T0, C>{in0}, rhs0, g1, h->lv[1].rhs);
-    timer_mark(h, HFSI_TIMER_L0_RESTRICT, st);
   }
+  timer_mark(h, HFSI_TIMER_L0_RESTRICT, st);
   // ---- grid-launched coarse levels down
   float* cur[kMaxLevels] = {};
   for (int l = 1; l < tail; ++l) {
     LevelMem& m = h->lv[l];
     OpGeo<C> g = geo<C>(h, l, outlet);
+    const bool mk = march_ok(h, l);
     Ld<float, C> R{m.rhs};
     float* in = nullptr;
     for (int s = 1; s <= pre; ++s) {
       float* out = sweep_target(m.p, m.t, W, s);
-      if (s == 1)
+      if (mk) {
+        if (s == 1)
+          launch_march<C, float, float, float, float, kModeFromZero, false>(
+              h, l, outlet, (const float*)nullptr, m.rhs, (C)1, out, omega, nullptr, scale, (const float*)nullptr,
+              FinNone{}, st);
+        else
+          launch_march<C, float, float, float, float, kModeNormal, false>(h, l, outlet, in, m.rhs, (C)1, out, omega,
+                                                                          nullptr, scale, (const float*)nullptr,
+                                                                          FinNone{}, st);
+      } else if (s == 1) {
         hipLaunchKernelGGL((k_jacobi<C, LdZero<C>, Ld<float, C>, float>), dim3(blocks_for(g.n)), dim3(kBlock), 0,
                            st, g, LdZero<C>{}, R, out, omega);
-      else
+      } else {
         hipLaunchKernelGGL((k_jacobi<C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)), dim3(kBlock),
                            0, st, g, Ld<float, C>{in}, R, out, omega);
+      }
       in = out;
     }
     cur[l] = in;
-    OpGeo<C> gc = geo<C>(h, l + 1, outlet);
-    hipLaunchKernelGGL((k_resid_restrict<float, C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(gc.n)),
-                       dim3(kBlock), 0, st, g, Ld<float, C>{in}, R, gc, h->lv[l + 1].rhs);
+    if (mk) {
+      launch_march_restrict<C, float, float, float>(h, l, outlet, in, m.rhs, (C)1, st);
+    } else {
+      OpGeo<C> gc = geo<C>(h, l + 1, outlet);
+      hipLaunchKernelGGL((k_resid_restrict<float, C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(gc.n)),
+                         dim3(kBlock), 0, st, g, Ld<float, C>{in}, R, gc, h->lv[l + 1].rhs);
+    }
   }
   // ---- tail: levels tail .. L-1 in one workgroup
   {
-    const void* args = std::is_same<C, double>::value ? h->tail_args_d : h->tail_args_f;
-    hipLaunchKernelGGL((k_vcycle_tail<C, float>), dim3(1), dim3(kTailThreads), 0, st,
-                       (const TailArgs<C, float>*)args, pre, coarse, post, omega, scale);
+    const bool dbl = std::is_same<C, double>::value;
+    const void* args = dbl ? h->tail_args_d : h->tail_args_f;
+    const uint32_t lds = dbl ? h->tail_lds_bytes_d : h->tail_lds_bytes_f;
+    if (lds > 0)
+      hipLaunchKernelGGL((k_vcycle_tail_lds<C>), dim3(1), dim3(kTailThreads), lds, st,
+                         (const TailArgs<C, float>*)args, pre, coarse, post, omega, scale);
+    else
+      hipLaunchKernelGGL((k_vcycle_tail<C, float>), dim3(1), dim3(kTailThreads), 0, st,
+                         (const TailArgs<C, float>*)args, pre, coarse, post, omega, scale);
     cur[tail] = h->lv[tail].p;  // final smoothed value always lands in P
   }
   // ---- up through the grid-launched coarse levels
@@ -1092,16 +1280,28 @@ static int run_vcycle(hfsi_fluid* h, T0* P0, T0* T0buf, R0 rhs0, bool zero_init,
     LevelMem& m = h->lv[l];
     LevelMem& mc = h->lv[l + 1];
     OpGeo<C> g = geo<C>(h, l, outlet);
+    const bool mk = march_ok(h, l);
     Ld<float, C> R{m.rhs};
-    LdProlong<float, float, C> PP{cur[l], cur[l + 1], m.flags, mc.nx, mc.ny, mc.nz, mc.ny * mc.nz, mc.nz, scale};
     float* out = sweep_target(m.p, m.t, W, pre + 1);
-    hipLaunchKernelGGL((k_jacobi<C, LdProlong<float, float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)),
-                       dim3(kBlock), 0, st, g, PP, R, out, omega);
+    if (mk) {
+      launch_march<C, float, float, float, float, kModeProlong, false>(h, l, outlet, cur[l], m.rhs, (C)1, out, omega,
+                                                                       cur[l + 1], scale, (const float*)nullptr,
+                                                                       FinNone{}, st);
+    } else {
+      LdProlong<float, float, C> PP{cur[l], cur[l + 1], m.flags, mc.nx, mc.ny, mc.nz, mc.ny * mc.nz, mc.nz, scale};
+      hipLaunchKernelGGL((k_jacobi<C, LdProlong<float, float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)),
+                         dim3(kBlock), 0, st, g, PP, R, out, omega);
+    }
     float* in = out;
     for (int s = pre + 2; s <= W; ++s) {
       out = sweep_target(m.p, m.t, W, s);
-      hipLaunchKernelGGL((k_jacobi<C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)), dim3(kBlock), 0,
-                         st, g, Ld<float, C>{in}, R, out, omega);
+      if (mk)
+        launch_march<C, float, float, float, float, kModeNormal, false>(h, l, outlet, in, m.rhs, (C)1, out, omega,
+                                                                        nullptr, scale, (const float*)nullptr,
+                                                                        FinNone{}, st);
+      else
+        hipLaunchKernelGGL((k_jacobi<C, Ld<float, C>, Ld<float, C>, float>), dim3(blocks_for(g.n)), dim3(kBlock), 0,
+                           st, g, Ld<float, C>{in}, R, out, omega);
       in = out;
     }
     cur[l] = in;
@@ -1109,18 +1309,44 @@ static int run_vcycle(hfsi_fluid* h, T0* P0, T0* T0buf, R0 rhs0, bool zero_init,
   // ---- level 0 up
   {
     LevelMem& mc = h->lv[1];
-    LdProlong<T0, float, C> PP{in0, cur[1], h->lv[0].flags, mc.nx, mc.ny, mc.nz, mc.ny * mc.nz, mc.nz, scale};
     T0* out = sweep_target(P0, T0buf, W, pre + 1);
+    const bool fuse_last_prolong = dots && m0 && W == pre + 1;
     timer_mark(h, HFSI_TIMER_L0_PROLONG_SWEEP, st);
-    hipLaunchKernelGGL((k_jacobi<C, LdProlong<T0, float, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st,
-                       g0, PP, rhs0, out, omega);
+    if (m0) {
+      if (fuse_last_prolong) {
+        launch_march<C, T0, TR0, T0, T0, kModeProlong, true>(
+            h, 0, outlet, in0, rhs0.p, sign0, out, omega, cur[1], scale, (const T0*)dots->q,
+            FinDots5{dots->sc, dots->remove_mean, dots->use_mg}, st);
+        dots->done = true;
+      } else {
+        launch_march<C, T0, TR0, T0, T0, kModeProlong, false>(h, 0, outlet, in0, rhs0.p, sign0, out, omega, cur[1],
+                                                              scale, (const T0*)nullptr, FinNone{}, st);
+      }
+    } else {
+      LdProlong<T0, float, C> PP{in0, cur[1], h->lv[0].flags, mc.nx, mc.ny, mc.nz, mc.ny * mc.nz, mc.nz, scale};
+      hipLaunchKernelGGL((k_jacobi<C, LdProlong<T0, float, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st,
+                         g0, PP, rhs0, out, omega);
+    }
     timer_mark(h, HFSI_TIMER_L0_PROLONG_SWEEP, st);
     T0* in = out;
     for (int s = pre + 2; s <= W; ++s) {
       out = sweep_target(P0, T0buf, W, s);
+      const bool last = s == W;
       timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
-      hipLaunchKernelGGL((k_jacobi<C, Ld<T0, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
-                         Ld<T0, C>{in}, rhs0, out, omega);
+      if (m0) {
+        if (last && dots) {
+          launch_march<C, T0, TR0, T0, T0, kModeNormal, true>(h, 0, outlet, in, rhs0.p, sign0, out, omega, nullptr,
+                                                             scale, (const T0*)dots->q,
+                                                             FinDots5{dots->sc, dots->remove_mean, dots->use_mg}, st);
+          dots->done = true;
+        } else {
+          launch_march<C, T0, TR0, T0, T0, kModeNormal, false>(h, 0, outlet, in, rhs0.p, sign0, out, omega, nullptr,
+                                                              scale, (const T0*)nullptr, FinNone{}, st);
+        }
+      } else {
+        hipLaunchKernelGGL((k_jacobi<C, Ld<T0, C>, R0, T0>), dim3(blocks_for(g0.n)), dim3(kBlock), 0, st, g0,
+                           Ld<T0, C>{in}, rhs0, out, omega);
+      }
       timer_mark(h, HFSI_TIMER_L0_SWEEP, st);
       in = out;
     }
@@ -1150,10 +1376,13 @@ Vecs<float, float> vecs<float, float>(hfsi_fluid* h) {
 }
 
 template <typename C, typename TV>
-static int precondition(hfsi_fluid* h, bool use_mg, int pre, int coarse, int post, int outlet, hipStream_t st) {
+static int precondition(hfsi_fluid* h, bool use_mg, int pre, int coarse, int post, int outlet, hipStream_t st,
+                        FusedDots* dots = nullptr) {
   Vecs<C, TV> v = vecs<C, TV>(h);
+  if (dots) dots->done = false;
   if (use_mg)
-    return run_vcycle<C, TV, LdNeg<TV, C>>(h, v.z, v.t, LdNeg<TV, C>{v.r}, true, pre, coarse, post, outlet, st);
+    return run_vcycle<C, TV, LdNeg<TV, C>>(h, v.z, v.t, LdNeg<TV, C>{v.r}, true, pre, coarse, post, outlet, st,
+                                           dots);
   OpGeo<C> g = geo<C>(h, 0, outlet);
   hipLaunchKernelGGL((k_jacobi_precond<C, TV, TV>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.r, v.z);
   HFSI_LAUNCH_CHECK();
@@ -1263,8 +1492,12 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   while (it < max_it) {
     ++it;
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
-    hipLaunchKernelGGL((k_spmv_dot<C, TV, TV, FinDad>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.d, v.q,
-                       h->partials, h->counter, h->totals, FinDad{sc});
+    if (march_ok(h, 0))
+      launch_march<C, TV, TV, TV, TV, kModeApply, true>(h, 0, outlet, v.d, (const TV*)nullptr, (C)1, v.q, (C)0,
+                                                        nullptr, (C)0, (const TV*)nullptr, FinDad{sc}, st);
+    else
+      hipLaunchKernelGGL((k_spmv_dot<C, TV, TV, FinDad>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.d, v.q,
+                         h->partials, h->counter, h->totals, FinDad{sc});
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
     HFSI_LAUNCH_CHECK();
     HFSI_TRY(launch_pointwise(h, 0, OpCgUpdate<TV, TV, TV>{x, v.d, v.r, v.q, sc}, st));
@@ -1331,10 +1564,12 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
         break;
       }
     }
-    HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st)));
+    FusedDots fd{v.q, sc, rm, use_mg ? 1 : 0, false};
+    HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st, &fd)));
     if (use_mg) S.vcycles += 1;
     if (rm) S.mean_projection_count += 1;
-    HFSI_TRY(launch_sums<5>(h, g64, OpDots5<TV, TV, TV>{v.r, v.z, v.q}, FinDots5{sc, rm, use_mg ? 1 : 0}, st));
+    if (!fd.done)
+      HFSI_TRY(launch_sums<5>(h, g64, OpDots5<TV, TV, TV>{v.r, v.z, v.q}, FinDots5{sc, rm, use_mg ? 1 : 0}, st));
     HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 0}, st));
   }
   HFSI_TRY(sync_scalars(h, st));
@@ -1361,6 +1596,39 @@ static int write_tail_args(hfsi_fluid* h, void* dst, hipStream_t st) {
     A.lv[l].t = h->lv[l].t;
     A.lv[l].rhs = h->lv[l].rhs;
   }
+  // LDS plan: level descriptors, then per level p / t / rhs / flags and the
+  // per-axis coefficient and width arrays, 16-byte aligned
+  {
+    auto al = [](uint32_t v) { return (v + 15u) & ~15u; };
+    uint32_t off = 0;
+    A.plan.off_lv = off;
+    off = al(off + (uint32_t)sizeof(TailLevel<C, float>) * kMaxLevels);
+    for (int l = A.first; l <= A.last && l >= 1; ++l) {
+      const LevelMem& m = h->lv[l];
+      A.plan.off_p[l] = off;
+      off = al(off + 4 * m.n);
+      A.plan.off_t[l] = off;
+      off = al(off + 4 * m.n);
+      A.plan.off_rhs[l] = off;
+      off = al(off + 4 * m.n);
+      A.plan.off_flags[l] = off;
+      off = al(off + m.n);
+      const uint32_t len[3] = {m.nx, m.ny, m.nz};
+      for (int a = 0; a < 3; ++a) {
+        A.plan.off_cl[l][a] = off;
+        off = al(off + (uint32_t)sizeof(C) * len[a]);
+        A.plan.off_ch[l][a] = off;
+        off = al(off + (uint32_t)sizeof(C) * len[a]);
+        A.plan.off_wf[l][a] = off;
+        off = al(off + 4 * len[a]);
+      }
+    }
+    A.plan.bytes = (h->L > 1 && off <= kTailLdsMax) ? off : 0;
+  }
+  if (std::is_same<C, double>::value)
+    h->tail_lds_bytes_d = A.plan.bytes;
+  else
+    h->tail_lds_bytes_f = A.plan.bytes;
   // the outlet flag is per solve: the tail reads outlet from lv[].g.outlet,
   // written below for both values by the solve entry (see set_tail_outlet)
   HFSI_CHECK(hipMemcpyAsync(dst, &A, sizeof(A), hipMemcpyHostToDevice, st));
@@ -1570,6 +1838,12 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
     // switches are refreshed by hfsi_fluid_set_masks)
     rc |= dalloc(h, (char**)&h->tail_args_d, sizeof(TailArgs<double, float>));
     rc |= dalloc(h, (char**)&h->tail_args_f, sizeof(TailArgs<float, float>));
+  }
+  if (rc == HFSI_OK) {
+    (void)hipFuncSetAttribute((const void*)k_vcycle_tail_lds<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kTailLdsMax);
+    (void)hipFuncSetAttribute((const void*)k_vcycle_tail_lds<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kTailLdsMax);
   }
   if (rc == HFSI_OK) {
     // Valid masks from the start (every cell free), so that no kernel can see

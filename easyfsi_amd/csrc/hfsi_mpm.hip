@@ -247,9 +247,12 @@ __device__ inline void grid_coord(const MpmArgs& a, const float* xp, float X[3])
   for (int d = 0; d < 3; ++d) X[d] = (xp[d] - a.bmin[d]) / a.dx[d];
 }
 
+// Written as the negation of "inside" so that a NaN coordinate counts as out
+// of bounds and never reaches the node-index computation.
 __device__ inline bool stencil_oob(const MpmArgs& a, const float X[3]) {
-  return X[0] < 0.5f || X[0] >= (float)a.nx - 1.5f || X[1] < 0.5f || X[1] >= (float)a.ny - 1.5f || X[2] < 0.5f ||
-         X[2] >= (float)a.nz - 1.5f;
+  const bool inside = X[0] >= 0.5f && X[0] < (float)a.nx - 1.5f && X[1] >= 0.5f && X[1] < (float)a.ny - 1.5f &&
+                      X[2] >= 0.5f && X[2] < (float)a.nz - 1.5f;
+  return !inside;
 }
 
 __device__ inline void bspline(float f, float w[3]) {
@@ -301,7 +304,7 @@ __device__ inline bool reduce_stage(double (&v)[NS], uint64_t maxmask, double* p
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(kBlock) k_mpm_clear(uint32_t n, float* gm, float* gv, float* gf) {
-  HFSI_CELL_LOOP_MPM(n) {
+  HFSI_CELL_LOOP(n) {
     gm[c] = 0.0f;
     gv[3 * (size_t)c + 0] = 0.0f;
     gv[3 * (size_t)c + 1] = 0.0f;
@@ -320,7 +323,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_p2g(MpmArgs a) {
 #pragma unroll
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
   const uint32_t n = (uint32_t)a.count;
-  HFSI_CELL_LOOP_MPM(n) {
+  HFSI_CELL_LOOP(n) {
     const uint32_t p = c;
     float X[3];
     grid_coord(a, a.x + 3 * (size_t)p, X);
@@ -453,7 +456,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_grid(MpmArgs a) {
 #pragma unroll
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
   const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
-  HFSI_CELL_LOOP_MPM(n) {
+  HFSI_CELL_LOOP(n) {
     const float m = a.gm[c];
     if (!(m > 1.0e-20f)) continue;
     float v[3];
@@ -504,7 +507,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_g2p(MpmArgs a) {
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
   const uint32_t n = (uint32_t)a.count;
   const uint32_t sx = (uint32_t)a.ny * a.nz, sy = a.nz;
-  HFSI_CELL_LOOP_MPM(n) {
+  HFSI_CELL_LOOP(n) {
     const uint32_t p = c;
     float* xp = a.x + 3 * (size_t)p;
     float X[3];
@@ -621,7 +624,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_radial(MpmArgs a) {
   const double cc[3] = {a.totals[T_CCUR] / cnt, a.totals[T_CCUR + 1] / cnt, a.totals[T_CCUR + 2] / cnt};
   const double rc[3] = {a.totals[T_CREST] / cnt, a.totals[T_CREST + 1] / cnt, a.totals[T_CREST + 2] / cnt};
   const uint32_t n = (uint32_t)a.count;
-  HFSI_CELL_LOOP_MPM(n) {
+  HFSI_CELL_LOOP(n) {
     const float* xp = a.x + 3 * (size_t)c;
     float X[3];
     grid_coord(a, xp, X);

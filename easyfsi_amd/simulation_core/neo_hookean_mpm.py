@@ -1,0 +1,251 @@
+"""NeoHookeanMpmState on libhfsi (MI355X).
+
+Drop-in for ``simulation_core/neo_hookean_mpm.py::NeoHookeanMpmState`` on the
+hot path: same constructor, ``initialize_box``, load setters, ``step`` /
+``report`` (``NeoHookeanMpmReport``), ``save_state`` / ``restore_state`` and
+field names.  The step runs in the hand-written HIP kernels of libhfsi
+(``hfsi_mpm_step``).  Layered tri-surface initialisation and region loads
+(neo_hookean_mpm.py:240-520) are outside this build.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .fields import scalar_field, vector_field, matrix_field
+from .runtime import RuntimeConfig, init_runtime
+
+MIN_DEFORMATION_SINGULAR_VALUE = 1.0e-2
+MAX_DEFORMATION_SINGULAR_VALUE = 1.0e2
+
+
+def _vector3(value, name: str) -> tuple[float, float, float]:
+    if len(value) != 3:
+        raise ValueError(f"{name} must contain exactly 3 components")
+    return (float(value[0]), float(value[1]), float(value[2]))
+
+
+def _raise_if_all_particles_out_of_bounds(particle_count: int, oob: int) -> None:
+    """neo_hookean_mpm.py:18-26."""
+    if particle_count > 0 and oob == particle_count:
+        raise RuntimeError(f"all {particle_count} MPM particles are outside the background grid; "
+                           "the solid has left the simulation domain")
+
+
+@dataclass(frozen=True)
+class NeoHookeanMpmReport:
+    particle_count: int
+    active_grid_nodes: int
+    grid_out_of_bounds_particle_count: int
+    particle_spacing_m: float
+    grid_spacing_m: tuple[float, float, float]
+    total_mass_kg: float
+    total_volume_m3: float
+    primary_mean_displacement_m: tuple[float, float, float]
+    primary_mean_velocity_mps: tuple[float, float, float]
+    secondary_mean_displacement_m: tuple[float, float, float]
+    secondary_mean_velocity_mps: tuple[float, float, float]
+    particle_momentum_kg_mps: tuple[float, float, float]
+    grid_momentum_kg_mps: tuple[float, float, float]
+    external_force_n: tuple[float, float, float]
+    transfer_relative_error: float
+    max_speed_mps: float
+    max_abs_j: float
+    deformation_clamp_count: int
+    mean_radial_stretch: float
+    max_radial_stretch_error: float
+
+
+class NeoHookeanMpmState:
+    """Neo-Hookean APIC/MLS-MPM particle state stepped by libhfsi."""
+
+    def __init__(self, particle_capacity: int, bounds_min_m, bounds_max_m, grid_nodes,
+                 runtime: RuntimeConfig | None = None):
+        init_runtime(runtime)
+        self._lib = _lib.lib()
+        if particle_capacity <= 0:
+            raise ValueError("particle_capacity must be positive")
+        if min(grid_nodes) < 4:
+            raise ValueError("grid_nodes must be at least 4 in each direction")
+        self.particle_capacity = int(particle_capacity)
+        self.particle_count = 0
+        self.grid_nodes = tuple(int(v) for v in grid_nodes)
+        self.nx, self.ny, self.nz = self.grid_nodes
+        self.bounds_min = tuple(float(v) for v in bounds_min_m)
+        self.bounds_max = tuple(float(v) for v in bounds_max_m)
+        self.dx = tuple((self.bounds_max[a] - self.bounds_min[a]) / self.grid_nodes[a] for a in range(3))
+        if min(self.dx) <= 0.0:
+            raise ValueError("bounds must define a positive grid spacing")
+        P = self.particle_capacity
+        f32, i32 = torch.float32, torch.int32
+        self.x = vector_field(3, (P,), f32)
+        self.rest_x = vector_field(3, (P,), f32)
+        self.v = vector_field(3, (P,), f32)
+        self.C = matrix_field(3, 3, (P,), f32)
+        self.F = matrix_field(3, 3, (P,), f32)
+        self.saved_x = vector_field(3, (P,), f32)
+        self.saved_v = vector_field(3, (P,), f32)
+        self.saved_C = matrix_field(3, 3, (P,), f32)
+        self.saved_F = matrix_field(3, 3, (P,), f32)
+        self.mass_kg = scalar_field((P,), f32)
+        self.volume_m3 = scalar_field((P,), f32)
+        self.area_weight_m2 = scalar_field((P,), f32)
+        self.rest_area_weight_m2 = scalar_field((P,), f32)
+        self.region_id = scalar_field((P,), i32)
+        self.fixed_particle = scalar_field((P,), i32)
+        self.surface_normal = vector_field(3, (P,), f32)
+        self.rest_surface_normal = vector_field(3, (P,), f32)
+        self.external_force_n = vector_field(3, (P,), f32)
+        self.grid_mass_kg = scalar_field(self.grid_nodes, f32)
+        self.grid_velocity_mps = vector_field(3, self.grid_nodes, f32)
+        self.grid_force_n = vector_field(3, self.grid_nodes, f32)
+        self.last_report_host_reads = 0
+        self._report = np.zeros(34, dtype=np.float64)
+        h = ctypes.c_void_p()
+        bmin = np.ascontiguousarray(self.bounds_min, np.float64)
+        bmax = np.ascontiguousarray(self.bounds_max, np.float64)
+        nodes = np.ascontiguousarray(self.grid_nodes, np.int32)
+        _lib.check(self._lib.hfsi_mpm_create(ctypes.byref(h), P, bmin.ctypes.data, bmax.ctypes.data,
+                                             nodes.ctypes.data, _lib.stream_handle()), "hfsi_mpm_create")
+        self._handle = h
+        eye = torch.eye(3, dtype=f32, device="cuda")
+        self.F.tensor[:] = eye
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.hfsi_mpm_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    # ------------------------------------------------------------------
+    def initialize_box(self, *, particle_counts, box_min_m, box_max_m, density_kgm3: float) -> None:
+        """Cell-centred lattice in a box (neo_hookean_mpm.py:144-229)."""
+        nxp, nyp, nzp = (int(v) for v in particle_counts)
+        if min(nxp, nyp, nzp) <= 0:
+            raise ValueError("particle_counts must be positive")
+        count = nxp * nyp * nzp
+        if count > self.particle_capacity:
+            raise ValueError("particle count exceeds capacity")
+        self.particle_count = count
+        f = np.float32
+        lo = np.asarray(box_min_m, f)
+        hi = np.asarray(box_max_m, f)
+        vol = f((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]))
+        pv = f(vol / max(f(count), f(1.0)))
+        p = np.arange(count)
+        ix, iy, iz = p % nxp, (p // nxp) % nyp, p // (nxp * nyp)
+        pos = np.stack([lo[0] + (ix.astype(f) + f(0.5)) * (hi[0] - lo[0]) / f(nxp),
+                        lo[1] + (iy.astype(f) + f(0.5)) * (hi[1] - lo[1]) / f(nyp),
+                        lo[2] + (iz.astype(f) + f(0.5)) * (hi[2] - lo[2]) / f(nzp)], axis=1).astype(f)
+        t = torch.from_numpy(pos).cuda()
+        self.x.tensor[:count] = t
+        self.rest_x.tensor[:count] = t
+        self.v.tensor[:count] = 0
+        self.C.tensor[:count] = 0
+        self.F.tensor[:count] = torch.eye(3, dtype=torch.float32, device="cuda")
+        self.volume_m3.tensor[:count] = float(pv)
+        self.mass_kg.tensor[:count] = float(f(density_kgm3) * pv)
+        for fld in (self.area_weight_m2, self.rest_area_weight_m2, self.region_id, self.fixed_particle,
+                    self.surface_normal, self.rest_surface_normal, self.external_force_n):
+            fld.tensor[:count] = 0
+
+    def set_uniform_velocity(self, velocity_mps) -> None:
+        v = torch.tensor(_vector3(velocity_mps, "velocity_mps"), dtype=torch.float32, device="cuda")
+        self.v.tensor[: self.particle_count] = v
+
+    def set_uniform_external_force(self, force_n) -> None:
+        f = torch.tensor(_vector3(force_n, "force_n"), dtype=torch.float32, device="cuda")
+        self.external_force_n.tensor[: self.particle_count] = f
+
+    # ------------------------------------------------------------------
+    def _particles(self) -> _lib.MpmParticles:
+        P = _lib.MpmParticles()
+        P.x, P.v, P.C, P.F = self.x.ptr, self.v.ptr, self.C.ptr, self.F.ptr
+        P.mass, P.volume, P.rest_x = self.mass_kg.ptr, self.volume_m3.ptr, self.rest_x.ptr
+        P.fixed, P.region, P.external_force = self.fixed_particle.ptr, self.region_id.ptr, self.external_force_n.ptr
+        P.normal, P.rest_normal = self.surface_normal.ptr, self.rest_surface_normal.ptr
+        P.area, P.rest_area = self.area_weight_m2.ptr, self.rest_area_weight_m2.ptr
+        return P
+
+    def _grid(self) -> _lib.MpmGrid:
+        G = _lib.MpmGrid()
+        G.mass, G.velocity, G.force = self.grid_mass_kg.ptr, self.grid_velocity_mps.ptr, self.grid_force_n.ptr
+        return G
+
+    def step(self, *, dt_s: float, mu_pa: float, lambda_pa: float, primary_region_id: int,
+             secondary_region_id: int, velocity_damping: float = 1.0, read_report: bool = True):
+        """neo_hookean_mpm.py:961-986."""
+        if self.particle_count <= 0:
+            raise ValueError("initialize particles before stepping")
+        P, G = self._particles(), self._grid()
+        out = self._report.ctypes.data if read_report else None
+        _lib.check(self._lib.hfsi_mpm_step(self._handle, ctypes.byref(P), int(self.particle_count), ctypes.byref(G),
+                                           float(dt_s), float(mu_pa), float(lambda_pa), float(velocity_damping),
+                                           int(primary_region_id), int(secondary_region_id), out,
+                                           _lib.stream_handle()), "hfsi_mpm_step")
+        if not read_report:
+            self.last_report_host_reads = 0
+            return None
+        return self.report()
+
+    def report(self) -> NeoHookeanMpmReport:
+        """Decode the packed snapshot of the last step (neo_hookean_mpm.py:1016-1066)."""
+        s = self._report.astype(np.float32).astype(np.float64)  # the reference's snapshot is float32
+        values, counts = s[:28], s[28:34]
+        self.last_report_host_reads = 1
+        _raise_if_all_particles_out_of_bounds(int(self.particle_count), int(counts[1]))
+        total_volume = float(values[1])
+        spacing = (total_volume / self.particle_count) ** (1.0 / 3.0) if total_volume > 0 and self.particle_count else 0.0
+        pc, sc, rc = max(int(counts[4]), 1), max(int(counts[5]), 1), max(int(counts[3]), 1)
+        t = lambda a, n: (float(values[a]) / n, float(values[a + 1]) / n, float(values[a + 2]) / n)  # noqa: E731
+        return NeoHookeanMpmReport(
+            particle_count=int(self.particle_count), active_grid_nodes=int(counts[0]),
+            grid_out_of_bounds_particle_count=int(counts[1]), particle_spacing_m=spacing, grid_spacing_m=self.dx,
+            total_mass_kg=float(values[0]), total_volume_m3=total_volume,
+            primary_mean_displacement_m=t(16, pc), primary_mean_velocity_mps=t(19, pc),
+            secondary_mean_displacement_m=t(22, sc), secondary_mean_velocity_mps=t(25, sc),
+            particle_momentum_kg_mps=t(2, 1), grid_momentum_kg_mps=t(5, 1), external_force_n=t(8, 1),
+            transfer_relative_error=float(values[11]), max_speed_mps=float(values[12]), max_abs_j=float(values[13]),
+            deformation_clamp_count=int(counts[2]), mean_radial_stretch=float(values[14]) / rc,
+            max_radial_stretch_error=float(values[15]))
+
+    # ------------------------------------------------------------------
+    def save_state(self) -> None:
+        """Snapshot x, v, C, F (neo_hookean_mpm.py:988-1004)."""
+        if self.particle_count <= 0:
+            raise ValueError("initialize particles before saving state")
+        n = self.particle_count
+        for dst, src in ((self.saved_x, self.x), (self.saved_v, self.v), (self.saved_C, self.C),
+                         (self.saved_F, self.F)):
+            dst.tensor[:n] = src.tensor[:n]
+
+    def restore_state(self) -> None:
+        """Restore x, v, C, F, refresh the surface geometry, zero the external force (neo_hookean_mpm.py:996-1014)."""
+        if self.particle_count <= 0:
+            raise ValueError("initialize particles before restoring state")
+        n = self.particle_count
+        for dst, src in ((self.x, self.saved_x), (self.v, self.saved_v), (self.C, self.saved_C),
+                         (self.F, self.saved_F)):
+            dst.tensor[:n] = src.tensor[:n]
+        self.external_force_n.tensor[:n] = 0
+        F = self.F.tensor[:n].double()
+        ra = self.rest_area_weight_m2.tensor[:n].double()
+        rn = self.rest_surface_normal.tensor[:n].double()
+        J = torch.linalg.det(F)
+        ok = (ra > 0) & (torch.linalg.vector_norm(rn, dim=1) > 1e-12) & (J > 1e-12)
+        if bool(ok.any()):
+            Fi = torch.linalg.inv(F[ok]).transpose(1, 2)
+            av = J[ok, None] * torch.einsum("pij,pj->pi", Fi, rn[ok])
+            s = torch.linalg.vector_norm(av, dim=1)
+            good = s > 1e-12
+            idx = torch.nonzero(ok).squeeze(1)[good]
+            self.surface_normal.tensor[idx] = (av[good] / s[good, None]).float()
+            self.area_weight_m2.tensor[idx] = (ra[ok][good] * s[good]).float()

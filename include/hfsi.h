@@ -187,6 +187,56 @@ int hfsi_fluid_divergence_report(hfsi_fluid* h, const float* divergence, const f
                                  const double* iface_div, const int32_t* vd_active, int32_t outlet_zmin,
                                  double* out54, void* stream);
 
+/* ---- neo-Hookean MPM ------------------------------------------------------ */
+
+typedef struct hfsi_mpm hfsi_mpm;
+
+/* Particle arrays (caller-owned device memory, float32 unless noted):
+ * x, v, rest_x, external_force, normal, rest_normal: [P][3];  C, F: [P][3][3]
+ * row-major;  mass, volume, area, rest_area: [P];  fixed, region: int32 [P].
+ * fixed, region, external_force may be NULL (all zero); the four surface
+ * arrays are all NULL or all set. */
+typedef struct hfsi_mpm_particles {
+  float* x;
+  float* v;
+  float* C;
+  float* F;
+  const float* mass;
+  const float* volume;
+  const float* rest_x;
+  const int32_t* fixed;
+  const int32_t* region;
+  const float* external_force;
+  float* normal;
+  const float* rest_normal;
+  float* area;
+  const float* rest_area;
+} hfsi_mpm_particles;
+
+/* Background grid (caller-owned): mass [n], velocity / momentum [n][3],
+ * force [n][3], n = nx*ny*nz nodes, k fastest. */
+typedef struct hfsi_mpm_grid {
+  float* mass;
+  float* velocity;
+  float* force;
+} hfsi_mpm_grid;
+
+/* Uniform background grid: dx_a = (max_a - min_a) / n_a
+ * (NeoHookeanMpmState.__init__, simulation_core/neo_hookean_mpm.py:57-142).
+ * Allocates the reduction workspace. */
+int hfsi_mpm_create(hfsi_mpm** out, int32_t particle_capacity, const double* bounds_min3, const double* bounds_max3,
+                    const int32_t* grid_nodes3, void* stream);
+int hfsi_mpm_destroy(hfsi_mpm* h);
+
+/* One step: grid clear, P2G with the SVD clamp and neo-Hookean stress, grid
+ * update with damping, G2P / APIC / F update, surface update and the report
+ * reductions (NeoHookeanMpmState._step_kernel, neo_hookean_mpm.py:695-959).
+ * When report34 (host) is not NULL the call synchronises and writes the 28
+ * values + 6 counts of the reference's packed snapshot (neo_hookean_mpm.py:909-959). */
+int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* particles, int32_t count, const hfsi_mpm_grid* grid,
+                  float dt_s, float mu_pa, float lambda_pa, float velocity_damping, int32_t primary_region_id,
+                  int32_t secondary_region_id, double* report34, void* stream);
+
 /* Live kernel timing (no reference counterpart; measurement support):
  * while enabled, HIP events are recorded on the launch stream around every
  * launch of the selected level-0 kernel and their elapsed times summed.
