@@ -75,6 +75,16 @@ struct Pack4<__half> {
   }
 };
 
+// Value of v after a round trip through storage type T (register-only).
+template <typename T, typename C>
+__device__ inline C storage_round(C v) {
+  return (C)(T)v;
+}
+template <>
+__device__ inline float storage_round<__half, float>(float v) {
+  return __half2float(__float2half_rn(v));
+}
+
 __device__ inline double to_c_exact(double v) { return v; }
 __device__ inline float to_c_exact(float v) { return v; }
 __device__ inline float to_c_exact(__half v) { return __half2float(v); }
@@ -293,8 +303,9 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a, F
       Pack4<TO>::store(a.out + c, out);
       if (DOTS) {
         const double wx = a.wd[0][i];
-        C stored[4];
-        Pack4<TO>::load(a.out + c, stored);  // the rounded values actually kept
+        C stored[4];  // the rounded values actually kept
+#pragma unroll
+        for (int m = 0; m < 4; ++m) stored[m] = storage_round<TO, C>(out[m]);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
           const uint32_t f = (f4 >> (8 * m)) & 0xFFu;

@@ -5,7 +5,8 @@ Tolerances: the device state is float32 and P2G scatters with float32
 atomics (summation order is not fixed), the oracle is float64.  Fields are
 compared with |gpu - oracle| <= rel * max|oracle| per field: positions 1e-6 of
 the box size (float32 rounding of ~1e-2 m coordinates), F 1e-5, velocities,
-affine C, grid velocity and momenta 2e-4.  Integer report counts are exact.
+affine C, grid velocity and momenta 2e-4 (C, which vanishes for rigid motion,
+against at least 1e-3 of its natural scale 4|v|/dx).  Integer report counts are exact.
 """
 
 import numpy as np
@@ -51,7 +52,9 @@ def _compare(s, o: M.MpmState, rep, ref: M.StepReport, *, x_scale):
     n = s.particle_count
     _close("x", s.x.to_numpy()[:n], o.x, 1e-6 * x_scale / max(np.abs(o.x).max(), 1e-30))
     _close("v", s.v.to_numpy()[:n], o.v, 2e-4, floor=1e-12)
-    _close("C", s.C.to_numpy()[:n], o.C, 2e-4, floor=1e-9)
+    # C is ~0 for rigid motion: its float32 rounding floor is 1e-3 of |v| * 4 / dx
+    c_nat = 4.0 * float(np.abs(o.v).max()) / min(s.dx)
+    _close("C", s.C.to_numpy()[:n], o.C, 2e-4, floor=max(1e-3 * c_nat, 1e-9))
     _close("F", s.F.to_numpy()[:n], o.F, 1e-5)
     _close("grid mass", s.grid_mass_kg.to_numpy(), ref.grid_mass, 1e-5)
     _close("grid velocity", s.grid_velocity_mps.to_numpy(), ref.grid_velocity, 2e-4, floor=1e-12)
@@ -224,5 +227,6 @@ def test_large_block_conserves_mass_and_momentum():
     gm = s.grid_mass_kg.tensor.double().sum().item()
     assert gm == pytest.approx(rep.total_mass_kg, rel=1e-5)
     np.testing.assert_allclose(rep.grid_momentum_kg_mps, rep.particle_momentum_kg_mps, rtol=1e-4)
-    np.testing.assert_allclose(s.v.to_numpy()[: s.particle_count].mean(axis=0), (0.03, -0.02, 0.01), rtol=1e-4)
+    v = s.v.to_numpy()[: s.particle_count].astype(np.float64)
+    np.testing.assert_allclose(v, np.broadcast_to((0.03, -0.02, 0.01), v.shape), rtol=1e-5)
     assert rep.max_abs_j == pytest.approx(1.0, abs=1e-4)
