@@ -10,11 +10,17 @@ field per step).  A step is one pressure solve from x0 = 0 to a float64 TRUE
 volume-weighted relative residual ||b - A x||_V / ||b||_V <= 1e-8.
 
 Variants (all on the GPU, same grid and right-hand sides):
+  mixed      : red-black Gauss-Seidel V-cycle on a semi-coarsened hierarchy
+               with fp16-stored large levels and float32 arithmetic, inside
+               flexible CG with float32 vectors and float64 solution, dots
+               and true-residual replacement every 10 iterations.
   plain_fp32 : the reference's V-cycle (Jacobi 2/24/2, x0.7 prolongation)
                in float32 inside flexible CG with float32 vectors, float64
                solution, dots and true-residual replacement every 16 its.
   ref64      : the same algorithm at the reference's precision (float64
                vectors, float32 coarse levels), recursive-residual test.
+The headline value is the fastest method variant (mixed when run); the
+speed-up over plain_fp32 is reported beside it.
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this build
 measures one GPU.
@@ -98,12 +104,20 @@ def make_rhs(solver, count: int, seed0: int):
 
 
 VARIANTS = {
-    "plain_fp32": dict(precision="fp32", true_residual=1, dtype="f32"),
-    "ref64": dict(precision="ref64", true_residual=0, dtype="f64"),
+    "mixed": dict(precision="mixed", true_residual=1, check=10, dtype="f32", timer="mix_rb",
+                  roofline_kernel="mixed V-cycle level-0 red-black colour pass (k_mix_rb<half,float>)",
+                  # colour pass: e of the other colour read at full-line granularity (2 B/cell, fp16),
+                  # r and flags of the pass colour (2 + 0.5 B/cell), e of the pass colour written (1 B/cell)
+                  bytes_per_cell=5.5),
+    "plain_fp32": dict(precision="fp32", true_residual=1, check=16, dtype="f32", timer="l0_sweep",
+                       roofline_kernel="level-0 weighted-Jacobi sweep (k_march, float32)",
+                       bytes_per_cell=13.0),  # p + rhs read, p write (4 B each), 1 B flags
+    "ref64": dict(precision="ref64", true_residual=0, check=16, dtype="f64", timer="l0_sweep",
+                  roofline_kernel="level-0 weighted-Jacobi sweep (k_march, float64)", bytes_per_cell=25.0),
 }
 
 
-def run_variant(name, solver, divs, steps, warmup, timer_kernel="l0_sweep"):
+def run_variant(name, solver, divs, steps, warmup):
     import ctypes
 
     import torch
@@ -113,13 +127,15 @@ def run_variant(name, solver, divs, steps, warmup, timer_kernel="l0_sweep"):
     cfg = VARIANTS[name]
     L = solver._lib
     opt = _lib.CgOptions()
-    opt.precision = {"fp32": _lib.HFSI_PREC_FP32, "ref64": _lib.HFSI_PREC_REF64}[cfg["precision"]]
+    opt.precision = {"fp32": _lib.HFSI_PREC_FP32, "ref64": _lib.HFSI_PREC_REF64,
+                     "mixed": _lib.HFSI_PREC_MIXED}[cfg["precision"]]
+    timer_kernel = cfg["timer"]
     opt.preconditioner = _lib.PRECONDITIONERS["fv_multigrid"]
     opt.max_iterations = 500
     opt.remove_mean = 1
     opt.outlet_zmin = 0
     opt.interface_active = 0
-    opt.check_interval = 16
+    opt.check_interval = cfg["check"]
     opt.restart_interval = 512
     opt.true_residual = cfg["true_residual"]
     opt.true_residual_at_exit = 1
@@ -166,6 +182,7 @@ def run_variant(name, solver, divs, steps, warmup, timer_kernel="l0_sweep"):
         "true_rel_residual_max": max(true_res),
         "all_converged": all(r["converged"] for r in records) and max(true_res) <= 1e-8,
         "vcycles_total": sum(r["vcycles"] for r in records),
+        "used_multigrid": records[-1]["used_multigrid"],
         "timer_launches": int(launches.value),
         "timer_ms_total": float(ms.value),
         "dtype": cfg["dtype"],
@@ -218,7 +235,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--variants", default="plain_fp32,ref64")
+    ap.add_argument("--variants", default="mixed,plain_fp32,ref64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args(argv)
@@ -237,7 +254,7 @@ def main(argv=None) -> int:
             print(f"unknown variant {v!r}; choose from {sorted(VARIANTS)}", file=sys.stderr)
             return 2
     print(f"[bench] building {args.size}^3 graded grid", file=sys.stderr, flush=True)
-    solver, w = make_solver(args.size, "fp32")
+    solver, w = make_solver(args.size, "mixed")
     n = solver.nx * solver.ny * solver.nz
     divs = make_rhs(solver, max(args.steps, 1), seed0=1000)
     results = {}
@@ -247,10 +264,10 @@ def main(argv=None) -> int:
         print(f"[bench] {v}: {results[v]['solves_per_s']:.2f} solves/s, "
               f"{results[v]['iterations_median']} its, true res {results[v]['true_rel_residual_max']:.2e}",
               file=sys.stderr, flush=True)
-    head_name = "plain_fp32" if "plain_fp32" in results else variants[0]
+    head_name = next(v for v in ("mixed", "plain_fp32", "ref64") if v in results)
     head = results[head_name]
-    # roofline of the level-0 smoother sweep of the headline variant
-    bytes_per_cell = 13.0 if head["dtype"] == "f32" else 25.0  # p + rhs(-r) read, out write, 1 B mask
+    # roofline of the level-0 smoother pass of the headline variant
+    bytes_per_cell = VARIANTS[head_name]["bytes_per_cell"]
     avg_s = head["timer_ms_total"] / max(head["timer_launches"], 1) / 1e3
     achieved = bytes_per_cell * n / avg_s / 1e9 if head["timer_launches"] else 0.0
     cpu = None
@@ -280,7 +297,7 @@ def main(argv=None) -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "level-0 weighted-Jacobi sweep (k_jacobi)",
+            "kernel": VARIANTS[head_name]["roofline_kernel"],
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -297,6 +314,8 @@ def main(argv=None) -> int:
     if "ref64" in results and "plain_fp32" in results:
         line["speedup_plain_fp32_vs_ref64"] = round(results["plain_fp32"]["solves_per_s"]
                                                     / results["ref64"]["solves_per_s"], 3)
+    if head_name != "plain_fp32" and "plain_fp32" in results:
+        line["speedup_vs_plain_fp32"] = round(head["solves_per_s"] / results["plain_fp32"]["solves_per_s"], 3)
     print(json.dumps(line), flush=True)
     return 0
 

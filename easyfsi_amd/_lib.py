@@ -117,13 +117,17 @@ SIGNATURES = {
     "hfsi_fluid_divergence_report": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P]),
     "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
     "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),
+    "hfsi_fluid_mixed_levels": (_I, [_P, ctypes.POINTER(_I)]),
+    "hfsi_fluid_mixed_level_shape": (ctypes.c_int, [_P, _I, _P]),
+    "hfsi_fluid_mixed_set_half_min_cells": (ctypes.c_int, [_P, ctypes.c_int64]),
     "hfsi_mpm_create": (ctypes.c_int, [ctypes.POINTER(_P), _I, _P, _P, _P, _P]),
     "hfsi_mpm_destroy": (ctypes.c_int, [_P]),
     "hfsi_mpm_step": (ctypes.c_int, [_P, ctypes.POINTER(MpmParticles), _I, ctypes.POINTER(MpmGrid), _F, _F, _F,
                                      _F, _I, _I, _P, _P]),
 }
 
-TIMER_KERNELS = {"l0_sweep": 1, "spmv_dot": 2, "l0_restrict": 3, "l0_prolong_sweep": 4}
+TIMER_KERNELS = {"l0_sweep": 1, "spmv_dot": 2, "l0_restrict": 3, "l0_prolong_sweep": 4, "mix_rb": 5,
+                 "cg_update": 6}
 
 _lock = threading.Lock()
 _lib = None

@@ -1,0 +1,334 @@
+// Mixed-precision multigrid preconditioner on a semi-coarsened hierarchy (gfx950).
+//
+// No reference counterpart: this is the preconditioner of the HFSI_PREC_MIXED
+// solve, built for the graded grids of the reference's fluid solver
+// (simulation_core/fluid.py:249-570), whose side cells reach width ratios of
+// 1.04^64 = 12.3 against the core.  Point smoothers stall on such anisotropic
+// cells under the reference's 2:1 coarsening, so:
+//
+//   * hierarchy: along each axis, adjacent cells are merged only while the
+//     merged width stays <= 3 T_l (T_l = 2^l x the smallest level-0 width:
+//     semi-coarsening); wide cells wait until the core has caught up, after
+//     which (all widths <= 2.5 T_l) coarsening is plain 2:1.  Per-axis tables give the parent
+//     of each fine cell and the 1 or 2 children of each coarse cell.
+//   * smoother: red-black Gauss-Seidel (omega = 1), red then black before the
+//     coarse correction, black then red after it (a symmetric V-cycle).
+//   * restriction: volume-weighted mean of the free children's residuals;
+//     a coarse cell with any obstacle child is an obstacle (the reference's
+//     coarse-mask rule).
+//   * prolongation: per-axis linear interpolation between the parent and the
+//     nearer neighbouring coarse cell (graded weights from the cell centres),
+//     renormalised over free coarse cells.
+//   * storage: e and rhs in fp16 on the large levels, fp32 on the small ones,
+//     fp32 arithmetic.  The level operators are divided by kappa = 1 / h_min^2
+//     and level 0 reads r / sigma (sigma = RMS of r, set each iteration on the
+//     device), so stored values stay O(1)..O(1e3) and far from fp16 limits;
+//     the result is written as z = e * sigma / kappa.
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include "fluid_kernels.hpp"
+
+namespace hfsi {
+
+template <typename T>
+__device__ inline float ldf(const T* p, uint32_t c) {
+  return (float)p[c];
+}
+template <>
+__device__ inline float ldf<__half>(const __half* p, uint32_t c) {
+  return __half2float(p[c]);
+}
+template <typename T>
+__device__ inline void stf(T* p, uint32_t c, float v) {
+  p[c] = (T)v;
+}
+template <>
+__device__ inline void stf<__half>(__half* p, uint32_t c, float v) {
+  p[c] = __float2half_rn(v);
+}
+
+// Per-axis transfer tables between level l (fine) and l + 1 (coarse).
+struct MixXfer {
+  const int32_t* par[3];    // fine index -> parent coarse index
+  const int32_t* pnb[3];    // fine index -> interpolation partner (coarse), == parent at walls
+  const float* pw[3];       // interpolation weight of the parent (partner gets 1 - pw)
+  const int32_t* first[3];  // coarse index -> first fine child
+  const int32_t* cnt[3];    // coarse index -> number of fine children (1 or 2)
+};
+
+// Scale of the level-0 right-hand side: rhs value = rs * stored (device scalar
+// when rs_dev is set, else 1).
+__device__ inline float rhs_scale(const double* rs_dev) { return rs_dev ? (float)(*rs_dev) : 1.0f; }
+
+// One colour pass of red-black Gauss-Seidel on L e = rhs, L = den e - nb.
+// A thread owns the k-pair (2kk, 2kk+1) of row (i, j) and updates the cell of
+// the pass colour, (i + j + k) % 2 == color.  FROMZERO: the iterate starts at
+// zero (neighbours read as 0) and the other cell of the pair is cleared.
+template <typename TE, typename TR, bool FROMZERO>
+__global__ void __launch_bounds__(kBlock) k_mix_rb(OpGeo<float> g, TE* e, const TR* rhs, const double* rs_dev,
+                                                    int color) {
+  const uint32_t kp = (g.nz + 1) >> 1;
+  const uint32_t npairs = g.nx * g.ny * kp;
+  const float rs = rhs_scale(rs_dev);
+  HFSI_CELL_LOOP(npairs) {
+    const uint32_t kk = c % kp;
+    const uint32_t t = c / kp;
+    const uint32_t j = t % g.ny, i = t / g.ny;
+    const uint32_t k = 2 * kk + ((color + i + j) & 1u);
+    const uint32_t row = i * g.sx + j * g.sy;
+    if (FROMZERO) {
+      const uint32_t ko = k ^ 1u;
+      if (ko < g.nz) stf<TE>(e, row + ko, 0.0f);
+    }
+    if (k >= g.nz) continue;
+    const uint32_t cc = row + k;
+    const uint8_t f = g.flags[cc];
+    float v = 0.0f;
+    if (!(f & kNotFree)) {
+      float nb = 0.0f, den = 0.0f;
+      if (f & kXm) { const float q = g.cl[0][i]; if (!FROMZERO) nb += q * ldf<TE>(e, cc - g.sx); den += q; }
+      if (f & kXp) { const float q = g.ch[0][i]; if (!FROMZERO) nb += q * ldf<TE>(e, cc + g.sx); den += q; }
+      if (f & kYm) { const float q = g.cl[1][j]; if (!FROMZERO) nb += q * ldf<TE>(e, cc - g.sy); den += q; }
+      if (f & kYp) { const float q = g.ch[1][j]; if (!FROMZERO) nb += q * ldf<TE>(e, cc + g.sy); den += q; }
+      if (f & kZm) { const float q = g.cl[2][k]; if (!FROMZERO) nb += q * ldf<TE>(e, cc - 1); den += q; }
+      if (f & kZp) { const float q = g.ch[2][k]; if (!FROMZERO) nb += q * ldf<TE>(e, cc + 1); den += q; }
+      if (g.outlet && k == 0) den += g.outlet_add;
+      if (den > 0.0f) v = (nb + rs * ldf<TR>(rhs, cc)) / den;
+    }
+    stf<TE>(e, cc, v);
+  }
+}
+
+// Residual of level f restricted to level f + 1: for each free coarse cell the
+// volume-weighted mean over its free children of rs * rhs - L e.
+template <typename TE, typename TR, typename TC>
+__global__ void __launch_bounds__(kBlock) k_mix_restrict(OpGeo<float> g, const TE* e, const TR* rhs,
+                                                          const double* rs_dev, MixXfer x, uint32_t cnx, uint32_t cny,
+                                                          uint32_t cnz, const uint8_t* cflags, TC* crhs) {
+  const uint32_t cn = cnx * cny * cnz;
+  const float rs = rhs_scale(rs_dev);
+  HFSI_CELL_LOOP(cn) {
+    const uint32_t K = c % cnz;
+    const uint32_t t = c / cnz;
+    const uint32_t J = t % cny, I = t / cny;
+    float out = 0.0f;
+    if (!(cflags[c] & kNotFree)) {
+      const int32_t i0 = x.first[0][I], j0 = x.first[1][J], k0 = x.first[2][K];
+      const int32_t ni = x.cnt[0][I], nj = x.cnt[1][J], nk = x.cnt[2][K];
+      float s = 0.0f, vs = 0.0f;
+      for (int32_t oi = 0; oi < ni; ++oi) {
+        const uint32_t i = (uint32_t)(i0 + oi);
+        const float wi = g.wf[0][i];
+        for (int32_t oj = 0; oj < nj; ++oj) {
+          const uint32_t j = (uint32_t)(j0 + oj);
+          const float wij = wi * g.wf[1][j];
+          for (int32_t ok = 0; ok < nk; ++ok) {
+            const uint32_t k = (uint32_t)(k0 + ok);
+            const uint32_t cc = i * g.sx + j * g.sy + k;
+            const uint8_t f = g.flags[cc];
+            if (f & kNotFree) continue;
+            float nb = 0.0f, den = 0.0f;
+            if (f & kXm) { const float q = g.cl[0][i]; nb += q * ldf<TE>(e, cc - g.sx); den += q; }
+            if (f & kXp) { const float q = g.ch[0][i]; nb += q * ldf<TE>(e, cc + g.sx); den += q; }
+            if (f & kYm) { const float q = g.cl[1][j]; nb += q * ldf<TE>(e, cc - g.sy); den += q; }
+            if (f & kYp) { const float q = g.ch[1][j]; nb += q * ldf<TE>(e, cc + g.sy); den += q; }
+            if (f & kZm) { const float q = g.cl[2][k]; nb += q * ldf<TE>(e, cc - 1); den += q; }
+            if (f & kZp) { const float q = g.ch[2][k]; nb += q * ldf<TE>(e, cc + 1); den += q; }
+            if (g.outlet && k == 0) den += g.outlet_add;
+            const float res = rs * ldf<TR>(rhs, cc) - (den * ldf<TE>(e, cc) - nb);
+            const float V = wij * g.wf[2][k];
+            s += V * res;
+            vs += V;
+          }
+        }
+      }
+      out = vs > 0.0f ? s / vs : 0.0f;
+    }
+    stf<TC>(crhs, c, out);
+  }
+}
+
+// e_f += P e_c on free fine cells (per-axis linear interpolation, renormalised
+// over free coarse cells).
+template <typename TF, typename TCo>
+__global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, MixXfer x, uint32_t cny, uint32_t cnz,
+                                                         const uint8_t* cflags, const TCo* ec) {
+  const uint32_t csx = cny * cnz, csy = cnz;
+  HFSI_CELL_LOOP(g.n) {
+    if (g.flags[c] & kNotFree) continue;
+    const uint32_t k = c % g.nz;
+    const uint32_t t = c / g.nz;
+    const uint32_t j = t % g.ny, i = t / g.ny;
+    const int32_t I[2] = {x.par[0][i], x.pnb[0][i]};
+    const int32_t J[2] = {x.par[1][j], x.pnb[1][j]};
+    const int32_t K[2] = {x.par[2][k], x.pnb[2][k]};
+    const float wI = x.pw[0][i], wJ = x.pw[1][j], wK = x.pw[2][k];
+    const float wi[2] = {wI, 1.0f - wI}, wj[2] = {wJ, 1.0f - wJ}, wk[2] = {wK, 1.0f - wK};
+    float s = 0.0f, ws = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const float w = wi[a] * wj[b] * wk[d];
+          if (w == 0.0f) continue;
+          const uint32_t cc = (uint32_t)I[a] * csx + (uint32_t)J[b] * csy + (uint32_t)K[d];
+          if (cflags[cc] & kNotFree) continue;
+          s += w * ldf<TCo>(ec, cc);
+          ws += w;
+        }
+    if (ws > 0.0f) stf<TF>(ef, c, ldf<TF>(ef, c) + s / ws);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small levels: one workgroup runs every level from `first` down to the
+// coarsest and back (global memory, L2-resident), with workgroup barriers
+// between passes.
+// ---------------------------------------------------------------------------
+
+constexpr int kMixMaxLevels = 24;
+constexpr int kMixTailThreads = 1024;
+
+struct MixTailLevel {
+  OpGeo<float> g;
+  float* e;
+  float* rhs;
+  MixXfer x;  // to the next coarser level
+};
+
+struct MixTailArgs {
+  int first, last;
+  MixTailLevel lv[kMixMaxLevels];
+};
+
+__device__ inline void mix_tail_rb(const OpGeo<float>& g, float* e, const float* rhs, int color, bool from_zero) {
+  const uint32_t kp = (g.nz + 1) >> 1;
+  const uint32_t npairs = g.nx * g.ny * kp;
+  for (uint32_t c = threadIdx.x; c < npairs; c += blockDim.x) {
+    const uint32_t kk = c % kp;
+    const uint32_t t = c / kp;
+    const uint32_t j = t % g.ny, i = t / g.ny;
+    const uint32_t k = 2 * kk + ((color + i + j) & 1u);
+    const uint32_t row = i * g.sx + j * g.sy;
+    if (from_zero) {
+      const uint32_t ko = k ^ 1u;
+      if (ko < g.nz) e[row + ko] = 0.0f;
+    }
+    if (k >= g.nz) continue;
+    const uint32_t cc = row + k;
+    const uint8_t f = g.flags[cc];
+    float v = 0.0f;
+    if (!(f & kNotFree)) {
+      float nb = 0.0f, den = 0.0f;
+      if (f & kXm) { const float q = g.cl[0][i]; if (!from_zero) nb += q * e[cc - g.sx]; den += q; }
+      if (f & kXp) { const float q = g.ch[0][i]; if (!from_zero) nb += q * e[cc + g.sx]; den += q; }
+      if (f & kYm) { const float q = g.cl[1][j]; if (!from_zero) nb += q * e[cc - g.sy]; den += q; }
+      if (f & kYp) { const float q = g.ch[1][j]; if (!from_zero) nb += q * e[cc + g.sy]; den += q; }
+      if (f & kZm) { const float q = g.cl[2][k]; if (!from_zero) nb += q * e[cc - 1]; den += q; }
+      if (f & kZp) { const float q = g.ch[2][k]; if (!from_zero) nb += q * e[cc + 1]; den += q; }
+      if (g.outlet && k == 0) den += g.outlet_add;
+      if (den > 0.0f) v = (nb + rhs[cc]) / den;
+    }
+    e[cc] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kMixTailThreads) k_mix_tail(const MixTailArgs* A, int pre, int coarse, int post) {
+  const int first = A->first, last = A->last;
+  for (int l = first; l < last; ++l) {
+    const MixTailLevel& L = A->lv[l];
+    for (int s = 0; s < pre; ++s) {
+      mix_tail_rb(L.g, L.e, L.rhs, 0, s == 0);
+      __syncthreads();
+      mix_tail_rb(L.g, L.e, L.rhs, 1, false);
+      __syncthreads();
+    }
+    // residual + restriction into level l + 1
+    const MixTailLevel& Cl = A->lv[l + 1];
+    const OpGeo<float>& g = L.g;
+    for (uint32_t c = threadIdx.x; c < Cl.g.n; c += blockDim.x) {
+      const uint32_t K = c % Cl.g.nz;
+      const uint32_t t = c / Cl.g.nz;
+      const uint32_t J = t % Cl.g.ny, I = t / Cl.g.ny;
+      float out = 0.0f;
+      if (!(Cl.g.flags[c] & kNotFree)) {
+        float s = 0.0f, vs = 0.0f;
+        for (int32_t oi = 0; oi < L.x.cnt[0][I]; ++oi)
+          for (int32_t oj = 0; oj < L.x.cnt[1][J]; ++oj)
+            for (int32_t ok = 0; ok < L.x.cnt[2][K]; ++ok) {
+              const uint32_t i = (uint32_t)(L.x.first[0][I] + oi), j = (uint32_t)(L.x.first[1][J] + oj),
+                             k = (uint32_t)(L.x.first[2][K] + ok);
+              const uint32_t cc = i * g.sx + j * g.sy + k;
+              const uint8_t f = g.flags[cc];
+              if (f & kNotFree) continue;
+              float nb = 0.0f, den = 0.0f;
+              if (f & kXm) { const float q = g.cl[0][i]; nb += q * L.e[cc - g.sx]; den += q; }
+              if (f & kXp) { const float q = g.ch[0][i]; nb += q * L.e[cc + g.sx]; den += q; }
+              if (f & kYm) { const float q = g.cl[1][j]; nb += q * L.e[cc - g.sy]; den += q; }
+              if (f & kYp) { const float q = g.ch[1][j]; nb += q * L.e[cc + g.sy]; den += q; }
+              if (f & kZm) { const float q = g.cl[2][k]; nb += q * L.e[cc - 1]; den += q; }
+              if (f & kZp) { const float q = g.ch[2][k]; nb += q * L.e[cc + 1]; den += q; }
+              if (g.outlet && k == 0) den += g.outlet_add;
+              const float V = g.wf[0][i] * g.wf[1][j] * g.wf[2][k];
+              s += V * (L.rhs[cc] - (den * L.e[cc] - nb));
+              vs += V;
+            }
+        out = vs > 0.0f ? s / vs : 0.0f;
+      }
+      Cl.rhs[c] = out;
+    }
+    __syncthreads();
+  }
+  {
+    const MixTailLevel& L = A->lv[last];
+    for (int s = 0; s < coarse; ++s) {
+      mix_tail_rb(L.g, L.e, L.rhs, 0, s == 0);
+      __syncthreads();
+      mix_tail_rb(L.g, L.e, L.rhs, 1, false);
+      __syncthreads();
+    }
+  }
+  for (int l = last - 1; l >= first; --l) {
+    const MixTailLevel& L = A->lv[l];
+    const MixTailLevel& Cl = A->lv[l + 1];
+    const OpGeo<float>& g = L.g;
+    const uint32_t csx = Cl.g.ny * Cl.g.nz, csy = Cl.g.nz;
+    for (uint32_t c = threadIdx.x; c < g.n; c += blockDim.x) {
+      if (g.flags[c] & kNotFree) continue;
+      const uint32_t k = c % g.nz;
+      const uint32_t t = c / g.nz;
+      const uint32_t j = t % g.ny, i = t / g.ny;
+      const int32_t I[2] = {L.x.par[0][i], L.x.pnb[0][i]};
+      const int32_t J[2] = {L.x.par[1][j], L.x.pnb[1][j]};
+      const int32_t K[2] = {L.x.par[2][k], L.x.pnb[2][k]};
+      const float wi[2] = {L.x.pw[0][i], 1.0f - L.x.pw[0][i]};
+      const float wj[2] = {L.x.pw[1][j], 1.0f - L.x.pw[1][j]};
+      const float wk[2] = {L.x.pw[2][k], 1.0f - L.x.pw[2][k]};
+      float s = 0.0f, ws = 0.0f;
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          for (int d = 0; d < 2; ++d) {
+            const float w = wi[a] * wj[b] * wk[d];
+            if (w == 0.0f) continue;
+            const uint32_t cc = (uint32_t)I[a] * csx + (uint32_t)J[b] * csy + (uint32_t)K[d];
+            if (Cl.g.flags[cc] & kNotFree) continue;
+            s += w * Cl.e[cc];
+            ws += w;
+          }
+      if (ws > 0.0f) L.e[c] += s / ws;
+    }
+    __syncthreads();
+    for (int s = 0; s < post; ++s) {
+      mix_tail_rb(L.g, L.e, L.rhs, 1, false);
+      __syncthreads();
+      mix_tail_rb(L.g, L.e, L.rhs, 0, false);
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace hfsi

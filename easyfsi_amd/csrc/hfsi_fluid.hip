@@ -24,6 +24,7 @@
 
 #include "../../include/hfsi.h"
 #include "fluid_march.hpp"
+#include "fluid_mixed.hpp"
 
 namespace hfsi {
 
@@ -43,6 +44,8 @@ constexpr uint32_t kTailLdsMax = 152 * 1024;  // dynamic LDS budget of the tail 
 struct Scalars {
   double rz, dAd, alpha, beta, rr, mz, mx, mr, mb, bnorm2, free_vol, code_dAd;
   int32_t code, pad;
+  // mixed V-cycle scaling: level-0 input r * vin (vin = 1 / RMS_V(r)), output z = e * vout (vout = RMS / kappa)
+  double vin, vout;
 };
 
 struct FinNone {
@@ -293,6 +296,126 @@ struct OpFill {
   T v;
   __device__ void operator()(uint32_t c, bool, uint8_t) const { a[c] = v; }
 };
+
+// ---------------------------------------------------------------------------
+// Mixed-precision solve helpers
+// ---------------------------------------------------------------------------
+
+// vin / vout of the mixed V-cycle from t = {<r,r>_V}: RMS = sqrt(<r,r>_V / V_free).
+struct FinVScale {
+  Scalars* s;
+  double kappa;
+  __device__ void operator()(const double* t) const { set(t[0]); }
+  __device__ void set(double rr) const {
+    const double V = s->free_vol > 0.0 ? s->free_vol : 1.0;
+    const double rms = sqrt(fmax(rr, 0.0) / V);
+    if (rms > 1.0e-300 && isfinite(rms)) {
+      s->vin = 1.0 / rms;
+      s->vout = rms / kappa;
+    } else {
+      s->vin = 1.0;
+      s->vout = 1.0 / kappa;
+    }
+  }
+};
+
+// Scale from the residual norm already in the scalars (after a true residual).
+__global__ void k_mix_scale_from_rr(Scalars* s, double kappa) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) FinVScale{s, kappa}.set(s->rr);
+}
+
+// x += alpha d, r -= alpha q (obstacle rows zeroed, fluid.py:4576-4587) and
+// <r,r>_V over free cells for the mixed V-cycle scale.
+template <typename Fin>
+__global__ void __launch_bounds__(kBlock) k_cg_update_rr(OpGeo<double> g, double* x, const float* d, float* r,
+                                                          const float* q, const Scalars* s, double* partials,
+                                                          unsigned int* counter, double* totals, Fin fin) {
+  __shared__ double lds[kBlock / 64];
+  double acc[1] = {0.0};
+  const double a = s->alpha;
+  HFSI_CELL_LOOP(g.n) {
+    const uint8_t f = g.flags[c];
+    if (f & kObst) {
+      x[c] = 0.0;
+      r[c] = 0.0f;
+      continue;
+    }
+    x[c] += a * (double)d[c];
+    const float rn = (float)((double)r[c] - a * (double)q[c]);
+    r[c] = rn;
+    if (!(f & kNotFree)) {
+      uint32_t i, j, k;
+      decompose(c, g.ny, g.nz, i, j, k);
+      acc[0] += (double)rn * (double)rn * volume64(g, i, j, k);
+    }
+  }
+  if (grid_sum<1>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+}
+
+// z = e * vout on free cells (0 elsewhere) and the PCG sums of FinDots5
+// (NS = 5: <r,z>, <z,q>, sum Vz, sum Vr, sum Vq) or FinDots3 (NS = 3: <r,z>,
+// sum Vz, sum Vr).
+template <int NS, typename TE, typename Fin>
+__global__ void __launch_bounds__(kBlock) k_mix_finish(OpGeo<double> g, const TE* e, const float* r, const float* q,
+                                                        float* z, const Scalars* s, double* partials,
+                                                        unsigned int* counter, double* totals, Fin fin) {
+  __shared__ double lds[NS * (kBlock / 64)];
+  double acc[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) acc[t] = 0.0;
+  const float vout = (float)s->vout;
+  HFSI_CELL_LOOP(g.n) {
+    const uint8_t f = g.flags[c];
+    if (f & kNotFree) {
+      z[c] = 0.0f;
+      continue;
+    }
+    const float zf = ldf<TE>(e, c) * vout;
+    z[c] = zf;
+    uint32_t i, j, k;
+    decompose(c, g.ny, g.nz, i, j, k);
+    const double V = volume64(g, i, j, k);
+    const double zv = (double)zf, rv = (double)r[c];
+    acc[0] += rv * zv * V;
+    if (NS == 5) {
+      const double qv = (double)q[c];
+      acc[1] += zv * qv * V;
+      acc[2] += zv * V;
+      acc[3] += rv * V;
+      acc[4] += qv * V;
+    } else {
+      acc[1] += zv * V;
+      acc[2] += rv * V;
+    }
+  }
+  if (grid_sum<NS>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+}
+
+// Coarse masks of the semi-coarsened hierarchy, through the per-axis child
+// tables: a coarse cell is solid (obstacle row, not free) only when no child
+// is free.  The reference's 2:1 hierarchy marks a coarse cell solid when any
+// child is an obstacle row (fluid.py:5334-5388); that thickens the solid on
+// every level, and the coarse correction next to it then stalls the V-cycle
+// (energy-norm contraction per cycle 0.73 against 0.19 with this rule on a
+// sphere, uniform 16^3).
+__global__ void k_mix_coarse_flags(uint32_t cnx, uint32_t cny, uint32_t cnz, const int32_t* fi0, const int32_t* fin,
+                                   const int32_t* fj0, const int32_t* fjn, const int32_t* fk0, const int32_t* fkn,
+                                   uint32_t fny, uint32_t fnz, const uint8_t* fflags, uint8_t* cflags) {
+  const uint32_t n = cnx * cny * cnz;
+  HFSI_CELL_LOOP(n) {
+    uint32_t ci, cj, ck;
+    decompose(c, cny, cnz, ci, cj, ck);
+    bool any_free = false;
+    for (int32_t oi = 0; oi < fin[ci] && !any_free; ++oi)
+      for (int32_t oj = 0; oj < fjn[cj] && !any_free; ++oj)
+        for (int32_t ok = 0; ok < fkn[ck] && !any_free; ++ok) {
+          const uint32_t fc =
+              (uint32_t)(fi0[ci] + oi) * fny * fnz + (uint32_t)(fj0[cj] + oj) * fnz + (uint32_t)(fk0[ck] + ok);
+          if (!(fflags[fc] & kNotFree)) any_free = true;
+        }
+    cflags[c] = any_free ? (uint8_t)0 : (uint8_t)(kObst | kNotFree);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Mask / geometry kernels
@@ -913,6 +1036,28 @@ struct LevelMem {
   float* rhs = nullptr;
 };
 
+// One level of the semi-coarsened hierarchy of the mixed V-cycle.  Level 0
+// shares the main level-0 flags; e / rhs buffers are sized for float32 and
+// hold fp16 when the level is stored in half precision.
+struct MixLevel {
+  uint32_t nx = 0, ny = 0, nz = 0, n = 0;
+  uint8_t* flags = nullptr;
+  float* cl[3] = {};   // coefficients / kappa
+  float* ch[3] = {};
+  float* wf[3] = {};   // float widths (restriction volumes)
+  double* wd[3] = {};
+  float outlet_add = 0.0f;  // 2 / w_z[0]^2 / kappa
+  void* e = nullptr;
+  void* rhs = nullptr;      // unused on level 0 (the CG residual is the input)
+  // transfer to level + 1 (fine side): parent, interpolation partner, parent weight
+  int32_t* par[3] = {};
+  int32_t* pnb[3] = {};
+  float* pw[3] = {};
+  // children in level - 1 (coarse side)
+  int32_t* first[3] = {};
+  int32_t* cnt[3] = {};
+};
+
 }  // namespace hfsi
 
 struct hfsi_fluid {
@@ -945,6 +1090,14 @@ struct hfsi_fluid {
   void* tail_args_d = nullptr;  // TailArgs<double,float>
   void* tail_args_f = nullptr;  // TailArgs<float,float>
   uint32_t tail_lds_bytes_d = 0, tail_lds_bytes_f = 0;
+  // mixed-precision V-cycle (HFSI_PREC_MIXED)
+  int ML = 0;
+  int mix_tail_first = 0;
+  double mix_kappa = 1.0;
+  uint32_t mix_half_min_cells = 1u << 20;  // levels with at least this many cells store fp16
+  hfsi::MixLevel ml[hfsi::kMixMaxLevels];
+  void* mix_tail_args = nullptr;  // MixTailArgs (device)
+  int mix_tail_outlet = -1;       // outlet switch the tail arguments were written for
   // live kernel timer: HIP events recorded on the launch stream around every
   // launch of one selected kernel (HFSI_TIMER_*), summed at flush/read time
   int timer_id = 0;
@@ -1359,6 +1512,185 @@ static int run_vcycle(hfsi_fluid* h, T0* P0, T0* T0buf, R0 rhs0, bool zero_init,
 // PCG driver
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Mixed-precision V-cycle driver
+// ---------------------------------------------------------------------------
+
+static OpGeo<float> mix_geo(const hfsi_fluid* h, int l, int outlet) {
+  const MixLevel& m = h->ml[l];
+  OpGeo<float> g;
+  g.nx = m.nx; g.ny = m.ny; g.nz = m.nz; g.n = m.n; g.sx = m.ny * m.nz; g.sy = m.nz;
+  g.flags = m.flags;
+  g.fw = nullptr;
+  g.diag = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    g.cl[a] = m.cl[a];
+    g.ch[a] = m.ch[a];
+    g.wf[a] = m.wf[a];
+    g.wd[a] = m.wd[a];
+  }
+  g.outlet_add = m.outlet_add;
+  g.outlet = outlet;
+  return g;
+}
+
+static MixXfer mix_xfer(const hfsi_fluid* h, int l) {
+  MixXfer x;
+  for (int a = 0; a < 3; ++a) {
+    x.par[a] = h->ml[l].par[a];
+    x.pnb[a] = h->ml[l].pnb[a];
+    x.pw[a] = h->ml[l].pw[a];
+    x.first[a] = h->ml[l + 1].first[a];
+    x.cnt[a] = h->ml[l + 1].cnt[a];
+  }
+  return x;
+}
+
+// fp16 storage for the large grid-launched levels; the workgroup tail is float.
+static inline bool mix_half(const hfsi_fluid* h, int l) {
+  return (l < h->mix_tail_first || h->ML == 1) && h->ml[l].n >= h->mix_half_min_cells;
+}
+
+static int write_mix_tail_args(hfsi_fluid* h, int outlet, hipStream_t st) {
+  if (h->ML < 2 || h->mix_tail_outlet == outlet) return HFSI_OK;
+  MixTailArgs A{};
+  A.first = h->mix_tail_first;
+  A.last = h->ML - 1;
+  for (int l = A.first; l <= A.last; ++l) {
+    A.lv[l].g = mix_geo(h, l, outlet);
+    A.lv[l].e = (float*)h->ml[l].e;
+    A.lv[l].rhs = (float*)h->ml[l].rhs;
+    if (l < A.last) A.lv[l].x = mix_xfer(h, l);
+  }
+  HFSI_CHECK(hipMemcpyAsync(h->mix_tail_args, &A, sizeof(A), hipMemcpyHostToDevice, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  h->mix_tail_outlet = outlet;
+  return HFSI_OK;
+}
+
+template <typename TE, typename TR>
+static void mix_rb_pass(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int color,
+                        bool from_zero, hipStream_t st) {
+  const MixLevel& m = h->ml[l];
+  const uint32_t npairs = m.nx * m.ny * ((m.nz + 1) / 2);
+  OpGeo<float> g = mix_geo(h, l, outlet);
+  if (from_zero) {
+    hipLaunchKernelGGL((k_mix_rb<TE, TR, true>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs, color);
+  } else {
+    if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
+    hipLaunchKernelGGL((k_mix_rb<TE, TR, false>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs,
+                       color);
+    if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
+  }
+}
+
+// pre / post smoothing on level l: red then black (pre), black then red (post)
+template <typename TE, typename TR>
+static void mix_smooth(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int sweeps, bool pre,
+                       hipStream_t st) {
+  for (int s = 0; s < sweeps; ++s) {
+    const int c0 = pre ? 0 : 1;
+    mix_rb_pass<TE, TR>(h, l, outlet, e, rhs, rs, c0, pre && s == 0, st);
+    mix_rb_pass<TE, TR>(h, l, outlet, e, rhs, rs, 1 - c0, false, st);
+  }
+}
+
+template <typename TE, typename TR, typename TC>
+static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TE* e, const TR* rhs, const double* rs,
+                         hipStream_t st) {
+  const MixLevel& c = h->ml[l + 1];
+  hipLaunchKernelGGL((k_mix_restrict<TE, TR, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet),
+                     e, rhs, rs, mix_xfer(h, l), c.nx, c.ny, c.nz, (const uint8_t*)c.flags, (TC*)c.rhs);
+}
+
+template <typename TF, typename TCo>
+static void mix_prolong(hfsi_fluid* h, int l, int outlet, TF* ef, hipStream_t st) {
+  const MixLevel& c = h->ml[l + 1];
+  hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(h->ml[l].n)), dim3(kBlock), 0, st,
+                     mix_geo(h, l, outlet), ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags,
+                     (const TCo*)c.e);
+}
+
+// Down-leg of level l (smoothing + restriction), with the storage types of l
+// and l + 1 resolved at run time.
+template <typename TE, typename TR>
+static void mix_down(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int pre,
+                     hipStream_t st) {
+  mix_smooth<TE, TR>(h, l, outlet, e, rhs, rs, pre, true, st);
+  if (mix_half(h, l + 1))
+    mix_restrict<TE, TR, __half>(h, l, outlet, e, rhs, rs, st);
+  else
+    mix_restrict<TE, TR, float>(h, l, outlet, e, rhs, rs, st);
+}
+
+template <typename TE, typename TR>
+static void mix_up(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int post,
+                   hipStream_t st) {
+  if (mix_half(h, l + 1))
+    mix_prolong<TE, __half>(h, l, outlet, e, st);
+  else
+    mix_prolong<TE, float>(h, l, outlet, e, st);
+  mix_smooth<TE, TR>(h, l, outlet, e, rhs, rs, post, false, st);
+}
+
+// One mixed V-cycle on A e = r0 * vin (scaled level operators); leaves the
+// level-0 iterate in ml[0].e (fp16 or float per mix_half(h, 0)).
+static int run_mix_vcycle(hfsi_fluid* h, const float* r0, int pre, int coarse, int post, int outlet, hipStream_t st) {
+  HFSI_TRY(write_mix_tail_args(h, outlet, st));
+  const double* vin = (const double*)((const char*)h->scal + offsetof(Scalars, vin));
+  const int T = h->ML > 1 ? h->mix_tail_first : 1;
+  if (h->ML == 1) {
+    if (mix_half(h, 0))
+      mix_smooth<__half, float>(h, 0, outlet, (__half*)h->ml[0].e, r0, vin, coarse, true, st);
+    else
+      mix_smooth<float, float>(h, 0, outlet, (float*)h->ml[0].e, r0, vin, coarse, true, st);
+    HFSI_LAUNCH_CHECK();
+    return HFSI_OK;
+  }
+  for (int l = 0; l < T; ++l) {
+    const bool he = mix_half(h, l);
+    if (l == 0) {
+      if (he) mix_down<__half, float>(h, 0, outlet, (__half*)h->ml[0].e, r0, vin, pre, st);
+      else mix_down<float, float>(h, 0, outlet, (float*)h->ml[0].e, r0, vin, pre, st);
+    } else if (he) {
+      mix_down<__half, __half>(h, l, outlet, (__half*)h->ml[l].e, (const __half*)h->ml[l].rhs, nullptr, pre, st);
+    } else {
+      mix_down<float, float>(h, l, outlet, (float*)h->ml[l].e, (const float*)h->ml[l].rhs, nullptr, pre, st);
+    }
+  }
+  hipLaunchKernelGGL(k_mix_tail, dim3(1), dim3(kMixTailThreads), 0, st, (const MixTailArgs*)h->mix_tail_args, pre,
+                     coarse, post);
+  for (int l = T - 1; l >= 0; --l) {
+    const bool he = mix_half(h, l);
+    if (l == 0) {
+      if (he) mix_up<__half, float>(h, 0, outlet, (__half*)h->ml[0].e, r0, vin, post, st);
+      else mix_up<float, float>(h, 0, outlet, (float*)h->ml[0].e, r0, vin, post, st);
+    } else if (he) {
+      mix_up<__half, __half>(h, l, outlet, (__half*)h->ml[l].e, (const __half*)h->ml[l].rhs, nullptr, post, st);
+    } else {
+      mix_up<float, float>(h, l, outlet, (float*)h->ml[l].e, (const float*)h->ml[l].rhs, nullptr, post, st);
+    }
+  }
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+// z = M r (mixed V-cycle) with the NS PCG sums of z (FinDots3 / FinDots5).
+template <int NS, typename Fin>
+static int mix_precondition(hfsi_fluid* h, const float* r, const float* q, float* z, int pre, int coarse, int post,
+                            int outlet, Fin fin, hipStream_t st) {
+  HFSI_TRY(run_mix_vcycle(h, r, pre, coarse, post, outlet, st));
+  OpGeo<double> g64 = geo<double>(h, 0, outlet);
+  if (mix_half(h, 0))
+    hipLaunchKernelGGL((k_mix_finish<NS, __half, Fin>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64,
+                       (const __half*)h->ml[0].e, r, q, z, h->scal, h->partials, h->counter, h->totals, fin);
+  else
+    hipLaunchKernelGGL((k_mix_finish<NS, float, Fin>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64,
+                       (const float*)h->ml[0].e, r, q, z, h->scal, h->partials, h->counter, h->totals, fin);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
 template <typename C, typename TV>
 struct Vecs {
   TV *r, *d, *q, *z, *t;
@@ -1391,7 +1723,7 @@ static int precondition(hfsi_fluid* h, bool use_mg, int pre, int coarse, int pos
 
 template <typename C, typename TV>
 static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const double* iface, double* x,
-                    const hfsi_cg_options& o, hfsi_cg_stats& S, hipStream_t st) {
+                    const hfsi_cg_options& o, hfsi_cg_stats& S, hipStream_t st, bool mixed = false) {
   Vecs<C, TV> v = vecs<C, TV>(h);
   const int outlet = o.outlet_zmin ? 1 : 0;
   const int rm = o.remove_mean ? 1 : 0;
@@ -1463,8 +1795,24 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
                  (o.preconditioner == HFSI_PRECOND_AUTO && !h->uniform));
   const bool light = o.preconditioner == HFSI_PRECOND_MG_LIGHT;
   const int pre = light ? 1 : 2, coarse = light ? 12 : 24, post = light ? 1 : 2;
+  // mixed V-cycle: float32 CG vectors, no velocity-Dirichlet or interface terms
+  bool use_mix = mixed && use_mg && std::is_same<TV, float>::value && h->ML >= 1 && !h->has_vd && !h->has_diag;
+  const double kappa = h->mix_kappa;
+  auto mix_scale = [&]() -> int {
+    hipLaunchKernelGGL(k_mix_scale_from_rr, dim3(1), dim3(64), 0, st, sc, kappa);
+    HFSI_LAUNCH_CHECK();
+    return HFSI_OK;
+  };
 
   auto precond_and_rz = [&](bool project) -> int {
+    if (use_mix) {
+      HFSI_TRY(mix_scale());
+      HFSI_TRY((mix_precondition<3>(h, (const float*)v.r, (const float*)v.q, (float*)v.z, pre, coarse, post, outlet,
+                                    FinDots3{sc, project ? rm : 0}, st)));
+      S.vcycles += 1;
+      if (project && rm) S.mean_projection_count += 1;
+      return HFSI_OK;
+    }
     HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st)));
     if (use_mg) S.vcycles += 1;
     HFSI_TRY(launch_sums<3>(h, g64, OpDots3<TV, TV>{v.r, v.z}, FinDots3{sc, project ? rm : 0}, st));
@@ -1477,11 +1825,12 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   double rz = h->host_scal->rz;
   if (!std::isfinite(rz) || std::fabs(rz) <= 1.0e-300) {
     use_mg = false;
+    use_mix = false;
     HFSI_TRY(precond_and_rz(false));
     HFSI_TRY(sync_scalars(h, st));
     rz = h->host_scal->rz;
   }
-  S.used_multigrid = use_mg ? 1 : 0;
+  S.used_multigrid = use_mix ? 2 : (use_mg ? 1 : 0);
   if (!std::isfinite(rz) || rz <= 0.0) {
     S.breakdown_kind = HFSI_BREAKDOWN_INITIAL_DOT;
     return HFSI_OK;
@@ -1500,7 +1849,16 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
                          h->partials, h->counter, h->totals, FinDad{sc});
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
     HFSI_LAUNCH_CHECK();
-    HFSI_TRY(launch_pointwise(h, 0, OpCgUpdate<TV, TV, TV>{x, v.d, v.r, v.q, sc}, st));
+    if (use_mix) {
+      timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
+      hipLaunchKernelGGL((k_cg_update_rr<FinVScale>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x,
+                         (const float*)v.d, (float*)v.r, (const float*)v.q, sc, h->partials, h->counter, h->totals,
+                         FinVScale{sc, kappa});
+      timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
+      HFSI_LAUNCH_CHECK();
+    } else {
+      HFSI_TRY(launch_pointwise(h, 0, OpCgUpdate<TV, TV, TV>{x, v.d, v.r, v.q, sc}, st));
+    }
     const bool project = rm && (it % 16 == 0);
     const bool check = (it % check_every == 0) || it == max_it;
     if (project) {
@@ -1514,6 +1872,7 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
       if (exact_checks) {
         HFSI_TRY(true_residual(true));
         S.true_residual_replacements += 1;
+        if (use_mix) HFSI_TRY(mix_scale());
       } else if (!project) {
         HFSI_TRY(launch_sums<1>(h, g64, OpDot<TV, TV>{v.r, v.r}, FinRR{sc}, st));
       }
@@ -1565,7 +1924,13 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
       }
     }
     FusedDots fd{v.q, sc, rm, use_mg ? 1 : 0, false};
-    HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st, &fd)));
+    if (use_mix) {
+      HFSI_TRY((mix_precondition<5>(h, (const float*)v.r, (const float*)v.q, (float*)v.z, pre, coarse, post, outlet,
+                                    FinDots5{sc, rm, 1}, st)));
+      fd.done = true;
+    } else {
+      HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st, &fd)));
+    }
     if (use_mg) S.vcycles += 1;
     if (rm) S.mean_projection_count += 1;
     if (!fd.done)
@@ -1676,6 +2041,174 @@ int hfsi_device_ok(void) {
   return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
 }
 
+// Semi-coarsened hierarchy of the mixed V-cycle from the level-0 float widths
+// and float64 coefficients of the main operator.  With T_l = 2^l h_min (h_min:
+// smallest level-0 width over all axes), along each axis cells i and i + 1 of
+// level l merge when w_i + w_{i+1} <= 3 T_l, or always once the level's
+// largest width is <= 2.5 T_l; coarsening stops when an axis has <= 3 cells or
+// no pair merges.  On uniform grids this is the reference's 2:1 hierarchy.
+static int build_mix_hierarchy(hfsi_fluid* h, const std::vector<float> wf0[3], const std::vector<double> cl0[3],
+                               const std::vector<double> ch0[3]) {
+  double hmin0 = 1e300;
+  for (int a = 0; a < 3; ++a)
+    for (float w : wf0[a]) hmin0 = std::min(hmin0, (double)w);
+  if (!(hmin0 > 0.0)) {
+    set_error("hfsi_fluid_create: non-positive cell width");
+    return HFSI_ERR_ARG;
+  }
+  const double kappa = 1.0 / (hmin0 * hmin0);
+  h->mix_kappa = kappa;
+  struct HostLevel {
+    std::vector<double> w[3], ctr[3];
+    std::vector<int32_t> par[3], first[3], cnt[3];
+  };
+  std::vector<HostLevel> H(1);
+  for (int a = 0; a < 3; ++a) H[0].w[a].assign(wf0[a].begin(), wf0[a].end());
+  auto centres = [](HostLevel& L) {
+    for (int a = 0; a < 3; ++a) {
+      L.ctr[a].resize(L.w[a].size());
+      double face = 0.0;
+      for (size_t i = 0; i < L.w[a].size(); ++i) {
+        L.ctr[a][i] = face + 0.5 * L.w[a][i];
+        face += L.w[a][i];
+      }
+    }
+  };
+  centres(H[0]);
+  while ((int)H.size() < kMixMaxLevels) {
+    HostLevel& F = H.back();
+    size_t mn = std::min(F.w[0].size(), std::min(F.w[1].size(), F.w[2].size()));
+    if (mn <= 3) break;
+    // nominal width of this level: the level-0 minimum doubled per level (an odd
+    // leftover cell narrower than that does not hold the other pairs back)
+    const double hmin = hmin0 * std::ldexp(1.0, (int)H.size() - 1);
+    double hmax = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (double w : F.w[a]) hmax = std::max(hmax, w);
+    // once every width is within 2.5x of the nominal width, plain 2:1 coarsening
+    const bool full = hmax <= 2.5 * hmin;
+    HostLevel Cn;
+    bool any = false;
+    for (int a = 0; a < 3; ++a) {
+      const size_t n = F.w[a].size();
+      F.par[a].assign(n, 0);
+      for (size_t i = 0; i < n;) {
+        const int32_t ci = (int32_t)Cn.w[a].size();
+        Cn.first[a].push_back((int32_t)i);
+        if (i + 1 < n && (full || F.w[a][i] + F.w[a][i + 1] <= 3.0 * hmin)) {
+          Cn.w[a].push_back(F.w[a][i] + F.w[a][i + 1]);
+          Cn.cnt[a].push_back(2);
+          F.par[a][i] = F.par[a][i + 1] = ci;
+          i += 2;
+          any = true;
+        } else {
+          Cn.w[a].push_back(F.w[a][i]);
+          Cn.cnt[a].push_back(1);
+          F.par[a][i] = ci;
+          i += 1;
+        }
+      }
+    }
+    if (!any) {
+      for (int a = 0; a < 3; ++a) F.par[a].clear();
+      break;
+    }
+    centres(Cn);
+    H.push_back(std::move(Cn));
+  }
+  const int ML = (int)H.size();
+  h->ML = ML;
+  h->mix_tail_first = ML > 1 ? ML - 1 : 0;
+  for (int l = 1; l < ML; ++l) {
+    const size_t n = H[l].w[0].size() * H[l].w[1].size() * H[l].w[2].size();
+    if (n <= kTailCells) {
+      h->mix_tail_first = l;
+      break;
+    }
+  }
+  int rc = HFSI_OK;
+  for (int l = 0; l < ML && rc == HFSI_OK; ++l) {
+    HostLevel& L = H[l];
+    MixLevel& m = h->ml[l];
+    m.nx = (uint32_t)L.w[0].size();
+    m.ny = (uint32_t)L.w[1].size();
+    m.nz = (uint32_t)L.w[2].size();
+    m.n = m.nx * m.ny * m.nz;
+    if (l == 0) {
+      m.flags = h->lv[0].flags;
+    } else {
+      rc |= dalloc(h, &m.flags, m.n);
+      rc |= dalloc(h, (float**)&m.rhs, m.n);
+    }
+    rc |= dalloc(h, (float**)&m.e, m.n);
+    for (int a = 0; a < 3 && rc == HFSI_OK; ++a) {
+      const size_t n = L.w[a].size();
+      std::vector<float> cl(n, 0.0f), ch(n, 0.0f), wf(n);
+      std::vector<double> wd(n);
+      for (size_t i = 0; i < n; ++i) {
+        wf[i] = (float)L.w[a][i];
+        wd[i] = L.w[a][i];
+        if (l == 0) {
+          cl[i] = (float)(cl0[a][i] / kappa);
+          ch[i] = (float)(ch0[a][i] / kappa);
+        } else {
+          if (i > 0) cl[i] = (float)(1.0 / (L.w[a][i] * (L.ctr[a][i] - L.ctr[a][i - 1])) / kappa);
+          if (i + 1 < n) ch[i] = (float)(1.0 / (L.w[a][i] * (L.ctr[a][i + 1] - L.ctr[a][i])) / kappa);
+        }
+      }
+      if (a == 2) m.outlet_add = (float)(2.0 / (L.w[2][0] * L.w[2][0]) / kappa);
+      rc |= dalloc(h, &m.cl[a], n);
+      rc |= dalloc(h, &m.ch[a], n);
+      rc |= dalloc(h, &m.wf[a], n);
+      rc |= dalloc(h, &m.wd[a], n);
+      if (rc != HFSI_OK) break;
+      (void)hipMemcpy(m.cl[a], cl.data(), n * sizeof(float), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.ch[a], ch.data(), n * sizeof(float), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.wf[a], wf.data(), n * sizeof(float), hipMemcpyHostToDevice);
+      (void)hipMemcpy(m.wd[a], wd.data(), n * sizeof(double), hipMemcpyHostToDevice);
+      if (l + 1 < ML) {
+        // interpolation partner: the coarse neighbour on the side of the fine centre
+        const HostLevel& Cn = H[l + 1];
+        const int32_t cn = (int32_t)Cn.w[a].size();
+        std::vector<int32_t> pnb(n);
+        std::vector<float> pw(n);
+        for (size_t i = 0; i < n; ++i) {
+          const int32_t I = L.par[a][i];
+          const double x = L.ctr[a][i], X = Cn.ctr[a][I];
+          int32_t nb = I;
+          double w = 1.0;
+          if (std::fabs(x - X) > 1.0e-9 * L.w[a][i]) {
+            const int32_t cand = x < X ? I - 1 : I + 1;
+            if (cand >= 0 && cand < cn) {
+              nb = cand;
+              const double Xn = Cn.ctr[a][cand];
+              w = (Xn - x) / (Xn - X);
+            }
+          }
+          pnb[i] = nb;
+          pw[i] = (float)w;
+        }
+        rc |= dalloc(h, &m.par[a], n);
+        rc |= dalloc(h, &m.pnb[a], n);
+        rc |= dalloc(h, &m.pw[a], n);
+        if (rc != HFSI_OK) break;
+        (void)hipMemcpy(m.par[a], L.par[a].data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+        (void)hipMemcpy(m.pnb[a], pnb.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+        (void)hipMemcpy(m.pw[a], pw.data(), n * sizeof(float), hipMemcpyHostToDevice);
+      }
+      if (l > 0) {
+        rc |= dalloc(h, &m.first[a], n);
+        rc |= dalloc(h, &m.cnt[a], n);
+        if (rc != HFSI_OK) break;
+        (void)hipMemcpy(m.first[a], L.first[a].data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+        (void)hipMemcpy(m.cnt[a], L.cnt[a].data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+      }
+    }
+  }
+  if (rc == HFSI_OK && ML > 1) rc |= dalloc(h, (char**)&h->mix_tail_args, sizeof(MixTailArgs));
+  return rc;
+}
+
 static void axis_metrics(const std::vector<double>& w, double origin, std::vector<float>& wf,
                          std::vector<float>& df) {
   // centres by running faces (fluid.py:66-87); float32 storage like the reference's fields
@@ -1705,8 +2238,10 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
     set_error("hfsi_fluid_create: grid too large for 32-bit cell indices");
     return HFSI_ERR_ARG;
   }
-  if (!(precision_mask & (HFSI_PREC_REF64 | HFSI_PREC_FP32))) {
-    set_error("hfsi_fluid_create: precision_mask must include HFSI_PREC_REF64 or HFSI_PREC_FP32");
+  if (!(precision_mask & (HFSI_PREC_REF64 | HFSI_PREC_FP32 | HFSI_PREC_MIXED)) ||
+      (precision_mask & ~(HFSI_PREC_REF64 | HFSI_PREC_FP32 | HFSI_PREC_MIXED))) {
+    set_error("hfsi_fluid_create: precision_mask must be a non-empty combination of HFSI_PREC_REF64, "
+              "HFSI_PREC_FP32 and HFSI_PREC_MIXED");
     return HFSI_ERR_ARG;
   }
   hipStream_t st = (hipStream_t)stream;
@@ -1747,6 +2282,8 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
   }
   h->L = L;
   int rc = HFSI_OK;
+  std::vector<float> wf0[3];
+  std::vector<double> cl0[3], ch0[3];
   for (int l = 0; l < L && rc == HFSI_OK; ++l) {
     LevelMem& m = h->lv[l];
     m.nx = (uint32_t)W[l][0].size();
@@ -1772,6 +2309,11 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
         chf[i] = (float)ch[i];
       }
       if (a == 2) m.outlet_add = 2.0 / ((double)wf[0] * (double)wf[0]);
+      if (l == 0) {
+        wf0[a] = wf;
+        cl0[a] = cl;
+        ch0[a] = ch;
+      }
       rc |= dalloc(h, &m.cl[a], n);
       rc |= dalloc(h, &m.ch[a], n);
       rc |= dalloc(h, &m.clf[a], n);
@@ -1812,7 +2354,7 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
       rc |= dalloc(h, &h->q64, n0);
       rc |= dalloc(h, &h->z64, n0);
     }
-    if (precision_mask & HFSI_PREC_FP32) {
+    if (precision_mask & (HFSI_PREC_FP32 | HFSI_PREC_MIXED)) {
       rc |= dalloc(h, &h->r32, n0);
       rc |= dalloc(h, &h->d32, n0);
       rc |= dalloc(h, &h->q32, n0);
@@ -1839,6 +2381,7 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
     rc |= dalloc(h, (char**)&h->tail_args_d, sizeof(TailArgs<double, float>));
     rc |= dalloc(h, (char**)&h->tail_args_f, sizeof(TailArgs<float, float>));
   }
+  if (rc == HFSI_OK && (precision_mask & HFSI_PREC_MIXED)) rc = build_mix_hierarchy(h, wf0, cl0, ch0);
   if (rc == HFSI_OK) {
     (void)hipFuncSetAttribute((const void*)k_vcycle_tail_lds<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kTailLdsMax);
@@ -1850,6 +2393,11 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
     // uninitialised open-face bits before the first hfsi_fluid_set_masks.
     for (int l = 0; l < L; ++l) {
       LevelMem& m = h->lv[l];
+      (void)hipMemsetAsync(m.flags, 0, m.n, st);
+      hipLaunchKernelGGL(k_open_bits, dim3(blocks_for(m.n)), dim3(kBlock), 0, st, m.nx, m.ny, m.nz, m.flags);
+    }
+    for (int l = 1; l < h->ML; ++l) {
+      MixLevel& m = h->ml[l];
       (void)hipMemsetAsync(m.flags, 0, m.n, st);
       hipLaunchKernelGGL(k_open_bits, dim3(blocks_for(m.n)), dim3(kBlock), 0, st, m.nx, m.ny, m.nz, m.flags);
     }
@@ -1927,12 +2475,54 @@ int hfsi_fluid_set_masks(hfsi_fluid* h, const int32_t* obstacle, const int32_t* 
     hipLaunchKernelGGL(k_coarse_masks, dim3(blocks_for(c.n)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_open_bits, dim3(blocks_for(c.n)), dim3(kBlock), 0, st, c.nx, c.ny, c.nz, c.flags);
   }
+  for (int l = 1; l < h->ML; ++l) {
+    const MixLevel& f = h->ml[l - 1];
+    const MixLevel& c = h->ml[l];
+    hipLaunchKernelGGL(k_mix_coarse_flags, dim3(blocks_for(c.n)), dim3(kBlock), 0, st, c.nx, c.ny, c.nz, c.first[0],
+                       c.cnt[0], c.first[1], c.cnt[1], c.first[2], c.cnt[2], f.ny, f.nz, (const uint8_t*)f.flags,
+                       c.flags);
+    hipLaunchKernelGGL(k_open_bits, dim3(blocks_for(c.n)), dim3(kBlock), 0, st, c.nx, c.ny, c.nz, c.flags);
+  }
+  h->mix_tail_outlet = -1;
   HFSI_LAUNCH_CHECK();
   // free volume (V over free cells) into the device scalars
   OpGeo<double> g = geo<double>(h, 0, 0);
   HFSI_TRY(launch_sums<1>(h, g, OpVolume{}, FinFreeVol{h->scal}, st));
   HFSI_TRY(write_tail_args<double>(h, h->tail_args_d, st));
   HFSI_TRY(write_tail_args<float>(h, h->tail_args_f, st));
+  return HFSI_OK;
+}
+
+int hfsi_fluid_mixed_levels(const hfsi_fluid* h, int32_t* half_levels) {
+  if (!h) {
+    set_error("hfsi_fluid_mixed_levels: invalid arguments");
+    return -1;
+  }
+  if (half_levels) {
+    int nh = 0;
+    for (int l = 0; l < h->ML; ++l) nh += mix_half(h, l) ? 1 : 0;
+    *half_levels = nh;
+  }
+  return h->ML;
+}
+
+int hfsi_fluid_mixed_level_shape(const hfsi_fluid* h, int32_t level, int32_t* shape3) {
+  if (!h || !shape3 || level < 0 || level >= h->ML) {
+    set_error("hfsi_fluid_mixed_level_shape: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  shape3[0] = (int32_t)h->ml[level].nx;
+  shape3[1] = (int32_t)h->ml[level].ny;
+  shape3[2] = (int32_t)h->ml[level].nz;
+  return HFSI_OK;
+}
+
+int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells) {
+  if (!h || min_cells < 0) {
+    set_error("hfsi_fluid_mixed_set_half_min_cells: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  h->mix_half_min_cells = (uint32_t)std::min<int64_t>(min_cells, (int64_t)UINT32_MAX);
   return HFSI_OK;
 }
 
@@ -2007,6 +2597,13 @@ int hfsi_fluid_vcycle(hfsi_fluid* h, const double* r, double* z, int32_t precisi
     HFSI_TRY((run_vcycle<float, float, LdNeg<float, float>>(h, h->z32, h->t32, LdNeg<float, float>{h->r32}, true,
                                                              pre, coarse, post, outlet, st)));
     HFSI_TRY(launch_pointwise(h, 0, OpCopy<float, double>{h->z32, z}, st));
+  } else if (precision == HFSI_PREC_MIXED) {
+    // z = M r with the scale taken from r itself (vin = 1 / RMS_V(r))
+    HFSI_TRY(launch_pointwise(h, 0, OpCopy<double, float>{r, h->r32}, st));
+    OpGeo<double> g64 = geo<double>(h, 0, outlet);
+    HFSI_TRY(launch_sums<1>(h, g64, OpDot<float, float>{h->r32, h->r32}, FinVScale{h->scal, h->mix_kappa}, st));
+    HFSI_TRY((mix_precondition<3>(h, h->r32, h->q32, h->z32, pre, coarse, post, outlet, FinNone{}, st)));
+    HFSI_TRY(launch_pointwise(h, 0, OpCopy<float, double>{h->z32, z}, st));
   } else {
     set_error("hfsi_fluid_vcycle: unsupported precision");
     return HFSI_ERR_ARG;
@@ -2036,6 +2633,8 @@ int hfsi_fluid_solve_cg(hfsi_fluid* h, const float* divergence, const float* sou
     return solve_cg<double, double>(h, divergence, source, iface_rhs, pressure, *opt, S, st);
   if (opt->precision == HFSI_PREC_FP32)
     return solve_cg<float, float>(h, divergence, source, iface_rhs, pressure, *opt, S, st);
+  if (opt->precision == HFSI_PREC_MIXED)
+    return solve_cg<float, float>(h, divergence, source, iface_rhs, pressure, *opt, S, st, true);
   set_error("hfsi_fluid_solve_cg: unsupported precision");
   return HFSI_ERR_ARG;
 }
@@ -2159,7 +2758,7 @@ int hfsi_fluid_wall_clamp(hfsi_fluid* h, float* velocity, int32_t outlet_zmin, v
 }
 
 int hfsi_fluid_kernel_timer(hfsi_fluid* h, int32_t kernel_id) {
-  if (!h || kernel_id < 0 || kernel_id > HFSI_TIMER_L0_PROLONG_SWEEP) {
+  if (!h || kernel_id < 0 || kernel_id > HFSI_TIMER_CG_UPDATE) {
     set_error("hfsi_fluid_kernel_timer: invalid arguments");
     return HFSI_ERR_ARG;
   }

@@ -50,7 +50,7 @@ __all__ = [
 
 CG_PRECONDITIONER_CHOICES = ("auto", "jacobi", "fv_multigrid", "fv_multigrid_light")
 PRESSURE_SOLVERS = ("jacobi", "compact_jacobi", "fv_jacobi", "fv_multigrid", "fv_cg")
-SOLVER_PRECISIONS = {"ref64": _lib.HFSI_PREC_REF64, "fp32": _lib.HFSI_PREC_FP32}
+SOLVER_PRECISIONS = {"ref64": _lib.HFSI_PREC_REF64, "fp32": _lib.HFSI_PREC_FP32, "mixed": _lib.HFSI_PREC_MIXED}
 
 
 class VelocityDirichletBoundaryReport:
@@ -157,6 +157,8 @@ class CartesianFluidSolver:
         wz = np.ascontiguousarray(g.cell_widths_z_m, dtype=np.float64)
         origin = np.ascontiguousarray(self.bounds_min, dtype=np.float64)
         mask = SOLVER_PRECISIONS[solver_precision] | _lib.HFSI_PREC_REF64
+        if solver_precision == "mixed":
+            mask |= _lib.HFSI_PREC_FP32  # its float32 CG vectors serve the FP32 solve as well
         _lib.check(self._lib.hfsi_fluid_create(ctypes.byref(handle), self.nx, self.ny, self.nz,
                                                wx.ctypes.data, wy.ctypes.data, wz.ctypes.data, origin.ctypes.data,
                                                mask, _lib.stream_handle()), "hfsi_fluid_create")
@@ -417,7 +419,8 @@ class CartesianFluidSolver:
         opt.remove_mean = 1 if remove_nullspace_mean else 0
         opt.outlet_zmin = 1 if pressure_outlet_zmin else 0
         opt.interface_active = 1 if pressure_interface_matrix_active else 0
-        opt.check_interval = 16
+        # the mixed solve replaces its residual by the float64 true residual every 10 iterations
+        opt.check_interval = 10 if prec == "mixed" else 16
         opt.restart_interval = 256 if pressure_interface_matrix_active else 512
         opt.true_residual = 1 if prec != "ref64" else 0
         opt.true_residual_at_exit = 1 if true_residual_at_exit else 0

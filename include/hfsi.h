@@ -237,6 +237,22 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* particles, int32_t coun
                   float dt_s, float mu_pa, float lambda_pa, float velocity_damping, int32_t primary_region_id,
                   int32_t secondary_region_id, double* report34, void* stream);
 
+/* ---- mixed-precision V-cycle (HFSI_PREC_MIXED; no reference counterpart) ---
+ * The MIXED solve keeps the reference's flexible PCG driver with float32
+ * vectors and float64 solution, dots and true residual, and replaces its
+ * V-cycle by a red-black Gauss-Seidel V-cycle on a semi-coarsened hierarchy
+ * (cells merge along an axis while the merged width is <= 3x the level's
+ * smallest width), trilinear graded prolongation, and fp16 storage on the
+ * levels with at least half_min_cells cells (default 2^20).  Grids with
+ * velocity-Dirichlet rows or an interface diagonal fall back to the FP32
+ * V-cycle (hfsi_cg_stats.used_multigrid = 1; 2 when the mixed V-cycle ran).
+ * hfsi_fluid_vcycle with HFSI_PREC_MIXED applies it to r (scale from r). */
+/* Number of mixed levels (0 without HFSI_PREC_MIXED); *half_levels gets the
+ * number stored in fp16. */
+int hfsi_fluid_mixed_levels(const hfsi_fluid* h, int32_t* half_levels);
+int hfsi_fluid_mixed_level_shape(const hfsi_fluid* h, int32_t level, int32_t* shape3);
+int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells);
+
 /* Live kernel timing (no reference counterpart; measurement support):
  * while enabled, HIP events are recorded on the launch stream around every
  * launch of the selected level-0 kernel and their elapsed times summed.
@@ -246,6 +262,8 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* particles, int32_t coun
 #define HFSI_TIMER_SPMV_DOT 2         /* q = A d with <d, q>_V                                                   */
 #define HFSI_TIMER_L0_RESTRICT 3      /* level-0 residual + restriction                                          */
 #define HFSI_TIMER_L0_PROLONG_SWEEP 4 /* level-0 prolongation fused into the first post-smoothing sweep          */
+#define HFSI_TIMER_MIX_RB 5           /* mixed V-cycle: level-0 red-black colour pass (not the first from zero)  */
+#define HFSI_TIMER_CG_UPDATE 6        /* mixed solve: x += alpha d, r -= alpha q with <r,r>_V                    */
 int hfsi_fluid_kernel_timer(hfsi_fluid* h, int32_t kernel_id);
 /* Synchronises on the recorded events; totals since the last enable. */
 int hfsi_fluid_kernel_timer_read(hfsi_fluid* h, int64_t* launches, double* total_ms);

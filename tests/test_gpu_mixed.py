@@ -1,0 +1,201 @@
+"""GPU parity of the mixed-precision solve (HFSI_PREC_MIXED) through the C ABI.
+
+The mixed V-cycle has no reference counterpart; the device V-cycle is checked
+against its float64 restatement (oracle/mixed.py), and the mixed PCG against
+the reference algorithm's solution (oracle.fluid.pcg, float64, tight
+tolerance).  Tolerances:
+  hierarchy shapes                                 : exact
+  V-cycle, float32 storage (float32 arithmetic)    : 1e-4 of max |z|
+  V-cycle, fp16 storage on every grid level        : 5e-3 of max |z|
+  PCG to a float64 true residual of 1e-8           : pressure within 1e-5 of max |p|
+"""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from easyfsi_amd.simulation_core.grid import explicit_graded_axis
+from oracle import fluid as F
+from oracle import mixed as X
+
+pytestmark = pytest.mark.gpu
+
+PREC_REF64, PREC_FP32, PREC_MIXED = 1, 2, 4
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _solver(widths, precision="mixed", rho=1000.0, dt=1e-3):
+    from easyfsi_amd.simulation_core.fluid import CartesianFluidSolver
+    from easyfsi_amd.simulation_core.grid import CartesianGrid, FluidDomainSpec
+
+    g = CartesianGrid(bounds_min_m=(0.0, 0.0, 0.0), cell_widths_x_m=tuple(widths[0]),
+                      cell_widths_y_m=tuple(widths[1]), cell_widths_z_m=tuple(widths[2]))
+    spec = FluidDomainSpec(bounds_min_m=g.bounds_min_m, bounds_max_m=g.bounds_max_m, grid_nodes=None,
+                           density_kgm3=rho, viscosity_pa_s=1e-3, dt_s=dt, cartesian_grid=g)
+    return CartesianFluidSolver(spec, solver_precision=precision)
+
+
+def _graded_axis(core, side, growth):
+    w = np.asarray(explicit_graded_axis(core, 1.0, side, growth))
+    return w / w.sum()
+
+
+def _sphere_case(core, side, growth, radius):
+    wa = _graded_axis(core, side, growth)
+    w = [wa, wa[1:-1] / wa[1:-1].sum(), np.concatenate([wa[:1], wa, wa[-1:]]) / (wa.sum() + wa[0] + wa[-1])]
+    ctr = [np.cumsum(x) - 0.5 * x for x in w]
+    X_, Y_, Z_ = np.meshgrid(*ctr, indexing="ij")
+    obst = (((X_ - .5) ** 2 + (Y_ - .5) ** 2 + (Z_ - .5) ** 2) < radius ** 2).astype(np.int32)
+    return w, obst
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ok(s, rc):
+    assert rc == 0, s._lib.hfsi_last_error().decode()
+
+
+def _shapes(s):
+    n = s._lib.hfsi_fluid_mixed_levels(s._handle, None)
+    out = []
+    for l in range(n):
+        sh = np.zeros(3, np.int32)
+        _ok(s, s._lib.hfsi_fluid_mixed_level_shape(s._handle, l, sh.ctypes.data))
+        out.append(tuple(int(v) for v in sh))
+    return out
+
+
+def test_mixed_hierarchy_matches_oracle():
+    _say("mixed hierarchy")
+    w, obst = _sphere_case(16, 8, 1.2, 0.12)
+    s = _solver(w)
+    assert _shapes(s) == [lv.shape for lv in X.build_hierarchy(w, obst)]
+    u = [np.full(n, 1.0 / n) for n in (17, 12, 9)]
+    s = _solver(u)
+    assert _shapes(s) == [lv.shape for lv in X.build_hierarchy(u, np.zeros((17, 12, 9), np.int32))]
+
+
+def _vcycle(s, r, outlet):
+    rt = torch.from_numpy(np.ascontiguousarray(r)).cuda()
+    z = torch.zeros_like(rt)
+    _ok(s, s._lib.hfsi_fluid_vcycle(s._handle, rt.data_ptr(), z.data_ptr(), PREC_MIXED, 2, 24, 2, int(outlet),
+                                    _stream()))
+    return z.cpu().numpy()
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+@pytest.mark.parametrize("half", [False, True])
+def test_mixed_vcycle_matches_oracle(outlet, half):
+    _say(f"mixed vcycle outlet={outlet} half={half}")
+    w, obst = _sphere_case(16, 8, 1.2, 0.12)
+    rng = np.random.default_rng(31)
+    r = np.where(obst == 0, rng.normal(size=obst.shape), 0.0)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    if half:
+        _ok(s, s._lib.hfsi_fluid_mixed_set_half_min_cells(s._handle, 0))
+        nh = ctypes.c_int32()
+        s._lib.hfsi_fluid_mixed_levels(s._handle, ctypes.byref(nh))
+        assert nh.value >= 2
+    got = _vcycle(s, r, outlet)
+    want = X.vcycle(X.build_hierarchy(w, obst), r, outlet=outlet)
+    tol = 5e-3 if half else 1e-4
+    np.testing.assert_allclose(got, want, rtol=0, atol=tol * np.abs(want).max())
+
+
+def _cg(s, div, precision, tol, max_it, outlet, remove_mean, check=10):
+    from easyfsi_amd import _lib
+
+    opt = _lib.CgOptions()
+    opt.precision = precision
+    opt.preconditioner = 2
+    opt.max_iterations = max_it
+    opt.remove_mean = int(remove_mean)
+    opt.outlet_zmin = int(outlet)
+    opt.interface_active = 0
+    opt.check_interval = check
+    opt.restart_interval = 512
+    opt.true_residual = 1
+    opt.true_residual_at_exit = 1
+    opt.tolerance = tol
+    opt.rhs_scale = s.rho / s.dt
+    st = _lib.CgStats()
+    s.divergence.from_numpy(div)
+    _ok(s, s._lib.hfsi_fluid_solve_cg(s._handle, s.divergence.ptr, s.volume_source_s.ptr, None, s.pressure.ptr,
+                                      ctypes.byref(opt), ctypes.byref(st), _stream()))
+    return st
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+def test_mixed_pcg_matches_reference_solution(outlet):
+    _say(f"mixed pcg outlet={outlet}")
+    w, obst = _sphere_case(16, 8, 1.2, 0.12)
+    shape = obst.shape
+    rng = np.random.default_rng(41)
+    div = np.where(obst == 0, rng.normal(size=shape), 0.0).astype(np.float32)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    st = _cg(s, div, PREC_MIXED, 1e-8, 200, outlet, not outlet)
+    assert st.converged == 1 and st.used_multigrid == 2, st.as_dict()
+    assert st.true_relative_residual <= 1e-8, st.as_dict()
+    h = F.build_hierarchy(w, obst)
+    rhs0 = F.level0_rhs(div, np.zeros(shape, np.float32), None, h.levels[0], s.rho / s.dt)
+    ref = F.pcg(rhs0, np.zeros(shape), h, outlet=outlet, tolerance=1e-11, max_iters=400,
+                preconditioner="fv_multigrid", remove_mean=not outlet)
+    np.testing.assert_allclose(s.pressure.to_numpy(), ref.pressure, rtol=0, atol=1e-5 * np.abs(ref.pressure).max())
+    # the reference V-cycle in float32 needs more iterations for the same true residual
+    s.pressure.fill(0.0)
+    sf = _solver(w, precision="fp32")
+    sf.obstacle.from_numpy(obst)
+    sf._set_masks()
+    stf = _cg(sf, div, PREC_FP32, 1e-8, 400, outlet, not outlet, check=1)
+    assert stf.converged == 1
+    assert st.iterations < stf.iterations, (st.iterations, stf.iterations)
+
+
+def test_mixed_falls_back_with_dirichlet_rows():
+    _say("mixed fallback")
+    w, obst = _sphere_case(8, 4, 1.2, 0.15)
+    shape = obst.shape
+    rng = np.random.default_rng(43)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    vda = (rng.random(shape) < 0.05).astype(np.int32)
+    s.velocity_dirichlet_boundary_active.from_numpy(vda)
+    s.velocity_dirichlet_boundary_projection_weight.from_numpy(np.full(shape, 0.5, np.float32))
+    s._set_masks()
+    div = np.where(obst == 0, rng.normal(size=shape), 0.0).astype(np.float32)
+    st = _cg(s, div, PREC_MIXED, 1e-8, 300, True, False)
+    assert st.converged == 1 and st.used_multigrid == 1, st.as_dict()
+
+
+def test_mixed_project_matches_oracle_projection():
+    _say("mixed project")
+    w, obst = _sphere_case(12, 6, 1.15, 0.14)
+    shape = obst.shape
+    rng = np.random.default_rng(47)
+    u = rng.normal(size=shape + (3,)).astype(np.float32) * 0.1
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s.velocity.from_numpy(u)
+    rep = s.project(iterations=300, pressure_solver="fv_cg", cg_tolerance=1e-9, cg_preconditioner="fv_multigrid",
+                    reset_pressure=True)
+    u_ref, p_ref, div_ref, info = F.project_plain(u, np.zeros(shape), obst, w, rho=1000.0, dt=1e-3, solver="fv_cg",
+                                                  iterations=300, tolerance=1e-10, preconditioner="fv_multigrid",
+                                                  reset_pressure=True)
+    assert rep["cg_converged_all"] and info["cg"].converged
+    assert s.last_cg_stats["used_multigrid"] == 2
+    np.testing.assert_allclose(s.pressure.to_numpy(), p_ref, atol=1e-6 * np.abs(p_ref).max())
+    np.testing.assert_allclose(s.velocity.to_numpy(), u_ref, rtol=1e-5, atol=1e-6 * np.abs(u_ref).max())
+    assert rep["l2"] < 1e-4 * rep["pre_projection_l2"]

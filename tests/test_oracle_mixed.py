@@ -1,0 +1,134 @@
+"""Pins the float64 restatement of the mixed V-cycle (oracle/mixed.py).
+
+The mixed V-cycle has no counterpart in the reference, so the checks are
+closed cases and mathematical properties: on uniform grids the hierarchy is
+the reference's 2:1 hierarchy (fluid.py:692-729), the child tables partition
+every axis, RB-GS converges to scipy's direct solution, and the V-cycle used
+as a stationary iteration contracts the error in the energy norm.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from easyfsi_amd.simulation_core.grid import explicit_graded_axis, multigrid_shapes
+from oracle import fluid as F
+from oracle import mixed as X
+
+
+def _graded(core, side, growth):
+    w = np.asarray(explicit_graded_axis(core, 1.0, side, growth))
+    return w / w.sum()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (17, 12, 9), (40, 36, 33)])
+def test_uniform_hierarchy_is_the_reference_hierarchy(shape):
+    w = [np.full(n, 1.0 / n) for n in shape]
+    levels = X.build_hierarchy(w, np.zeros(shape, np.int32))
+    assert tuple(l.shape for l in levels) == multigrid_shapes(shape)
+
+
+def test_graded_hierarchy_partitions_axes_and_bounds_aspect():
+    w = _graded(16, 8, 1.2)  # side / core ratio 1.2^8 = 4.3
+    shape = (len(w),) * 3
+    levels = X.build_hierarchy([w, w, w], np.zeros(shape, np.int32))
+    assert len(levels) > 2
+    for f, c in zip(levels[:-1], levels[1:]):
+        for a in range(3):
+            # every fine cell has exactly one parent; children are contiguous and sum to the coarse width
+            assert np.all(np.diff(f.par[a]) >= 0) and f.par[a][0] == 0 and f.par[a][-1] == len(c.w[a]) - 1
+            np.testing.assert_array_equal(np.bincount(f.par[a]), c.cnt[a])
+            np.testing.assert_allclose(np.bincount(f.par[a], weights=f.w[a]), c.w[a], rtol=1e-14)
+            assert c.w[a].sum() == pytest.approx(f.w[a].sum(), rel=1e-14)
+    # semi-coarsening: the first step does not merge the widest side cells
+    assert levels[1].shape[0] > (shape[0] + 1) // 2
+    # the coarse levels' width ratio shrinks towards 1
+    ratios = [np.concatenate(l.w).max() / np.concatenate(l.w).min() for l in levels]
+    assert ratios[-1] < ratios[0] / 2
+
+
+def test_coarse_solid_only_when_no_child_is_free():
+    shape = (8, 8, 8)
+    w = [np.full(8, 1 / 8)] * 3
+    obst = np.zeros(shape, np.int32)
+    obst[2, 3, 5] = 1          # one solid child: the coarse cell stays free
+    obst[4:6, 0:2, 6:8] = 1    # all eight children solid
+    obst[6:8, 6:8, 0:2] = 2    # all not free (any nonzero value)
+    lv = X.build_hierarchy(w, obst)
+    assert lv[1].notfree[2, 0, 3] and lv[1].notfree[3, 3, 0] and lv[1].notfree.sum() == 2
+
+
+def _dense(lvl, outlet):
+    n = int(np.prod(lvl.shape))
+    cols = []
+    for c in range(n):
+        e = np.zeros(n)
+        e[c] = 1.0
+        cols.append(X.apply(lvl, e.reshape(lvl.shape), outlet).ravel())
+    return np.array(cols).T
+
+
+def test_rbgs_converges_to_direct_solution():
+    rng = np.random.default_rng(0)
+    shape = (6, 5, 7)
+    w = [rng.uniform(0.5, 1.5, n) / n for n in shape]
+    obst = (rng.random(shape) < 0.1).astype(np.int32)
+    lvl = X.build_hierarchy(w, obst)[0]
+    A = _dense(lvl, True)
+    free = ~lvl.notfree.ravel()
+    b = np.where(free, rng.normal(size=free.size), 0.0)
+    x = np.zeros(free.size)
+    x[free] = spla.spsolve(sp.csr_matrix(A[np.ix_(free, free)]), b[free])
+    e = np.zeros(shape)
+    for s in range(3000):
+        e = X.rb_pass(lvl, e, b.reshape(shape), 0, True, from_zero=(s == 0))
+        e = X.rb_pass(lvl, e, b.reshape(shape), 1, True)
+    np.testing.assert_allclose(e.ravel(), x, atol=1e-9 * np.abs(x).max())
+
+
+def test_level0_operator_is_the_reference_operator():
+    rng = np.random.default_rng(1)
+    shape = (9, 8, 7)
+    w = [rng.uniform(0.5, 1.5, n) / n for n in shape]
+    obst = (rng.random(shape) < 0.1).astype(np.int32)
+    p = rng.normal(size=shape)
+    lvl = X.build_hierarchy(w, obst)[0]
+    for outlet in (False, True):
+        np.testing.assert_allclose(X.apply(lvl, p, outlet), F.fv_apply(p, F.make_level(w, obst), outlet),
+                                   rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("outlet", [True, False])
+def test_vcycle_contracts_energy_error(outlet):
+    rng = np.random.default_rng(2)
+    wa = _graded(12, 6, 1.2)
+    shape = (len(wa), len(wa) - 2, len(wa) + 2)
+    w = [wa, wa[1:-1] / wa[1:-1].sum(), np.concatenate([wa[:1], wa, wa[-1:]]) / (wa.sum() + wa[0] + wa[-1])]
+    ctr = [np.cumsum(x) - 0.5 * x for x in w]
+    X_, Y_, Z_ = np.meshgrid(*ctr, indexing="ij")
+    obst = (((X_ - .5) ** 2 + (Y_ - .5) ** 2 + (Z_ - .5) ** 2) < 0.12 ** 2).astype(np.int32)
+    levels = X.build_hierarchy(w, obst)
+    lvl = levels[0]
+    V = w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
+    free = ~lvl.notfree
+    x_true = np.where(free, rng.normal(size=shape), 0.0)
+    if not outlet:
+        x_true = np.where(free, x_true - (V * x_true)[free].sum() / V[free].sum(), 0.0)
+    b = X.apply(lvl, x_true, outlet)
+
+    def energy(e):
+        return float(np.sqrt((V * e * X.apply(lvl, e, outlet)).sum()))
+
+    x = np.zeros(shape)
+    rates = []
+    prev = energy(x_true - x)
+    for _ in range(8):
+        x = x + X.vcycle(levels, b - X.apply(lvl, x, outlet), outlet=outlet)
+        if not outlet:
+            x = np.where(free, x - (V * x)[free].sum() / V[free].sum(), 0.0)
+        e = energy(x_true - x)
+        rates.append(e / prev)
+        prev = e
+    # asymptotic contraction per cycle (0.23 here; 0.73 with the "any child solid" coarse rule)
+    assert max(rates[4:]) < 0.3, rates
