@@ -104,7 +104,7 @@ def make_rhs(solver, count: int, seed0: int):
 
 
 VARIANTS = {
-    "mixed": dict(precision="mixed", true_residual=1, check=10, dtype="f32", timer="mix_rb",
+    "mixed": dict(precision="mixed", true_residual=1, check=10, monitor_lag=2, dtype="f32", timer="mix_rb",
                   roofline_kernel="mixed V-cycle level-0 red-black colour pass (k_mix_rb<half,float>)",
                   # colour pass: e of the other colour read at full-line granularity (2 B/cell, fp16),
                   # r and flags of the pass colour (2 + 0.5 B/cell), e of the pass colour written (1 B/cell)
@@ -139,6 +139,7 @@ def run_variant(name, solver, divs, steps, warmup):
     opt.restart_interval = 512
     opt.true_residual = cfg["true_residual"]
     opt.true_residual_at_exit = 1
+    opt.monitor_lag = cfg.get("monitor_lag", 0)
     opt.tolerance = 1e-8
     opt.rhs_scale = solver.rho / solver.dt
     stream = torch.cuda.current_stream().cuda_stream

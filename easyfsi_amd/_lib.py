@@ -68,6 +68,7 @@ class CgOptions(ctypes.Structure):
         ("restart_interval", ctypes.c_int32),
         ("true_residual", ctypes.c_int32),
         ("true_residual_at_exit", ctypes.c_int32),
+        ("monitor_lag", ctypes.c_int32),
         ("tolerance", ctypes.c_double),
         ("rhs_scale", ctypes.c_double),
     ]

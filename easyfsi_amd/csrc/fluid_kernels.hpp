@@ -234,10 +234,9 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(OpGeo<C> fine, P p, R
   }
 }
 
-// q = A d with the partial dot <d, q>_V; last workgroup finalises (fin).
-template <typename C, typename TD, typename TQ, typename Fin>
-__global__ void __launch_bounds__(kBlock) k_spmv_dot(OpGeo<C> g, const TD* d, TQ* q, double* partials,
-                                                     unsigned int* counter, double* totals, Fin fin) {
+// q = A d with the per-workgroup partials of <d, q>_V (combined by k_finish_sums).
+template <typename C, typename TD, typename TQ>
+__global__ void __launch_bounds__(kBlock) k_spmv_dot(OpGeo<C> g, const TD* d, TQ* q, double* partials) {
   __shared__ double lds[kBlock / 64];
   double acc[1] = {0.0};
   Ld<TD, C> P{d};
@@ -249,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(OpGeo<C> g, const TD* d, TQ
     q[c] = vs;
     if (!(g.flags[c] & kNotFree)) acc[0] += (double)d[c] * (double)vs * volume64(g, i, j, k);
   }
-  if (grid_sum<1>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+  block_partials<1>(acc, partials, lds);
 }
 
 // Plain apply (no dot), any storage.
@@ -281,10 +280,10 @@ __global__ void __launch_bounds__(kBlock) k_diagonal(OpGeo<C> g, double* out) {
   }
 }
 
-// Generic weighted sums: up to NS sums of V * (products chosen by Op).
-template <int NS, typename C, typename Op, typename Fin>
-__global__ void __launch_bounds__(kBlock) k_sums(OpGeo<C> g, Op op, double* partials, unsigned int* counter,
-                                                 double* totals, Fin fin) {
+// Generic weighted sums: up to NS sums of V * (products chosen by Op), as
+// per-workgroup partials (combined by k_finish_sums).
+template <int NS, typename C, typename Op>
+__global__ void __launch_bounds__(kBlock) k_sums(OpGeo<C> g, Op op, double* partials) {
   __shared__ double lds[NS * (kBlock / 64)];
   double acc[NS];
 #pragma unroll
@@ -294,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_sums(OpGeo<C> g, Op op, double* part
     decompose(c, g.ny, g.nz, i, j, k);
     if (!(g.flags[c] & kNotFree)) op(c, volume64(g, i, j, k), acc);
   }
-  if (grid_sum<NS>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+  block_partials<NS>(acc, partials, lds);
 }
 
 // Generic pointwise pass over all cells: op(c, free).

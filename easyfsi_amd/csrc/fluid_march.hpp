@@ -197,8 +197,8 @@ __device__ inline void march_load(const MarchArgs<C, TP, TR, TO, TQ>& a, uint32_
 // One marching kernel for: weighted-Jacobi sweep (normal / from zero /
 // prolongation fused), operator apply with <d, A d>_V (kModeApply + DOTS),
 // and the final level-0 post-sweep with the five CG sums (DOTS).
-template <typename C, typename TP, typename TR, typename TO, typename TQ, int MODE, bool DOTS, typename Fin>
-__global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a, Fin fin) {
+template <typename C, typename TP, typename TR, typename TO, typename TQ, int MODE, bool DOTS>
+__global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a) {
   constexpr int NS = (MODE == kModeApply) ? 1 : 5;
   __shared__ double lds[DOTS ? NS * 4 : 1];
   double acc[NS];
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a, F
     }
   }
   if (DOTS) {
-    if (grid_sum<NS>(acc, a.partials, a.counter, a.totals, lds) && threadIdx.x == 0) fin(a.totals);
+    block_partials<NS>(acc, a.partials, lds);
   }
 }
 

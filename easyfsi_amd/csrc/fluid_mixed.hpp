@@ -101,14 +101,40 @@ __global__ void __launch_bounds__(kBlock) k_mix_rb(OpGeo<float> g, TE* e, const 
   }
 }
 
-// Residual of level f restricted to level f + 1: for each free coarse cell the
-// volume-weighted mean over its free children of rs * rhs - L e.
-template <typename TE, typename TR, typename TC>
-__global__ void __launch_bounds__(kBlock) k_mix_restrict(OpGeo<float> g, const TE* e, const TR* rhs,
-                                                          const double* rs_dev, MixXfer x, uint32_t cnx, uint32_t cny,
-                                                          uint32_t cnz, const uint8_t* cflags, TC* crhs) {
-  const uint32_t cn = cnx * cny * cnz;
+// Residual rs * rhs - L e of every cell of a level (0 on cells that are not
+// free), first half of the two-pass restriction.
+template <typename TE, typename TR, typename TRes>
+__global__ void __launch_bounds__(kBlock) k_mix_residual(OpGeo<float> g, const TE* e, const TR* rhs,
+                                                          const double* rs_dev, TRes* res) {
   const float rs = rhs_scale(rs_dev);
+  HFSI_CELL_LOOP(g.n) {
+    const uint8_t f = g.flags[c];
+    float out = 0.0f;
+    if (!(f & kNotFree)) {
+      const uint32_t k = c % g.nz;
+      const uint32_t t = c / g.nz;
+      const uint32_t j = t % g.ny, i = t / g.ny;
+      float nb = 0.0f, den = 0.0f;
+      if (f & kXm) { const float q = g.cl[0][i]; nb += q * ldf<TE>(e, c - g.sx); den += q; }
+      if (f & kXp) { const float q = g.ch[0][i]; nb += q * ldf<TE>(e, c + g.sx); den += q; }
+      if (f & kYm) { const float q = g.cl[1][j]; nb += q * ldf<TE>(e, c - g.sy); den += q; }
+      if (f & kYp) { const float q = g.ch[1][j]; nb += q * ldf<TE>(e, c + g.sy); den += q; }
+      if (f & kZm) { const float q = g.cl[2][k]; nb += q * ldf<TE>(e, c - 1); den += q; }
+      if (f & kZp) { const float q = g.ch[2][k]; nb += q * ldf<TE>(e, c + 1); den += q; }
+      if (g.outlet && k == 0) den += g.outlet_add;
+      out = rs * ldf<TR>(rhs, c) - (den * ldf<TE>(e, c) - nb);
+    }
+    stf<TRes>(res, c, out);
+  }
+}
+
+// Second half: coarse rhs = volume-weighted mean of the stored residuals of
+// the free children (cells that are not free hold 0 and weigh nothing).
+template <typename TRes, typename TC>
+__global__ void __launch_bounds__(kBlock) k_mix_restrict_res(OpGeo<float> g, const TRes* res, MixXfer x, uint32_t cnx,
+                                                              uint32_t cny, uint32_t cnz, const uint8_t* cflags,
+                                                              TC* crhs) {
+  const uint32_t cn = cnx * cny * cnz;
   HFSI_CELL_LOOP(cn) {
     const uint32_t K = c % cnz;
     const uint32_t t = c / cnz;
@@ -124,22 +150,12 @@ __global__ void __launch_bounds__(kBlock) k_mix_restrict(OpGeo<float> g, const T
         for (int32_t oj = 0; oj < nj; ++oj) {
           const uint32_t j = (uint32_t)(j0 + oj);
           const float wij = wi * g.wf[1][j];
+          const uint32_t row = i * g.sx + j * g.sy;
           for (int32_t ok = 0; ok < nk; ++ok) {
             const uint32_t k = (uint32_t)(k0 + ok);
-            const uint32_t cc = i * g.sx + j * g.sy + k;
-            const uint8_t f = g.flags[cc];
-            if (f & kNotFree) continue;
-            float nb = 0.0f, den = 0.0f;
-            if (f & kXm) { const float q = g.cl[0][i]; nb += q * ldf<TE>(e, cc - g.sx); den += q; }
-            if (f & kXp) { const float q = g.ch[0][i]; nb += q * ldf<TE>(e, cc + g.sx); den += q; }
-            if (f & kYm) { const float q = g.cl[1][j]; nb += q * ldf<TE>(e, cc - g.sy); den += q; }
-            if (f & kYp) { const float q = g.ch[1][j]; nb += q * ldf<TE>(e, cc + g.sy); den += q; }
-            if (f & kZm) { const float q = g.cl[2][k]; nb += q * ldf<TE>(e, cc - 1); den += q; }
-            if (f & kZp) { const float q = g.ch[2][k]; nb += q * ldf<TE>(e, cc + 1); den += q; }
-            if (g.outlet && k == 0) den += g.outlet_add;
-            const float res = rs * ldf<TR>(rhs, cc) - (den * ldf<TE>(e, cc) - nb);
+            if (g.flags[row + k] & kNotFree) continue;
             const float V = wij * g.wf[2][k];
-            s += V * res;
+            s += V * ldf<TRes>(res, row + k);
             vs += V;
           }
         }

@@ -97,15 +97,15 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// Sum NS values over the workgroup; result valid in thread 0.  `lds` must hold
-// NS * (blockDim.x / 64) doubles.
+// Workgroup reduction of NS values with `maxmask` marking max-reduced slots;
+// result valid in thread 0.  `lds` must hold NS * (blockDim.x / 64) doubles.
 template <int NS>
-__device__ inline void block_sum(double (&v)[NS], double* lds) {
+__device__ inline void block_reduce(double (&v)[NS], double* lds, uint64_t maxmask) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) v[s] = wave_sum(v[s]);
+  for (int s = 0; s < NS; ++s) v[s] = ((maxmask >> s) & 1) ? wave_max(v[s]) : wave_sum(v[s]);
   if (lane == 0) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) lds[s * nw + wave] = v[s];
@@ -114,49 +114,51 @@ __device__ inline void block_sum(double (&v)[NS], double* lds) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      double acc = 0.0;
-      for (int w = 0; w < nw; ++w) acc += lds[s * nw + w];
+      double acc = lds[s * nw];
+      for (int w = 1; w < nw; ++w) acc = ((maxmask >> s) & 1) ? fmax(acc, lds[s * nw + w]) : acc + lds[s * nw + w];
       v[s] = acc;
     }
   }
   __syncthreads();
 }
 
-// Deterministic grid reduction: every workgroup stores its NS partial sums,
-// the last workgroup to finish adds them in workgroup order and leaves the
-// totals in `totals` (device memory); returns true in that workgroup's
-// threads.  `counter` must be zero on entry and is reset to zero.
+// Grid reductions are two launches on one stream: every workgroup of the
+// producing kernel stores its NS partial results (block_partials), then
+// k_finish_sums (one workgroup) combines them in workgroup order and runs the
+// finaliser.  The kernel boundary publishes the partials; a device-scope fence
+// per workgroup would instead write back the XCD's L2 (the producing kernels
+// stream their outputs through it) once per workgroup.
 template <int NS>
-__device__ inline bool grid_sum(double (&v)[NS], double* partials, unsigned int* counter, double* totals,
-                                double* lds) {
-  block_sum<NS>(v, lds);
-  __shared__ bool is_last;
+__device__ inline void block_partials(double (&v)[NS], double* partials, double* lds, uint64_t maxmask = 0) {
+  block_reduce<NS>(v, lds, maxmask);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) partials[(size_t)blockIdx.x * NS + s] = v[s];
-    __threadfence();
-    unsigned int done = atomicAdd(counter, 1u);
-    is_last = (done == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();  // acquire side: partial sums of all workgroups are visible
+}
+
+constexpr int kFinishThreads = 256;
+
+template <int NS, typename Fin>
+__global__ void __launch_bounds__(kFinishThreads) k_finish_sums(const double* partials, uint32_t nblocks,
+                                                                 double* totals, Fin fin, uint64_t maxmask) {
+  __shared__ double lds[NS * (kFinishThreads / 64)];
   double acc[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (int s = 0; s < NS; ++s) acc[s] = ((maxmask >> s) & 1) ? -INFINITY : 0.0;
+  for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
-      acc[s] += __hip_atomic_load(&partials[(size_t)b * NS + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int s = 0; s < NS; ++s) {
+      const double p = partials[(size_t)b * NS + s];
+      acc[s] = ((maxmask >> s) & 1) ? fmax(acc[s], p) : acc[s] + p;
+    }
   }
-  block_sum<NS>(acc, lds);
+  block_reduce<NS>(acc, lds, maxmask);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) totals[s] = acc[s];
-    *counter = 0u;
+    fin(totals);
   }
-  __syncthreads();
-  return true;
 }
 
 }  // namespace hfsi

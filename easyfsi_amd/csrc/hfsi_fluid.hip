@@ -302,10 +302,16 @@ struct OpFill {
 // ---------------------------------------------------------------------------
 
 // vin / vout of the mixed V-cycle from t = {<r,r>_V}: RMS = sqrt(<r,r>_V / V_free).
+constexpr int kMonRing = 8;  // slots of the convergence-monitor ring (lag < kMonRing)
+
 struct FinVScale {
   Scalars* s;
   double kappa;
-  __device__ void operator()(const double* t) const { set(t[0]); }
+  double* monitor = nullptr;  // pinned host slot receiving <r,r>_V (convergence monitor), or null
+  __device__ void operator()(const double* t) const {
+    set(t[0]);
+    if (monitor) *monitor = t[0];
+  }
   __device__ void set(double rr) const {
     const double V = s->free_vol > 0.0 ? s->free_vol : 1.0;
     const double rms = sqrt(fmax(rr, 0.0) / V);
@@ -326,10 +332,8 @@ __global__ void k_mix_scale_from_rr(Scalars* s, double kappa) {
 
 // x += alpha d, r -= alpha q (obstacle rows zeroed, fluid.py:4576-4587) and
 // <r,r>_V over free cells for the mixed V-cycle scale.
-template <typename Fin>
 __global__ void __launch_bounds__(kBlock) k_cg_update_rr(OpGeo<double> g, double* x, const float* d, float* r,
-                                                          const float* q, const Scalars* s, double* partials,
-                                                          unsigned int* counter, double* totals, Fin fin) {
+                                                          const float* q, const Scalars* s, double* partials) {
   __shared__ double lds[kBlock / 64];
   double acc[1] = {0.0};
   const double a = s->alpha;
@@ -349,16 +353,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_rr(OpGeo<double> g, double
       acc[0] += (double)rn * (double)rn * volume64(g, i, j, k);
     }
   }
-  if (grid_sum<1>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+  block_partials<1>(acc, partials, lds);
 }
 
 // z = e * vout on free cells (0 elsewhere) and the PCG sums of FinDots5
 // (NS = 5: <r,z>, <z,q>, sum Vz, sum Vr, sum Vq) or FinDots3 (NS = 3: <r,z>,
 // sum Vz, sum Vr).
-template <int NS, typename TE, typename Fin>
+template <int NS, typename TE>
 __global__ void __launch_bounds__(kBlock) k_mix_finish(OpGeo<double> g, const TE* e, const float* r, const float* q,
-                                                        float* z, const Scalars* s, double* partials,
-                                                        unsigned int* counter, double* totals, Fin fin) {
+                                                        float* z, const Scalars* s, double* partials) {
   __shared__ double lds[NS * (kBlock / 64)];
   double acc[NS];
 #pragma unroll
@@ -388,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) k_mix_finish(OpGeo<double> g, const TE
       acc[2] += rv * V;
     }
   }
-  if (grid_sum<NS>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+  block_partials<NS>(acc, partials, lds);
 }
 
 // Coarse masks of the semi-coarsened hierarchy, through the per-axis child
@@ -664,10 +667,9 @@ __global__ void k_wall_clamp(uint32_t nx, uint32_t ny, uint32_t nz, float* u, in
 
 // Float64 true residual r = b - A x on non-obstacle rows (axpby of
 // fluid.py:4518-4531), stored at TR, with {<r,r>, sum V r} in float64.
-template <typename TR, typename Fin>
+template <typename TR>
 __global__ void __launch_bounds__(kBlock) k_true_residual(OpGeo<double> g, const double* x, const double* b, TR* r,
-                                                           double* partials, unsigned int* counter, double* totals,
-                                                           Fin fin) {
+                                                           double* partials) {
   __shared__ double lds[2 * (kBlock / 64)];
   double acc[2] = {0.0, 0.0};
   Ld<double, double> P{x};
@@ -684,7 +686,7 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(OpGeo<double> g, const
       acc[1] += v * V;
     }
   }
-  if (grid_sum<2>(acc, partials, counter, totals, lds) && threadIdx.x == 0) fin(totals);
+  block_partials<2>(acc, partials, lds);
 }
 
 // z = r / diag (_apply_jacobi_preconditioner_kernel, fluid.py:4502-4516).
@@ -763,8 +765,11 @@ struct ReportArgs {
   double* totals;
 };
 
+constexpr int kReportSlots = 54;
+constexpr uint64_t kReportMaxMask = 0x9249249249249ull;  // bits 0, 3, 6, ..., 51
+
 __global__ void __launch_bounds__(kBlock) k_div_report(ReportArgs a) {
-  constexpr int NS = 54;
+  constexpr int NS = kReportSlots;
   __shared__ double lds[NS * (kBlock / 64)];
   double acc[NS];
 #pragma unroll
@@ -824,37 +829,8 @@ __global__ void __launch_bounds__(kBlock) k_div_report(ReportArgs a) {
       }
     }
   }
-  // workgroup reduction: max for slot*3, sums otherwise
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) acc[s] = (s % 3 == 0) ? wave_max(acc[s]) : wave_sum(acc[s]);
-  if (lane == 0)
-    for (int s = 0; s < NS; ++s) lds[s * nw + wave] = acc[s];
-  __syncthreads();
-  __shared__ bool is_last;
-  if (threadIdx.x == 0) {
-    for (int s = 0; s < NS; ++s) {
-      double v = 0.0;
-      for (int w = 0; w < nw; ++w) v = (s % 3 == 0) ? fmax(v, lds[s * nw + w]) : v + lds[s * nw + w];
-      a.partials[(size_t)blockIdx.x * NS + s] = v;
-    }
-    __threadfence();
-    is_last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  if (threadIdx.x < NS) {
-    int s = threadIdx.x;
-    double v = 0.0;
-    for (uint32_t b = 0; b < gridDim.x; ++b) {
-      double pv = __hip_atomic_load(&a.partials[(size_t)b * NS + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      v = (s % 3 == 0) ? fmax(v, pv) : v + pv;
-    }
-    a.totals[s] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) *a.counter = 0u;
+  // max for slot*3, sums otherwise
+  block_partials<NS>(acc, a.partials, lds, kReportMaxMask);
 }
 
 // ---------------------------------------------------------------------------
@@ -1049,6 +1025,7 @@ struct MixLevel {
   float outlet_add = 0.0f;  // 2 / w_z[0]^2 / kappa
   void* e = nullptr;
   void* rhs = nullptr;      // unused on level 0 (the CG residual is the input)
+  void* res = nullptr;      // residual scratch of the two-pass restriction (grid-launched levels)
   // transfer to level + 1 (fine side): parent, interpolation partner, parent weight
   int32_t* par[3] = {};
   int32_t* pnb[3] = {};
@@ -1098,6 +1075,9 @@ struct hfsi_fluid {
   hfsi::MixLevel ml[hfsi::kMixMaxLevels];
   void* mix_tail_args = nullptr;  // MixTailArgs (device)
   int mix_tail_outlet = -1;       // outlet switch the tail arguments were written for
+  // convergence monitor of the mixed solve: pinned ring of <r,r>_V values and their events
+  double* mon_ring = nullptr;
+  std::vector<hipEvent_t> mon_events;
   // live kernel timer: HIP events recorded on the launch stream around every
   // launch of one selected kernel (HFSI_TIMER_*), summed at flush/read time
   int timer_id = 0;
@@ -1177,10 +1157,18 @@ static int launch_pointwise(hfsi_fluid* h, int l, Op op, hipStream_t st) {
   return HFSI_OK;
 }
 
+// Combine the partials of a reducing launch of `nblocks` workgroups and run `fin`.
+template <int NS, typename Fin>
+static void finish_sums(hfsi_fluid* h, uint32_t nblocks, Fin fin, hipStream_t st, uint64_t maxmask = 0) {
+  hipLaunchKernelGGL((k_finish_sums<NS, Fin>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)h->partials,
+                     nblocks, h->totals, fin, maxmask);
+}
+
 template <int NS, typename C, typename Op, typename Fin>
 static int launch_sums(hfsi_fluid* h, const OpGeo<C>& g, Op op, Fin fin, hipStream_t st) {
-  hipLaunchKernelGGL((k_sums<NS, C, Op, Fin>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, op, h->partials,
-                     h->counter, h->totals, fin);
+  const uint32_t nb = blocks_for(g.n);
+  hipLaunchKernelGGL((k_sums<NS, C, Op>), dim3(nb), dim3(kBlock), 0, st, g, op, h->partials);
+  finish_sums<NS>(h, nb, fin, st);
   HFSI_LAUNCH_CHECK();
   return HFSI_OK;
 }
@@ -1264,7 +1252,8 @@ static void launch_march(hfsi_fluid* h, int l, int outlet, const TP* p, const TR
   a.partials = h->partials;
   a.counter = h->counter;
   a.totals = h->totals;
-  hipLaunchKernelGGL((k_march<C, TP, TR, TO, TQ, MODE, DOTS, Fin>), dim3(march_grid(a.m)), dim3(256), 0, st, a, fin);
+  hipLaunchKernelGGL((k_march<C, TP, TR, TO, TQ, MODE, DOTS>), dim3(march_grid(a.m)), dim3(256), 0, st, a);
+  if (DOTS) finish_sums<(MODE == kModeApply) ? 1 : 5>(h, march_grid(a.m), fin, st);
 }
 
 template <typename C, typename TP, typename TR, typename TRes>
@@ -1595,12 +1584,18 @@ static void mix_smooth(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, c
   }
 }
 
+// Residual of level l (stored at the level's own precision), then its
+// volume-weighted restriction into the rhs of level l + 1.
 template <typename TE, typename TR, typename TC>
 static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TE* e, const TR* rhs, const double* rs,
                          hipStream_t st) {
+  const MixLevel& f = h->ml[l];
   const MixLevel& c = h->ml[l + 1];
-  hipLaunchKernelGGL((k_mix_restrict<TE, TR, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet),
-                     e, rhs, rs, mix_xfer(h, l), c.nx, c.ny, c.nz, (const uint8_t*)c.flags, (TC*)c.rhs);
+  OpGeo<float> g = mix_geo(h, l, outlet);
+  hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, e, rhs, rs,
+                     (TE*)f.res);
+  hipLaunchKernelGGL((k_mix_restrict_res<TE, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, g, (const TE*)f.res,
+                     mix_xfer(h, l), c.nx, c.ny, c.nz, (const uint8_t*)c.flags, (TC*)c.rhs);
 }
 
 template <typename TF, typename TCo>
@@ -1681,12 +1676,14 @@ static int mix_precondition(hfsi_fluid* h, const float* r, const float* q, float
                             int outlet, Fin fin, hipStream_t st) {
   HFSI_TRY(run_mix_vcycle(h, r, pre, coarse, post, outlet, st));
   OpGeo<double> g64 = geo<double>(h, 0, outlet);
+  const uint32_t nb = blocks_for(g64.n);
   if (mix_half(h, 0))
-    hipLaunchKernelGGL((k_mix_finish<NS, __half, Fin>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64,
-                       (const __half*)h->ml[0].e, r, q, z, h->scal, h->partials, h->counter, h->totals, fin);
+    hipLaunchKernelGGL((k_mix_finish<NS, __half>), dim3(nb), dim3(kBlock), 0, st, g64, (const __half*)h->ml[0].e, r,
+                       q, z, h->scal, h->partials);
   else
-    hipLaunchKernelGGL((k_mix_finish<NS, float, Fin>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64,
-                       (const float*)h->ml[0].e, r, q, z, h->scal, h->partials, h->counter, h->totals, fin);
+    hipLaunchKernelGGL((k_mix_finish<NS, float>), dim3(nb), dim3(kBlock), 0, st, g64, (const float*)h->ml[0].e, r, q,
+                       z, h->scal, h->partials);
+  finish_sums<NS>(h, nb, fin, st);
   HFSI_LAUNCH_CHECK();
   return HFSI_OK;
 }
@@ -1768,8 +1765,9 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   }
 
   auto true_residual = [&](bool count_projection) -> int {
-    hipLaunchKernelGGL((k_true_residual<TV, FinMeanR>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, h->b,
-                       v.r, h->partials, h->counter, h->totals, FinMeanR{sc, rm});
+    hipLaunchKernelGGL((k_true_residual<TV>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, h->b, v.r,
+                       h->partials);
+    finish_sums<2>(h, blocks_for(g64.n), FinMeanR{sc, rm}, st);
     HFSI_LAUNCH_CHECK();
     if (rm) {
       HFSI_TRY(launch_sums<1>(h, g64, OpSubMeans<TV>{nullptr, v.r, sc}, FinRR{sc}, st));
@@ -1798,6 +1796,7 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   // mixed V-cycle: float32 CG vectors, no velocity-Dirichlet or interface terms
   bool use_mix = mixed && use_mg && std::is_same<TV, float>::value && h->ML >= 1 && !h->has_vd && !h->has_diag;
   const double kappa = h->mix_kappa;
+  const int lag = std::min(std::max(0, o.monitor_lag), kMonRing - 1);
   auto mix_scale = [&]() -> int {
     hipLaunchKernelGGL(k_mix_scale_from_rr, dim3(1), dim3(64), 0, st, sc, kappa);
     HFSI_LAUNCH_CHECK();
@@ -1837,6 +1836,7 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   }
   HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 1}, st));
 
+  const bool monitor = use_mix && lag > 0 && h->mon_ring != nullptr;
   int it = 0;
   while (it < max_it) {
     ++it;
@@ -1845,22 +1845,35 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
       launch_march<C, TV, TV, TV, TV, kModeApply, true>(h, 0, outlet, v.d, (const TV*)nullptr, (C)1, v.q, (C)0,
                                                         nullptr, (C)0, (const TV*)nullptr, FinDad{sc}, st);
     else
-      hipLaunchKernelGGL((k_spmv_dot<C, TV, TV, FinDad>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.d, v.q,
-                         h->partials, h->counter, h->totals, FinDad{sc});
+      hipLaunchKernelGGL((k_spmv_dot<C, TV, TV>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.d, v.q,
+                         h->partials);
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
+    if (!march_ok(h, 0)) finish_sums<1>(h, blocks_for(g.n), FinDad{sc}, st);
     HFSI_LAUNCH_CHECK();
+    bool monitor_hit = false;
     if (use_mix) {
       timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
-      hipLaunchKernelGGL((k_cg_update_rr<FinVScale>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x,
-                         (const float*)v.d, (float*)v.r, (const float*)v.q, sc, h->partials, h->counter, h->totals,
-                         FinVScale{sc, kappa});
+      hipLaunchKernelGGL(k_cg_update_rr, dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, (const float*)v.d,
+                         (float*)v.r, (const float*)v.q, sc, h->partials);
       timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
+      double* slot = monitor ? h->mon_ring + (it % kMonRing) : nullptr;
+      finish_sums<1>(h, blocks_for(g64.n), FinVScale{sc, kappa, slot}, st);
+      if (monitor) {
+        HFSI_CHECK(hipEventRecord(h->mon_events[it % kMonRing], st));
+        if (it > lag) {
+          // the value of iteration it - lag: that iteration's work is (nearly) done, the
+          // device keeps the later iterations queued while the host waits for it
+          HFSI_CHECK(hipEventSynchronize(h->mon_events[(it - lag) % kMonRing]));
+          const double rr = h->mon_ring[(it - lag) % kMonRing];
+          monitor_hit = std::sqrt(std::max(rr, 0.0)) <= tol * b_norm;
+        }
+      }
       HFSI_LAUNCH_CHECK();
     } else {
       HFSI_TRY(launch_pointwise(h, 0, OpCgUpdate<TV, TV, TV>{x, v.d, v.r, v.q, sc}, st));
     }
     const bool project = rm && (it % 16 == 0);
-    const bool check = (it % check_every == 0) || it == max_it;
+    const bool check = (it % check_every == 0) || it == max_it || monitor_hit;
     if (project) {
       HFSI_TRY(launch_sums<2>(h, g64, OpSumsXR<TV>{x, v.r}, FinMeansXR{sc}, st));
       HFSI_TRY(launch_sums<1>(h, g64, OpSubMeans<TV>{x, v.r, sc}, FinRR{sc}, st));
@@ -2141,6 +2154,7 @@ static int build_mix_hierarchy(hfsi_fluid* h, const std::vector<float> wf0[3], c
       rc |= dalloc(h, (float**)&m.rhs, m.n);
     }
     rc |= dalloc(h, (float**)&m.e, m.n);
+    if (l + 1 < ML && (l < h->mix_tail_first || ML == 1)) rc |= dalloc(h, (float**)&m.res, m.n);
     for (int a = 0; a < 3 && rc == HFSI_OK; ++a) {
       const size_t n = L.w[a].size();
       std::vector<float> cl(n, 0.0f), ch(n, 0.0f), wf(n);
@@ -2382,6 +2396,18 @@ int hfsi_fluid_create(hfsi_fluid** out, int32_t nx, int32_t ny, int32_t nz, cons
     rc |= dalloc(h, (char**)&h->tail_args_f, sizeof(TailArgs<float, float>));
   }
   if (rc == HFSI_OK && (precision_mask & HFSI_PREC_MIXED)) rc = build_mix_hierarchy(h, wf0, cl0, ch0);
+  if (rc == HFSI_OK && (precision_mask & HFSI_PREC_MIXED)) {
+    if (hipHostMalloc((void**)&h->mon_ring, kMonRing * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      set_error("hipHostMalloc failed");
+      rc = HFSI_ERR_HIP;
+    }
+    h->mon_events.assign(kMonRing, nullptr);
+    for (int i = 0; i < kMonRing && rc == HFSI_OK; ++i)
+      if (hipEventCreateWithFlags(&h->mon_events[i], hipEventDisableTiming) != hipSuccess) {
+        set_error("hipEventCreate failed");
+        rc = HFSI_ERR_HIP;
+      }
+  }
   if (rc == HFSI_OK) {
     (void)hipFuncSetAttribute((const void*)k_vcycle_tail_lds<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kTailLdsMax);
@@ -2418,6 +2444,9 @@ int hfsi_fluid_destroy(hfsi_fluid* h) {
   if (!h) return HFSI_OK;
   (void)hipDeviceSynchronize();
   for (hipEvent_t e : h->timer_events) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->mon_events)
+    if (e) (void)hipEventDestroy(e);
+  if (h->mon_ring) (void)hipHostFree(h->mon_ring);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
   if (h->host_totals) (void)hipHostFree(h->host_totals);
@@ -2802,6 +2831,7 @@ int hfsi_fluid_divergence_report(hfsi_fluid* h, const float* divergence, const f
   a.counter = h->counter;
   a.totals = h->totals;
   hipLaunchKernelGGL(k_div_report, dim3(blocks_for(a.g.n)), dim3(kBlock), 0, st, a);
+  finish_sums<kReportSlots>(h, blocks_for(a.g.n), FinNone{}, st, kReportMaxMask);
   HFSI_LAUNCH_CHECK();
   HFSI_CHECK(hipMemcpyAsync(h->host_totals, h->totals, 54 * sizeof(double), hipMemcpyDeviceToHost, st));
   HFSI_CHECK(hipStreamSynchronize(st));

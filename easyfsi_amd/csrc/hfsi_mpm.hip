@@ -9,8 +9,8 @@
 //                    surface normal / area update
 //   k_mpm_radial     centroid-relative radial stretch diagnostics
 // Every launch also reduces its report quantities deterministically
-// (workgroup partials, last workgroup adds them in order); the final
-// 34-value report snapshot is assembled on the device.
+// (workgroup partials, combined in workgroup order by a one-workgroup finish
+// launch); the final 34-value report snapshot is assembled on the device.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -242,6 +242,11 @@ enum {
   T_COUNT = 44
 };
 
+// max-reduced slots of each stage (relative to the stage's first total)
+constexpr uint64_t kP2gMax = 1ull << T_MAXJ;
+constexpr uint64_t kGridMax = 1ull << 7;
+constexpr uint64_t kRadialMax = 1ull << 2;
+
 __device__ inline void grid_coord(const MpmArgs& a, const float* xp, float X[3]) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) X[d] = (xp[d] - a.bmin[d]) / a.dx[d];
@@ -261,43 +266,6 @@ __device__ inline void bspline(float f, float w[3]) {
   w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
 }
 
-// Deterministic reduction of NS sums with `maxmask` marking max-reduced slots.
-template <int NS>
-__device__ inline bool reduce_stage(double (&v)[NS], uint64_t maxmask, double* partials, unsigned int* counter,
-                                    double* out, double* lds) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) v[s] = ((maxmask >> s) & 1) ? wave_max(v[s]) : wave_sum(v[s]);
-  if (lane == 0)
-#pragma unroll
-    for (int s = 0; s < NS; ++s) lds[s * nw + wave] = v[s];
-  __syncthreads();
-  __shared__ bool is_last;
-  if (threadIdx.x == 0) {
-    for (int s = 0; s < NS; ++s) {
-      double acc = 0.0;
-      for (int w = 0; w < nw; ++w) acc = ((maxmask >> s) & 1) ? fmax(acc, lds[s * nw + w]) : acc + lds[s * nw + w];
-      partials[(size_t)blockIdx.x * NS + s] = acc;
-    }
-    __threadfence();
-    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  if (threadIdx.x < NS) {
-    const int s = threadIdx.x;
-    double acc = 0.0;
-    for (uint32_t b = 0; b < gridDim.x; ++b) {
-      double pv = __hip_atomic_load(&partials[(size_t)b * NS + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      acc = ((maxmask >> s) & 1) ? fmax(acc, pv) : acc + pv;
-    }
-    out[s] = acc;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -445,7 +413,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_p2g(MpmArgs a) {
           }
         }
   }
-  reduce_stage<NS>(acc, (1ull << T_MAXJ), a.partials, a.counter, a.totals + 0, lds);
+  block_partials<NS>(acc, a.partials, lds, kP2gMax);
 }
 
 // Grid update (neo_hookean_mpm.py:803-817).  Totals 12..19.
@@ -472,7 +440,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_grid(MpmArgs a) {
     acc[6] += 1.0;
     acc[7] = fmax(acc[7], (double)sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
   }
-  reduce_stage<NS>(acc, (1ull << 7), a.partials, a.counter, a.totals + T_TRANS, lds);
+  block_partials<NS>(acc, a.partials, lds, kGridMax);
 }
 
 __device__ inline void update_surface(const MpmArgs& a, uint32_t p) {
@@ -612,7 +580,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_g2p(MpmArgs a) {
       acc[20] += 1.0;
     }
   }
-  reduce_stage<NS>(acc, 0ull, a.partials, a.counter, a.totals + T_PDISP, lds);
+  block_partials<NS>(acc, a.partials, lds, 0ull);
 }
 
 // Radial stretch and the final report snapshot (neo_hookean_mpm.py:876-959).
@@ -644,12 +612,27 @@ __global__ void __launch_bounds__(kBlock) k_mpm_radial(MpmArgs a) {
       acc[2] = fmax(acc[2], fabs(st - 1.0));
     }
   }
-  if (reduce_stage<NS>(acc, (1ull << 2), a.partials, a.counter, a.totals + T_RSUM, lds) && threadIdx.x == 0) {
-    const double* t = a.totals;
-    double* R = a.report;
+  block_partials<NS>(acc, a.partials, lds, kRadialMax);
+
+}
+
+
+struct FinNone {
+  __device__ void operator()(const double*) const {}
+};
+
+// Report snapshot (neo_hookean_mpm.py:899-959) from the stage totals, run by
+// the finishing workgroup of the radial-stretch reduction.
+struct FinMpmReport {
+  const double* totals;
+  double* report;
+  float dt;
+  __device__ void operator()(const double*) const {
+    const double* t = totals;
+    double* R = report;
     double e[3], diff2 = 0.0, en = 0.0;
     for (int d = 0; d < 3; ++d) {
-      e[d] = t[T_PMOM + d] + (double)a.dt * t[T_FEXT + d];
+      e[d] = t[T_PMOM + d] + (double)dt * t[T_FEXT + d];
       en += e[d] * e[d];
       diff2 += (t[T_TRANS + d] - e[d]) * (t[T_TRANS + d] - e[d]);
     }
@@ -677,7 +660,7 @@ __global__ void __launch_bounds__(kBlock) k_mpm_radial(MpmArgs a) {
     R[32] = t[T_PCNT];
     R[33] = t[T_SCNT];
   }
-}
+};
 
 }  // namespace hfsi
 
@@ -804,10 +787,20 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* P, int32_t count, const
   a.report = h->report;
   const uint32_t nodes = (uint32_t)h->nx * h->ny * h->nz;
   hipLaunchKernelGGL(k_mpm_clear, dim3(mpm_blocks(nodes)), dim3(kBlock), 0, st, nodes, a.gm, a.gv, a.gf);
-  hipLaunchKernelGGL(k_mpm_p2g, dim3(mpm_blocks(count)), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL(k_mpm_grid, dim3(mpm_blocks(nodes)), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL(k_mpm_g2p, dim3(mpm_blocks(count)), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL(k_mpm_radial, dim3(mpm_blocks(count)), dim3(kBlock), 0, st, a);
+  const uint32_t pb = mpm_blocks(count), nb = mpm_blocks(nodes);
+  hipLaunchKernelGGL(k_mpm_p2g, dim3(pb), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_finish_sums<12, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials, pb,
+                     a.totals + 0, FinNone{}, kP2gMax);
+  hipLaunchKernelGGL(k_mpm_grid, dim3(nb), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_finish_sums<8, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials, nb,
+                     a.totals + T_TRANS, FinNone{}, kGridMax);
+  hipLaunchKernelGGL(k_mpm_g2p, dim3(pb), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_finish_sums<21, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials,
+                     pb, a.totals + T_PDISP, FinNone{}, (uint64_t)0);
+  hipLaunchKernelGGL(k_mpm_radial, dim3(pb), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_finish_sums<3, FinMpmReport>), dim3(1), dim3(kFinishThreads), 0, st,
+                     (const double*)a.partials, pb, a.totals + T_RSUM, FinMpmReport{a.totals, a.report, a.dt},
+                     kRadialMax);
   HFSI_LAUNCH_CHECK();
   if (report34) {
     HFSI_CHECK(hipMemcpyAsync(h->host_report, h->report, 34 * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -84,6 +84,10 @@ typedef struct hfsi_cg_options {
   int32_t restart_interval;   /* exact-residual restart interval (reference: 512)         */
   int32_t true_residual;      /* 1: every check recomputes r = b - A x in float64        */
   int32_t true_residual_at_exit;
+  int32_t monitor_lag;        /* MIXED only, > 0: the host reads the recursive residual norm of
+                                 every iteration this many iterations behind the device (no
+                                 stall) and checks the true residual as soon as it reaches the
+                                 tolerance; 0: checks every check_interval iterations only  */
   double tolerance;           /* relative, on ||r||_V / ||b||_V                            */
   double rhs_scale;           /* rho / dt                                                 */
 } hfsi_cg_options;

@@ -113,7 +113,7 @@ def test_mixed_vcycle_matches_oracle(outlet, half):
     np.testing.assert_allclose(got, want, rtol=0, atol=tol * np.abs(want).max())
 
 
-def _cg(s, div, precision, tol, max_it, outlet, remove_mean, check=10):
+def _cg(s, div, precision, tol, max_it, outlet, remove_mean, check=10, monitor_lag=0):
     from easyfsi_amd import _lib
 
     opt = _lib.CgOptions()
@@ -127,6 +127,7 @@ def _cg(s, div, precision, tol, max_it, outlet, remove_mean, check=10):
     opt.restart_interval = 512
     opt.true_residual = 1
     opt.true_residual_at_exit = 1
+    opt.monitor_lag = monitor_lag
     opt.tolerance = tol
     opt.rhs_scale = s.rho / s.dt
     st = _lib.CgStats()
@@ -162,6 +163,28 @@ def test_mixed_pcg_matches_reference_solution(outlet):
     stf = _cg(sf, div, PREC_FP32, 1e-8, 400, outlet, not outlet, check=1)
     assert stf.converged == 1
     assert st.iterations < stf.iterations, (st.iterations, stf.iterations)
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+def test_mixed_monitor_stops_at_first_converged_iterate(outlet):
+    _say(f"mixed monitor outlet={outlet}")
+    w, obst = _sphere_case(16, 8, 1.2, 0.12)
+    shape = obst.shape
+    rng = np.random.default_rng(53)
+    div = np.where(obst == 0, rng.normal(size=shape), 0.0).astype(np.float32)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    every = _cg(s, div, PREC_MIXED, 1e-8, 200, outlet, not outlet, check=1)
+    p_every = s.pressure.to_numpy().copy()
+    s.pressure.fill(0.0)
+    lagged = _cg(s, div, PREC_MIXED, 1e-8, 200, outlet, not outlet, check=10, monitor_lag=2)
+    assert lagged.converged == 1 and lagged.true_relative_residual <= 1e-8, lagged.as_dict()
+    # the monitor reads the recursive residual 2 iterations late: a few iterations past checking the
+    # true residual every iteration, far fewer host checks
+    assert lagged.iterations <= every.iterations + 3, (every.iterations, lagged.iterations)
+    assert lagged.host_residual_checks < lagged.iterations // 2 + 2
+    np.testing.assert_allclose(s.pressure.to_numpy(), p_every, rtol=0, atol=1e-5 * np.abs(p_every).max())
 
 
 def test_mixed_falls_back_with_dirichlet_rows():
