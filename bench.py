@@ -230,6 +230,51 @@ def cpu_baseline(solver, w, div, iterations_needed: int, budget_s: float = 15.0)
     }
 
 
+def run_mpm(steps: int, warmup: int, lattice: int = 100, nodes: int = 256):
+    """Neo-Hookean MPM steps (P2G, grid update, G2P) of `lattice`^3 particles in a 0.3-side cube centred in
+    the unit box on a `nodes`^3 grid, E = 1e5, nu = 0.45, rho = 1000, dt = 1e-4, rigid rotation about z
+    plus a uniform drift; reports are not read inside the timed loop (read_report=False)."""
+    import torch
+
+    from easyfsi_amd.simulation_core.hyperelastic import NeoHookeanMaterial
+    from easyfsi_amd.simulation_core.neo_hookean_mpm import NeoHookeanMpmState
+
+    mat = NeoHookeanMaterial.from_youngs_modulus(name="bench", density_kgm3=1000.0, youngs_modulus_pa=1.0e5,
+                                                 poissons_ratio=0.45)
+    P = lattice ** 3
+    s = NeoHookeanMpmState(P, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (nodes, nodes, nodes))
+    s.initialize_box(particle_counts=(lattice,) * 3, box_min_m=(0.35,) * 3, box_max_m=(0.65,) * 3,
+                     density_kgm3=1000.0)
+    x = s.x.tensor[:P]
+    omega = 2.0
+    v = torch.zeros_like(x)
+    v[:, 0] = -omega * (x[:, 1] - 0.5)
+    v[:, 1] = omega * (x[:, 0] - 0.5)
+    v[:, 2] = 0.05
+    s.v.tensor[:P] = v
+    kw = dict(dt_s=1.0e-4, mu_pa=mat.shear_modulus_pa, lambda_pa=mat.lame_lambda_pa, primary_region_id=1,
+              secondary_region_id=2)
+    for _ in range(warmup):
+        s.step(**kw, read_report=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(**kw, read_report=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rep = s.step(**kw)
+    torch.cuda.synchronize()
+    # algorithmic bytes: ~290 B per particle (P2G reads x v C F m V f_ext fixed; G2P reads x F fixed, writes
+    # x v C F) + 80 B per active node (P2G write, update read + write, G2P read)
+    bytes_step = 290.0 * P + 80.0 * rep.active_grid_nodes
+    return {
+        "particles": P, "grid_nodes": nodes ** 3, "active_grid_nodes": rep.active_grid_nodes, "steps": steps,
+        "ms_per_step": 1e3 * dt / steps, "particle_steps_per_s": P * steps / dt,
+        "algorithmic_GBps": bytes_step * steps / dt / 1e9, "transfer_relative_error": rep.transfer_relative_error,
+        "max_abs_j": rep.max_abs_j,
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +284,7 @@ def main(argv=None) -> int:
     ap.add_argument("--variants", default="mixed,plain_fp32,ref64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--mpm-steps", type=int, default=20, help="neo-Hookean MPM steps timed beside (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -271,6 +317,12 @@ def main(argv=None) -> int:
     bytes_per_cell = VARIANTS[head_name]["bytes_per_cell"]
     avg_s = head["timer_ms_total"] / max(head["timer_launches"], 1) / 1e3
     achieved = bytes_per_cell * n / avg_s / 1e9 if head["timer_launches"] else 0.0
+    mpm = None
+    if args.mpm_steps > 0:
+        print(f"[bench] neo-Hookean MPM: 1M particles, {args.mpm_steps} steps", file=sys.stderr, flush=True)
+        mpm = {k: (float(f"{v:.6g}") if isinstance(v, float) else v)
+               for k, v in run_mpm(args.mpm_steps, 2).items()}
+        print(f"[bench] MPM: {mpm['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (C oracle)", file=sys.stderr, flush=True)
@@ -312,6 +364,8 @@ def main(argv=None) -> int:
         "variants": {k: {kk: (float(f"{vv:.6g}") if isinstance(vv, float) else vv) for kk, vv in r.items()}
                      for k, r in results.items()},
     }
+    if mpm is not None:
+        line["mpm"] = mpm
     if "ref64" in results and "plain_fp32" in results:
         line["speedup_plain_fp32_vs_ref64"] = round(results["plain_fp32"]["solves_per_s"]
                                                     / results["ref64"]["solves_per_s"], 3)

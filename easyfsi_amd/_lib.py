@@ -121,8 +121,20 @@ SIGNATURES = {
     "hfsi_fluid_mixed_levels": (_I, [_P, ctypes.POINTER(_I)]),
     "hfsi_fluid_mixed_level_shape": (ctypes.c_int, [_P, _I, _P]),
     "hfsi_fluid_mixed_set_half_min_cells": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "hfsi_unet_create": (ctypes.c_int, [ctypes.POINTER(_P), _I, _I, _I, _I, ctypes.c_uint64, _P]),
+    "hfsi_unet_destroy": (ctypes.c_int, [_P]),
+    "hfsi_unet_num_params": (ctypes.c_int64, [_P]),
+    "hfsi_unet_get_params": (ctypes.c_int, [_P, _P, _P]),
+    "hfsi_unet_set_params": (ctypes.c_int, [_P, _P, _P]),
+    "hfsi_unet_get_grads": (ctypes.c_int, [_P, _P, _P]),
+    "hfsi_unet_zero_grads": (ctypes.c_int, [_P, _P]),
+    "hfsi_unet_forward": (ctypes.c_int, [_P, _P, _P, _P]),
+    "hfsi_unet_backward": (ctypes.c_int, [_P, _P, _P]),
+    "hfsi_unet_loss": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
+    "hfsi_unet_adam_step": (ctypes.c_int, [_P, _F, _F, _F, _F, _F, _P]),
     "hfsi_mpm_create": (ctypes.c_int, [ctypes.POINTER(_P), _I, _P, _P, _P, _P]),
     "hfsi_mpm_destroy": (ctypes.c_int, [_P]),
+    "hfsi_mpm_set_p2g_mode": (ctypes.c_int, [_P, _I]),
     "hfsi_mpm_step": (ctypes.c_int, [_P, ctypes.POINTER(MpmParticles), _I, ctypes.POINTER(MpmGrid), _F, _F, _F,
                                      _F, _I, _I, _P, _P]),
 }

@@ -284,6 +284,136 @@ __global__ void __launch_bounds__(kBlock) k_mpm_clear(uint32_t n, float* gm, flo
 }
 
 // P2G (neo_hookean_mpm.py:712-801).  Totals 0..11.
+//
+// Per particle: the SVD clamp of F (persisted when it changed), the first
+// Piola stress of the clamped F and the affine momentum, plus the report
+// sums; the 27-node scatter is done by the caller (global atomics, or LDS
+// atomics into a block tile in the binned path).
+struct P2gPart {
+  bool inb;
+  int base[3];
+  float fx[3], w[3][3];
+  float m, mom[3], aff[3][3], fe[3];
+};
+
+__device__ inline void p2g_prepare(const MpmArgs& a, uint32_t p, double (&acc)[12], P2gPart& q) {
+  float X[3];
+  grid_coord(a, a.x + 3 * (size_t)p, X);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    q.base[d] = (int)floorf(X[d] - 0.5f);
+    q.fx[d] = X[d] - (float)q.base[d];
+    bspline(q.fx[d], q.w[d]);
+  }
+  M3 F;
+  const float* Fp_in = a.F + 9 * (size_t)p;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) F.a[i][j] = (double)Fp_in[3 * i + j];
+  const double raw_J = (double)(float)mdet(F);
+  M3 U, V;
+  double sig[3];
+  svd3(F, U, sig, V);
+  bool clamped = false;
+  double sc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    sc[d] = fmin(fmax(fabs(sig[d]), kMinSigma), kMaxSigma);
+    if (fabs(sc[d] - sig[d]) > 1.0e-6) clamped = true;
+  }
+  if (mdet(mmul(U, mtrans(V))) < 0.0) {
+    clamped = true;
+    U.a[0][2] = -U.a[0][2];
+    U.a[1][2] = -U.a[1][2];
+    U.a[2][2] = -U.a[2][2];
+  }
+  M3 Fc;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      Fc.a[i][j] = U.a[i][0] * sc[0] * V.a[j][0] + U.a[i][1] * sc[1] * V.a[j][1] + U.a[i][2] * sc[2] * V.a[j][2];
+  if (mdet(Fc) <= 0.0) {
+    clamped = true;
+    Fc.a[0][2] = -Fc.a[0][2];
+    Fc.a[1][2] = -Fc.a[1][2];
+    Fc.a[2][2] = -Fc.a[2][2];
+  }
+  if (raw_J <= 0.0) clamped = true;
+  // float32 state: the stress uses the stored (rounded) corrected F
+  M3 Fs;
+  float* Fp_out = a.F + 9 * (size_t)p;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float fv = (float)Fc.a[i][j];
+      Fs.a[i][j] = (double)fv;
+      if (clamped) Fp_out[3 * i + j] = fv;
+    }
+  const double J = mdet(Fs);
+  const M3 FinvT = minv_t(Fs, J);
+  const double lnJ = log(J);
+  M3 Pk;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      Pk.a[i][j] = (double)a.mu * (Fs.a[i][j] - FinvT.a[i][j]) + (double)a.lambda * lnJ * FinvT.a[i][j];
+  const M3 PFt = mmul(Pk, mtrans(Fs));
+  const float m_p = a.mass[p];
+  const float vol_p = a.volume[p];
+  const bool fixed = a.fixed && a.fixed[p] != 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double inv = 4.0 / ((double)a.dx[j] * (double)a.dx[j]);
+      const double stress = -(double)a.dt * (double)vol_p * PFt.a[i][j] * inv;
+      q.aff[i][j] = fixed ? 0.0f : (float)(stress + (double)m_p * (double)a.C[9 * (size_t)p + 3 * i + j]);
+    }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    q.mom[d] = fixed ? 0.0f : m_p * a.v[3 * (size_t)p + d];
+    q.fe[d] = (fixed || !a.f_ext) ? 0.0f : a.f_ext[3 * (size_t)p + d];
+  }
+  q.m = m_p;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    acc[T_PMOM + d] += q.mom[d];
+    acc[T_FEXT + d] += q.fe[d];
+  }
+  acc[T_MASS] += m_p;
+  acc[T_VOL] += vol_p;
+  acc[T_PSQ] += (double)q.mom[0] * q.mom[0] + (double)q.mom[1] * q.mom[1] + (double)q.mom[2] * q.mom[2];
+  acc[T_MAXJ] = fmax(acc[T_MAXJ], fabs(J));
+  acc[T_CLAMP] += clamped ? 1.0 : 0.0;
+  q.inb = !stencil_oob(a, X);
+  if (!q.inb) acc[T_OOB] += 1.0;
+}
+
+// The 27 node contributions of a particle: node(ox, oy, oz) gets mass wt m,
+// momentum wt (m v + aff dpos) and force wt f_ext.
+template <typename Sink>
+__device__ inline void p2g_scatter(const MpmArgs& a, const P2gPart& q, Sink sink) {
+#pragma unroll
+  for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+      for (int oz = 0; oz < 3; ++oz) {
+        const float wt = q.w[0][ox] * q.w[1][oy] * q.w[2][oz];
+        const float dp0 = ((float)ox - q.fx[0]) * a.dx[0];
+        const float dp1 = ((float)oy - q.fx[1]) * a.dx[1];
+        const float dp2 = ((float)oz - q.fx[2]) * a.dx[2];
+        float md[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) md[d] = wt * (q.mom[d] + (q.aff[d][0] * dp0 + q.aff[d][1] * dp1 + q.aff[d][2] * dp2));
+        sink(ox, oy, oz, wt * q.m, md, wt);
+      }
+}
+
 __global__ void __launch_bounds__(kBlock) k_mpm_p2g(MpmArgs a) {
   constexpr int NS = 12;
   __shared__ double lds[NS * (kBlock / 64)];
@@ -291,129 +421,192 @@ __global__ void __launch_bounds__(kBlock) k_mpm_p2g(MpmArgs a) {
 #pragma unroll
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
   const uint32_t n = (uint32_t)a.count;
+  const uint32_t sx = (uint32_t)a.ny * a.nz, sy = a.nz;
   HFSI_CELL_LOOP(n) {
-    const uint32_t p = c;
-    float X[3];
-    grid_coord(a, a.x + 3 * (size_t)p, X);
-    int base[3];
-    float fx[3], w[3][3];
+    P2gPart q;
+    p2g_prepare(a, c, acc, q);
+    if (!q.inb) continue;
+    p2g_scatter(a, q, [&](int ox, int oy, int oz, float mw, const float (&md)[3], float wt) {
+      const uint32_t node = (uint32_t)(q.base[0] + ox) * sx + (uint32_t)(q.base[1] + oy) * sy + (uint32_t)(q.base[2] + oz);
+      atomicAdd(&a.gm[node], mw);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      base[d] = (int)floorf(X[d] - 0.5f);
-      fx[d] = X[d] - (float)base[d];
-      bspline(fx[d], w[d]);
-    }
-    M3 F;
-    const float* Fp_in = a.F + 9 * (size_t)p;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) F.a[i][j] = (double)Fp_in[3 * i + j];
-    const double raw_J = (double)(float)mdet(F);
-    M3 U, V;
-    double sig[3];
-    svd3(F, U, sig, V);
-    bool clamped = false;
-    double sc[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      sc[d] = fmin(fmax(fabs(sig[d]), kMinSigma), kMaxSigma);
-      if (fabs(sc[d] - sig[d]) > 1.0e-6) clamped = true;
-    }
-    if (mdet(mmul(U, mtrans(V))) < 0.0) {
-      clamped = true;
-      U.a[0][2] = -U.a[0][2];
-      U.a[1][2] = -U.a[1][2];
-      U.a[2][2] = -U.a[2][2];
-    }
-    M3 Fc;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        Fc.a[i][j] = U.a[i][0] * sc[0] * V.a[j][0] + U.a[i][1] * sc[1] * V.a[j][1] + U.a[i][2] * sc[2] * V.a[j][2];
-    if (mdet(Fc) <= 0.0) {
-      clamped = true;
-      Fc.a[0][2] = -Fc.a[0][2];
-      Fc.a[1][2] = -Fc.a[1][2];
-      Fc.a[2][2] = -Fc.a[2][2];
-    }
-    if (raw_J <= 0.0) clamped = true;
-    // float32 state: the stress uses the stored (rounded) corrected F
-    M3 Fs;
-    float* Fp_out = a.F + 9 * (size_t)p;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        float fv = (float)Fc.a[i][j];
-        Fs.a[i][j] = (double)fv;
-        if (clamped) Fp_out[3 * i + j] = fv;
+      for (int d = 0; d < 3; ++d) {
+        atomicAdd(&a.gv[3 * (size_t)node + d], md[d]);
+        if (q.fe[d] != 0.0f) atomicAdd(&a.gf[3 * (size_t)node + d], wt * q.fe[d]);
       }
-    const double J = mdet(Fs);
-    const M3 FinvT = minv_t(Fs, J);
-    const double lnJ = log(J);
-    M3 Pk;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        Pk.a[i][j] = (double)a.mu * (Fs.a[i][j] - FinvT.a[i][j]) + (double)a.lambda * lnJ * FinvT.a[i][j];
-    const M3 PFt = mmul(Pk, mtrans(Fs));
-    const float m_p = a.mass[p];
-    const float vol_p = a.volume[p];
-    const bool fixed = a.fixed && a.fixed[p] != 0;
-    float aff[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double inv = 4.0 / ((double)a.dx[j] * (double)a.dx[j]);
-        const double stress = -(double)a.dt * (double)vol_p * PFt.a[i][j] * inv;
-        aff[i][j] = fixed ? 0.0f : (float)(stress + (double)m_p * (double)a.C[9 * (size_t)p + 3 * i + j]);
-      }
-    float mom[3], fe[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      mom[d] = fixed ? 0.0f : m_p * a.v[3 * (size_t)p + d];
-      fe[d] = (fixed || !a.f_ext) ? 0.0f : a.f_ext[3 * (size_t)p + d];
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      acc[T_PMOM + d] += mom[d];
-      acc[T_FEXT + d] += fe[d];
-    }
-    acc[T_MASS] += m_p;
-    acc[T_VOL] += vol_p;
-    acc[T_PSQ] += (double)mom[0] * mom[0] + (double)mom[1] * mom[1] + (double)mom[2] * mom[2];
-    acc[T_MAXJ] = fmax(acc[T_MAXJ], fabs(J));
-    acc[T_CLAMP] += clamped ? 1.0 : 0.0;
-    if (stencil_oob(a, X)) {
-      acc[T_OOB] += 1.0;
-      continue;
-    }
-    const uint32_t sx = (uint32_t)a.ny * a.nz, sy = a.nz;
-#pragma unroll
-    for (int ox = 0; ox < 3; ++ox)
-#pragma unroll
-      for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-        for (int oz = 0; oz < 3; ++oz) {
-          const float wt = w[0][ox] * w[1][oy] * w[2][oz];
-          const float dp0 = ((float)ox - fx[0]) * a.dx[0];
-          const float dp1 = ((float)oy - fx[1]) * a.dx[1];
-          const float dp2 = ((float)oz - fx[2]) * a.dx[2];
-          const uint32_t node = (uint32_t)(base[0] + ox) * sx + (uint32_t)(base[1] + oy) * sy + (uint32_t)(base[2] + oz);
-          atomicAdd(&a.gm[node], wt * m_p);
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            const float md = wt * (mom[d] + (aff[d][0] * dp0 + aff[d][1] * dp1 + aff[d][2] * dp2));
-            atomicAdd(&a.gv[3 * (size_t)node + d], md);
-            if (fe[d] != 0.0f) atomicAdd(&a.gf[3 * (size_t)node + d], wt * fe[d]);
-          }
-        }
+    });
   }
   block_partials<NS>(acc, a.partials, lds, kP2gMax);
+}
+
+// ---------------------------------------------------------------------------
+// Cell-binned P2G: particles are bucketed by the 8^3-node block of their
+// stencil base node; one workgroup accumulates a block's particles into a
+// 10^3-node LDS tile with LDS atomics and adds the tile to the grid with one
+// global atomic per touched node value (instead of 27 x 7 per particle).
+// ---------------------------------------------------------------------------
+
+constexpr int kBinEdge = 8;                 // nodes per block edge
+constexpr int kTileEdge = kBinEdge + 2;     // stencil reach of a block
+constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
+
+struct BinArgs {
+  int32_t nbx, nby, nbz;   // block grid
+  int32_t* key;            // per particle: block, or -1 when out of bounds
+  int32_t* rank;           // per particle: slot within its block
+  int32_t* count;          // per block
+  int32_t* offset;         // per block: exclusive prefix sum of count
+  int32_t* sorted;         // particles ordered by block
+  int32_t* active;         // blocks with particles (unordered)
+  int32_t* nactive;
+  int32_t* oob;            // out-of-bounds particles (unordered)
+  int32_t* noob;
+  int32_t* scan_sums;      // per 1024-block chunk
+};
+
+__global__ void k_bin_clear(BinArgs b) {
+  const int32_t nb = b.nbx * b.nby * b.nbz;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) b.count[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *b.nactive = 0;
+    *b.noob = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin_count(MpmArgs a, BinArgs b) {
+  const uint32_t n = (uint32_t)a.count;
+  HFSI_CELL_LOOP(n) {
+    float X[3];
+    grid_coord(a, a.x + 3 * (size_t)c, X);
+    if (stencil_oob(a, X)) {
+      b.key[c] = -1;
+      b.oob[atomicAdd(b.noob, 1)] = (int32_t)c;
+      continue;
+    }
+    const int bx = (int)floorf(X[0] - 0.5f) / kBinEdge, by = (int)floorf(X[1] - 0.5f) / kBinEdge,
+              bz = (int)floorf(X[2] - 0.5f) / kBinEdge;
+    const int32_t k = (bx * b.nby + by) * b.nbz + bz;
+    b.key[c] = k;
+    const int32_t r = atomicAdd(&b.count[k], 1);
+    b.rank[c] = r;
+    if (r == 0) b.active[atomicAdd(b.nactive, 1)] = k;
+  }
+}
+
+// Exclusive prefix sum of count over nb blocks: per-chunk (1024) scan, scan
+// of the chunk sums (<= 1024 chunks, one workgroup), then add.
+constexpr int kScanChunk = 1024;
+
+__global__ void __launch_bounds__(kScanChunk) k_scan_chunks(const int32_t* in, int32_t* out, int32_t n,
+                                                            int32_t* sums) {
+  __shared__ int32_t t[kScanChunk];
+  const int32_t i = blockIdx.x * kScanChunk + threadIdx.x;
+  const int32_t v = i < n ? in[i] : 0;
+  t[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanChunk; off <<= 1) {
+    const int32_t add = threadIdx.x >= (unsigned)off ? t[threadIdx.x - off] : 0;
+    __syncthreads();
+    t[threadIdx.x] += add;
+    __syncthreads();
+  }
+  if (i < n) out[i] = t[threadIdx.x] - v;  // exclusive
+  if (threadIdx.x == kScanChunk - 1) sums[blockIdx.x] = t[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kScanChunk) k_scan_add(int32_t* out, int32_t n, const int32_t* sums,
+                                                         int32_t nchunks) {
+  __shared__ int32_t t[kScanChunk];
+  // every workgroup scans the (<= 1024) chunk sums itself
+  const int32_t v = (int32_t)threadIdx.x < nchunks ? sums[threadIdx.x] : 0;
+  t[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanChunk; off <<= 1) {
+    const int32_t add = threadIdx.x >= (unsigned)off ? t[threadIdx.x - off] : 0;
+    __syncthreads();
+    t[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const int32_t base = blockIdx.x > 0 ? t[blockIdx.x - 1] : 0;
+  const int32_t i = blockIdx.x * kScanChunk + threadIdx.x;
+  if (i < n) out[i] += base;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin_scatter(int32_t n, BinArgs b) {
+  HFSI_CELL_LOOP((uint32_t)n) {
+    const int32_t k = b.key[c];
+    if (k >= 0) b.sorted[b.offset[k] + b.rank[c]] = (int32_t)c;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_mpm_p2g_binned(MpmArgs a, BinArgs b) {
+  constexpr int NS = 12;
+  __shared__ double lds[NS * (kBlock / 64)];
+  __shared__ float tm[kTileNodes];
+  __shared__ float tp[3][kTileNodes];
+  __shared__ float tf[3][kTileNodes];
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  const int32_t nact = *b.nactive;
+  const uint32_t sx = (uint32_t)a.ny * a.nz, sy = a.nz;
+  for (int32_t ab = blockIdx.x; ab < nact; ab += gridDim.x) {
+    const int32_t k = b.active[ab];
+    const int ox0 = (k / (b.nby * b.nbz)) * kBinEdge, oy0 = ((k / b.nbz) % b.nby) * kBinEdge,
+              oz0 = (k % b.nbz) * kBinEdge;
+    for (int i = threadIdx.x; i < kTileNodes; i += blockDim.x) {
+      tm[i] = 0.0f;
+      tp[0][i] = tp[1][i] = tp[2][i] = 0.0f;
+      tf[0][i] = tf[1][i] = tf[2][i] = 0.0f;
+    }
+    __syncthreads();
+    const int32_t beg = b.offset[k], cnt = b.count[k];
+    for (int32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+      P2gPart q;
+      p2g_prepare(a, (uint32_t)b.sorted[beg + t], acc, q);
+      // q.inb holds: only in-bounds particles are binned
+      const int lx = q.base[0] - ox0, ly = q.base[1] - oy0, lz = q.base[2] - oz0;
+      p2g_scatter(a, q, [&](int ox, int oy, int oz, float mw, const float (&md)[3], float wt) {
+        const int li = ((lx + ox) * kTileEdge + (ly + oy)) * kTileEdge + (lz + oz);
+        atomicAdd(&tm[li], mw);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          atomicAdd(&tp[d][li], md[d]);
+          if (q.fe[d] != 0.0f) atomicAdd(&tf[d][li], wt * q.fe[d]);
+        }
+      });
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTileNodes; i += blockDim.x) {
+      const int tz = i % kTileEdge, ty = (i / kTileEdge) % kTileEdge, tx = i / (kTileEdge * kTileEdge);
+      const int gx = ox0 + tx, gy = oy0 + ty, gz = oz0 + tz;
+      if (gx >= a.nx || gy >= a.ny || gz >= a.nz) continue;
+      const uint32_t node = (uint32_t)gx * sx + (uint32_t)gy * sy + (uint32_t)gz;
+      if (tm[i] != 0.0f) atomicAdd(&a.gm[node], tm[i]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (tp[d][i] != 0.0f) atomicAdd(&a.gv[3 * (size_t)node + d], tp[d][i]);
+        if (tf[d][i] != 0.0f) atomicAdd(&a.gf[3 * (size_t)node + d], tf[d][i]);
+      }
+    }
+    __syncthreads();  // the tile is reused by the next block
+  }
+  block_partials<NS>(acc, a.partials, lds, kP2gMax);
+}
+
+// Out-of-bounds particles of the binned path: the clamp and the report sums only.
+__global__ void __launch_bounds__(kBlock) k_mpm_p2g_oob(MpmArgs a, BinArgs b, double* partials) {
+  constexpr int NS = 12;
+  __shared__ double lds[NS * (kBlock / 64)];
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  const int32_t n = *b.noob;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    P2gPart q;
+    p2g_prepare(a, (uint32_t)b.oob[i], acc, q);
+  }
+  block_partials<NS>(acc, partials, lds, kP2gMax);
 }
 
 // Grid update (neo_hookean_mpm.py:803-817).  Totals 12..19.
@@ -674,6 +867,9 @@ struct hfsi_mpm {
   double* totals = nullptr;
   double* report = nullptr;
   double* host_report = nullptr;
+  int32_t p2g_mode = HFSI_MPM_P2G_BINNED;
+  hfsi::BinArgs bins{};
+  std::vector<void*> allocs;
 };
 
 using namespace hfsi;
@@ -707,6 +903,33 @@ int hfsi_mpm_create(hfsi_mpm** out, int32_t capacity, const double* bounds_min3,
     h->bmin[d] = (float)bounds_min3[d];
     h->dx[d] = (float)((bounds_max3[d] - bounds_min3[d]) / grid_nodes3[d]);
   }
+  {
+    BinArgs& b = h->bins;
+    b.nbx = (h->nx + kBinEdge - 1) / kBinEdge;
+    b.nby = (h->ny + kBinEdge - 1) / kBinEdge;
+    b.nbz = (h->nz + kBinEdge - 1) / kBinEdge;
+    const size_t nb = (size_t)b.nbx * b.nby * b.nbz;
+    const size_t nchunks = (nb + kScanChunk - 1) / kScanChunk;
+    if (nchunks > (size_t)kScanChunk) {
+      set_error("hfsi_mpm_create: grid too large for the particle binning");
+      hfsi_mpm_destroy(h);
+      return HFSI_ERR_ARG;
+    }
+    auto al = [&](int32_t** p, size_t n) {
+      void* q = nullptr;
+      if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(int32_t)) != hipSuccess) return false;
+      h->allocs.push_back(q);
+      *p = (int32_t*)q;
+      return true;
+    };
+    const size_t P = (size_t)capacity;
+    if (!(al(&b.key, P) && al(&b.rank, P) && al(&b.count, nb) && al(&b.offset, nb) && al(&b.sorted, P) &&
+          al(&b.active, nb) && al(&b.nactive, 1) && al(&b.oob, P) && al(&b.noob, 1) && al(&b.scan_sums, nchunks))) {
+      set_error("hfsi_mpm_create: device allocation failed");
+      hfsi_mpm_destroy(h);
+      return HFSI_ERR_HIP;
+    }
+  }
   bool ok = hipMalloc((void**)&h->partials, sizeof(double) * 2048 * 32) == hipSuccess &&
             hipMalloc((void**)&h->counter, sizeof(unsigned int)) == hipSuccess &&
             hipMalloc((void**)&h->totals, sizeof(double) * T_COUNT) == hipSuccess &&
@@ -724,6 +947,15 @@ int hfsi_mpm_create(hfsi_mpm** out, int32_t capacity, const double* bounds_min3,
   return HFSI_OK;
 }
 
+int hfsi_mpm_set_p2g_mode(hfsi_mpm* h, int32_t mode) {
+  if (!h || (mode != HFSI_MPM_P2G_BINNED && mode != HFSI_MPM_P2G_ATOMIC)) {
+    set_error("hfsi_mpm_set_p2g_mode: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  h->p2g_mode = mode;
+  return HFSI_OK;
+}
+
 int hfsi_mpm_destroy(hfsi_mpm* h) {
   if (!h) return HFSI_OK;
   (void)hipDeviceSynchronize();
@@ -732,6 +964,7 @@ int hfsi_mpm_destroy(hfsi_mpm* h) {
   if (h->totals) (void)hipFree(h->totals);
   if (h->report) (void)hipFree(h->report);
   if (h->host_report) (void)hipHostFree(h->host_report);
+  for (void* p : h->allocs) (void)hipFree(p);
   delete h;
   return HFSI_OK;
 }
@@ -788,9 +1021,27 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* P, int32_t count, const
   const uint32_t nodes = (uint32_t)h->nx * h->ny * h->nz;
   hipLaunchKernelGGL(k_mpm_clear, dim3(mpm_blocks(nodes)), dim3(kBlock), 0, st, nodes, a.gm, a.gv, a.gf);
   const uint32_t pb = mpm_blocks(count), nb = mpm_blocks(nodes);
-  hipLaunchKernelGGL(k_mpm_p2g, dim3(pb), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL((k_finish_sums<12, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials, pb,
-                     a.totals + 0, FinNone{}, kP2gMax);
+  if (h->p2g_mode == HFSI_MPM_P2G_BINNED) {
+    const BinArgs& b = h->bins;
+    const int32_t nb = b.nbx * b.nby * b.nbz;
+    const int32_t nchunks = (nb + kScanChunk - 1) / kScanChunk;
+    const uint32_t G = (uint32_t)std::min<int32_t>(nb, 2048), G2 = 64;
+    hipLaunchKernelGGL(k_bin_clear, dim3(std::min<int32_t>((nb + 255) / 256, 1024)), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_bin_count, dim3(pb), dim3(kBlock), 0, st, a, b);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(nchunks), dim3(kScanChunk), 0, st, (const int32_t*)b.count, b.offset, nb,
+                       b.scan_sums);
+    hipLaunchKernelGGL(k_scan_add, dim3(nchunks), dim3(kScanChunk), 0, st, b.offset, nb, (const int32_t*)b.scan_sums,
+                       nchunks);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(pb), dim3(kBlock), 0, st, count, b);
+    hipLaunchKernelGGL(k_mpm_p2g_binned, dim3(G), dim3(kBlock), 0, st, a, b);
+    hipLaunchKernelGGL(k_mpm_p2g_oob, dim3(G2), dim3(kBlock), 0, st, a, b, a.partials + (size_t)G * 12);
+    hipLaunchKernelGGL((k_finish_sums<12, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials,
+                       G + G2, a.totals + 0, FinNone{}, kP2gMax);
+  } else {
+    hipLaunchKernelGGL(k_mpm_p2g, dim3(pb), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_finish_sums<12, FinNone>), dim3(1), dim3(kFinishThreads), 0, st,
+                       (const double*)a.partials, pb, a.totals + 0, FinNone{}, kP2gMax);
+  }
   hipLaunchKernelGGL(k_mpm_grid, dim3(nb), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_finish_sums<8, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials, nb,
                      a.totals + T_TRANS, FinNone{}, kGridMax);

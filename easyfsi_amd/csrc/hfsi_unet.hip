@@ -1,0 +1,523 @@
+// libhfsi: learned pressure initial guess, a 3-level 3D U-Net in bf16 on MFMA.
+//
+// No reference counterpart.  Topology (channels 16 / 32 / 64, two 3x3x3
+// convolutions per level, 2x2x2 average pooling, nearest upsampling with skip
+// concatenation, ReLU after every 3x3x3 convolution, 1x1 output):
+//   x0 (5 -> 16 padded)   E1A 16->16, E1B 16->16 = s1
+//   pool                  E2A 16->32, E2B 32->32 = s2
+//   pool                  B3A 32->64, B3B 64->64
+//   up, [.., s2] (96)     D2A 96->32, D2B 32->32
+//   up, [.., s1] (48)     D1A 48->16, D1B 16->16
+//   output                1x1 16->1 (float32)
+// Parameters are float32 masters (Adam), converted to bf16 working weights
+// after every update; activations and their gradients are bf16, every
+// accumulation is float32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/hfsi.h"
+#include "unet_kernels.hpp"
+
+namespace hfsi {
+void set_error(const std::string& msg);
+
+enum { E1A, E1B, E2A, E2B, B3A, B3B, D2A, D2B, D1A, D1B, NCONV };
+
+struct UConv {
+  int cin, cout, cin_true;
+  size_t w_off, b_off;
+  bf16* wf = nullptr;  // forward weights [cout][KP(cin)]
+  bf16* wb = nullptr;  // data-gradient weights [cin][KP(cout)] (null for E1A)
+};
+
+static inline int grid1d(long n, int threads = 256) {
+  long b = (n + threads - 1) / threads;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+}  // namespace hfsi
+
+using namespace hfsi;
+
+struct hfsi_unet {
+  int D = 0, H = 0, W = 0;
+  long n1 = 0, n2 = 0, n3 = 0;
+  bool train = false;
+  UConv conv[NCONV];
+  size_t w_out_off = 0, b_out_off = 0, nparams = 0;
+  float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+  bf16 *x0 = nullptr, *a1 = nullptr, *s1 = nullptr, *p1 = nullptr, *a2 = nullptr, *s2 = nullptr, *p2 = nullptr,
+       *a3 = nullptr, *b3 = nullptr, *u2 = nullptr, *a4 = nullptr, *d2 = nullptr, *u1 = nullptr, *a5 = nullptr,
+       *d1 = nullptr;
+  bf16 *g_a1 = nullptr, *g_s1 = nullptr, *g_p1 = nullptr, *g_a2 = nullptr, *g_s2 = nullptr, *g_p2 = nullptr,
+       *g_a3 = nullptr, *g_b3 = nullptr, *g_u2 = nullptr, *g_a4 = nullptr, *g_d2 = nullptr, *g_u1 = nullptr,
+       *g_a5 = nullptr, *g_d1 = nullptr;
+  float* wpartial = nullptr;
+  float* bpartial = nullptr;
+  double* partials = nullptr;
+  double* totals = nullptr;
+  double* host_totals = nullptr;
+  long adam_step = 0;
+  std::vector<void*> allocs;
+};
+
+namespace hfsi {
+
+template <typename T>
+static int ualloc(hfsi_unet* h, T** out, size_t count) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+    set_error("hfsi_unet: device allocation failed");
+    return HFSI_ERR_HIP;
+  }
+  h->allocs.push_back(p);
+  *out = (T*)p;
+  return HFSI_OK;
+}
+
+// ---- dispatch over the channel pairs of the topology ----------------------
+
+template <int CI, int CO>
+static void conv_fwd_t(const bf16* x, const bf16* w, const float* b, bf16* y, int D, int H, int W, bool relu,
+                       hipStream_t st) {
+  const long n = (long)D * H * W;
+  const int vox_per_block = kConvVoxWave * (kConvThreads / 64);
+  const int blocks = (int)((n + vox_per_block - 1) / vox_per_block);
+  if (relu)
+    hipLaunchKernelGGL((k_conv3_fwd<CI, CO, true>), dim3(blocks), dim3(kConvThreads), 0, st, x, w, b, y, CO, 0, D, H,
+                       W);
+  else
+    hipLaunchKernelGGL((k_conv3_fwd<CI, CO, false>), dim3(blocks), dim3(kConvThreads), 0, st, x, w, b, y, CO, 0, D,
+                       H, W);
+}
+
+static int conv_fwd(int ci, int co, const bf16* x, const bf16* w, const float* b, bf16* y, int D, int H, int W,
+                    bool relu, hipStream_t st) {
+#define HFSI_CONV_CASE(A, B)                                 \
+  if (ci == A && co == B) {                                  \
+    conv_fwd_t<A, B>(x, w, b, y, D, H, W, relu, st);         \
+    return HFSI_OK;                                          \
+  }
+  HFSI_CONV_CASE(16, 16)
+  HFSI_CONV_CASE(16, 32)
+  HFSI_CONV_CASE(32, 32)
+  HFSI_CONV_CASE(32, 64)
+  HFSI_CONV_CASE(64, 64)
+  HFSI_CONV_CASE(96, 32)
+  HFSI_CONV_CASE(48, 16)
+  // data-gradient shapes (input = layer output channels)
+  HFSI_CONV_CASE(16, 48)
+  HFSI_CONV_CASE(32, 96)
+  HFSI_CONV_CASE(64, 32)
+  HFSI_CONV_CASE(32, 16)
+#undef HFSI_CONV_CASE
+  set_error("hfsi_unet: unsupported convolution shape");
+  return HFSI_ERR_ARG;
+}
+
+static int wgrad_chunks(long n, int* chunk) {
+  long nch = (n + 1023) / 1024;
+  if (nch > 64) nch = 64;
+  if (nch < 1) nch = 1;
+  long c = (n + nch - 1) / nch;
+  c = (c + kWgStep - 1) / kWgStep * kWgStep;
+  *chunk = (int)c;
+  return (int)((n + c - 1) / c);
+}
+
+template <int CI, int CO>
+static void conv_wgrad_t(const bf16* x, const bf16* dy, float* partial, int D, int H, int W, int nch, int chunk,
+                         hipStream_t st) {
+  hipLaunchKernelGGL((k_conv3_wgrad<CI, CO>), dim3(nch, 27), dim3(kConvThreads), 0, st, x, dy, partial, D, H, W,
+                     chunk);
+}
+
+static int conv_wgrad(hfsi_unet* h, const UConv& L, const bf16* x, const bf16* dy, int D, int H, int W,
+                      hipStream_t st) {
+  const long n = (long)D * H * W;
+  int chunk = 0;
+  const int nch = wgrad_chunks(n, &chunk);
+  const int ci = L.cin, co = L.cout;
+  bool ok = true;
+  if (ci == 16 && co == 16) conv_wgrad_t<16, 16>(x, dy, h->wpartial, D, H, W, nch, chunk, st);
+  else if (ci == 16 && co == 32) conv_wgrad_t<16, 32>(x, dy, h->wpartial, D, H, W, nch, chunk, st);
+  else if (ci == 32 && co == 32) conv_wgrad_t<32, 32>(x, dy, h->wpartial, D, H, W, nch, chunk, st);
+  else if (ci == 32 && co == 64) conv_wgrad_t<32, 64>(x, dy, h->wpartial, D, H, W, nch, chunk, st);
+  else if (ci == 64 && co == 64) conv_wgrad_t<64, 64>(x, dy, h->wpartial, D, H, W, nch, chunk, st);
+  else if (ci == 96 && co == 32) conv_wgrad_t<96, 32>(x, dy, h->wpartial, D, H, W, nch, chunk, st);
+  else if (ci == 48 && co == 16) conv_wgrad_t<48, 16>(x, dy, h->wpartial, D, H, W, nch, chunk, st);
+  else ok = false;
+  if (!ok) {
+    set_error("hfsi_unet: unsupported weight-gradient shape");
+    return HFSI_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid1d(27L * ci * co)), dim3(256), 0, st, (const float*)h->wpartial, nch,
+                     ci, co, h->grads + L.w_off);
+  // bias gradient
+  const int nb = grid1d(n / (256 / co) + 1);
+  if (co == 16) hipLaunchKernelGGL((k_bias_grad<16>), dim3(nb), dim3(256), 0, st, dy, n, h->bpartial);
+  else if (co == 32) hipLaunchKernelGGL((k_bias_grad<32>), dim3(nb), dim3(256), 0, st, dy, n, h->bpartial);
+  else hipLaunchKernelGGL((k_bias_grad<64>), dim3(nb), dim3(256), 0, st, dy, n, h->bpartial);
+  hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(64), 0, st, (const float*)h->bpartial, nb, co,
+                     h->grads + L.b_off);
+  return HFSI_OK;
+}
+
+// Output-layer gradient accumulation from the combined sums {sum dy u[c], sum dy}.
+struct FinOut {
+  float* gw;
+  float* gb;
+  __device__ void operator()(const double* t) const {
+    for (int c = 0; c < 16; ++c) gw[c] += (float)t[c];
+    gb[0] += (float)t[16];
+  }
+};
+
+struct FinLoss {
+  __device__ void operator()(const double*) const {}
+};
+
+static void relu_bwd(bf16* g, const bf16* y, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_relu_bwd, dim3(grid1d(n)), dim3(256), 0, st, g, y, n);
+}
+
+static int refresh_weights(hfsi_unet* h, hipStream_t st) {
+  for (int l = 0; l < NCONV; ++l) {
+    const UConv& L = h->conv[l];
+    const long n = (long)L.cout * conv_kp(L.cin) + (L.wb ? (long)L.cin * conv_kp(L.cout) : 0);
+    hipLaunchKernelGGL(k_conv_weights, dim3(grid1d(n)), dim3(256), 0, st, (const float*)h->params + L.w_off, L.cin,
+                       L.cout, L.wf, L.wb);
+  }
+  if (hipGetLastError() != hipSuccess) {
+    set_error("hfsi_unet: weight conversion launch failed");
+    return HFSI_ERR_HIP;
+  }
+  return HFSI_OK;
+}
+
+}  // namespace hfsi
+
+extern "C" {
+
+int hfsi_unet_create(hfsi_unet** out, int32_t nx, int32_t ny, int32_t nz, int32_t train, uint64_t seed,
+                     void* stream) {
+  if (!out || nx < 4 || ny < 4 || nz < 4 || nx % 4 || ny % 4 || nz % 4) {
+    set_error("hfsi_unet_create: grid dimensions must be multiples of 4 (two 2x pooling levels)");
+    return HFSI_ERR_ARG;
+  }
+  if ((long)nx * ny * nz * 96 >= (1L << 40)) {
+    set_error("hfsi_unet_create: grid too large");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hfsi_unet* h = new hfsi_unet();
+  h->D = nx;
+  h->H = ny;
+  h->W = nz;
+  h->n1 = (long)nx * ny * nz;
+  h->n2 = h->n1 / 8;
+  h->n3 = h->n2 / 8;
+  h->train = train != 0;
+  const int shapes[NCONV][3] = {{16, 16, 5},  {16, 16, 16}, {16, 32, 16}, {32, 32, 32}, {32, 64, 32},
+                                {64, 64, 64}, {96, 32, 96}, {32, 32, 32}, {48, 16, 48}, {16, 16, 16}};
+  size_t off = 0;
+  for (int l = 0; l < NCONV; ++l) {
+    UConv& L = h->conv[l];
+    L.cin = shapes[l][0];
+    L.cout = shapes[l][1];
+    L.cin_true = shapes[l][2];
+    L.w_off = off;
+    off += (size_t)L.cout * 27 * L.cin;
+    L.b_off = off;
+    off += (size_t)L.cout;
+  }
+  h->w_out_off = off;
+  off += 16;
+  h->b_out_off = off;
+  off += 1;
+  h->nparams = off;
+  int rc = HFSI_OK;
+  rc |= ualloc(h, &h->params, off);
+  rc |= ualloc(h, &h->grads, off);
+  rc |= ualloc(h, &h->m, off);
+  rc |= ualloc(h, &h->v, off);
+  for (int l = 0; l < NCONV && rc == HFSI_OK; ++l) {
+    UConv& L = h->conv[l];
+    rc |= ualloc(h, &L.wf, (size_t)L.cout * conv_kp(L.cin));
+    if (l != E1A) rc |= ualloc(h, &L.wb, (size_t)L.cin * conv_kp(L.cout));
+  }
+  const long n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  if (rc == HFSI_OK) {
+    rc |= ualloc(h, &h->x0, n1 * 16);
+    rc |= ualloc(h, &h->a1, n1 * 16);
+    rc |= ualloc(h, &h->s1, n1 * 16);
+    rc |= ualloc(h, &h->p1, n2 * 16);
+    rc |= ualloc(h, &h->a2, n2 * 32);
+    rc |= ualloc(h, &h->s2, n2 * 32);
+    rc |= ualloc(h, &h->p2, n3 * 32);
+    rc |= ualloc(h, &h->a3, n3 * 64);
+    rc |= ualloc(h, &h->b3, n3 * 64);
+    rc |= ualloc(h, &h->u2, n2 * 96);
+    rc |= ualloc(h, &h->a4, n2 * 32);
+    rc |= ualloc(h, &h->d2, n2 * 32);
+    rc |= ualloc(h, &h->u1, n1 * 48);
+    rc |= ualloc(h, &h->a5, n1 * 16);
+    rc |= ualloc(h, &h->d1, n1 * 16);
+    rc |= ualloc(h, &h->partials, 8192 * 32);
+    rc |= ualloc(h, &h->totals, 64);
+  }
+  if (rc == HFSI_OK && h->train) {
+    rc |= ualloc(h, &h->g_a1, n1 * 16);
+    rc |= ualloc(h, &h->g_s1, n1 * 16);
+    rc |= ualloc(h, &h->g_p1, n2 * 16);
+    rc |= ualloc(h, &h->g_a2, n2 * 32);
+    rc |= ualloc(h, &h->g_s2, n2 * 32);
+    rc |= ualloc(h, &h->g_p2, n3 * 32);
+    rc |= ualloc(h, &h->g_a3, n3 * 64);
+    rc |= ualloc(h, &h->g_b3, n3 * 64);
+    rc |= ualloc(h, &h->g_u2, n2 * 96);
+    rc |= ualloc(h, &h->g_a4, n2 * 32);
+    rc |= ualloc(h, &h->g_d2, n2 * 32);
+    rc |= ualloc(h, &h->g_u1, n1 * 48);
+    rc |= ualloc(h, &h->g_a5, n1 * 16);
+    rc |= ualloc(h, &h->g_d1, n1 * 16);
+    rc |= ualloc(h, &h->wpartial, (size_t)64 * 27 * 96 * 32);
+    rc |= ualloc(h, &h->bpartial, (size_t)8192 * 64);
+  }
+  if (rc == HFSI_OK && hipHostMalloc((void**)&h->host_totals, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    set_error("hfsi_unet_create: hipHostMalloc failed");
+    rc = HFSI_ERR_HIP;
+  }
+  if (rc == HFSI_OK) {
+    // He-normal convolution weights over the true fan-in, zero biases; the
+    // padded input channels of E1A start (and stay near) zero.
+    std::vector<float> p(off, 0.0f);
+    std::mt19937_64 gen(seed);
+    for (int l = 0; l < NCONV; ++l) {
+      const UConv& L = h->conv[l];
+      std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (27.0f * L.cin_true)));
+      for (int co = 0; co < L.cout; ++co)
+        for (int tap = 0; tap < 27; ++tap)
+          for (int ci = 0; ci < L.cin; ++ci)
+            p[L.w_off + ((size_t)co * 27 + tap) * L.cin + ci] = ci < L.cin_true ? nd(gen) : 0.0f;
+    }
+    std::normal_distribution<float> no(0.0f, 0.25f);
+    for (int c = 0; c < 16; ++c) p[h->w_out_off + c] = no(gen);
+    if (hipMemcpy(h->params, p.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(h->grads, 0, off * sizeof(float)) != hipSuccess ||
+        hipMemset(h->m, 0, off * sizeof(float)) != hipSuccess ||
+        hipMemset(h->v, 0, off * sizeof(float)) != hipSuccess) {
+      set_error("hfsi_unet_create: parameter upload failed");
+      rc = HFSI_ERR_HIP;
+    }
+  }
+  if (rc == HFSI_OK) rc = refresh_weights(h, st);
+  if (rc == HFSI_OK && hipStreamSynchronize(st) != hipSuccess) {
+    set_error("hfsi_unet_create: synchronisation failed");
+    rc = HFSI_ERR_HIP;
+  }
+  if (rc != HFSI_OK) {
+    hfsi_unet_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return HFSI_OK;
+}
+
+int hfsi_unet_destroy(hfsi_unet* h) {
+  if (!h) return HFSI_OK;
+  (void)hipDeviceSynchronize();
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->host_totals) (void)hipHostFree(h->host_totals);
+  delete h;
+  return HFSI_OK;
+}
+
+int64_t hfsi_unet_num_params(const hfsi_unet* h) { return h ? (int64_t)h->nparams : -1; }
+
+int hfsi_unet_get_params(hfsi_unet* h, float* host, void* stream) {
+  if (!h || !host) {
+    set_error("hfsi_unet_get_params: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  HFSI_CHECK(hipMemcpyAsync(host, h->params, h->nparams * sizeof(float), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  return HFSI_OK;
+}
+
+int hfsi_unet_set_params(hfsi_unet* h, const float* host, void* stream) {
+  if (!h || !host) {
+    set_error("hfsi_unet_set_params: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  HFSI_CHECK(hipMemcpyAsync(h->params, host, h->nparams * sizeof(float), hipMemcpyHostToDevice, st));
+  int rc = refresh_weights(h, st);
+  HFSI_CHECK(hipStreamSynchronize(st));
+  return rc;
+}
+
+int hfsi_unet_get_grads(hfsi_unet* h, float* host, void* stream) {
+  if (!h || !host) {
+    set_error("hfsi_unet_get_grads: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  HFSI_CHECK(hipMemcpyAsync(host, h->grads, h->nparams * sizeof(float), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  return HFSI_OK;
+}
+
+int hfsi_unet_zero_grads(hfsi_unet* h, void* stream) {
+  if (!h) {
+    set_error("hfsi_unet_zero_grads: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  HFSI_CHECK(hipMemsetAsync(h->grads, 0, h->nparams * sizeof(float), (hipStream_t)stream));
+  return HFSI_OK;
+}
+
+int hfsi_unet_forward(hfsi_unet* h, const float* in5, float* out, void* stream) {
+  if (!h || !in5 || !out) {
+    set_error("hfsi_unet_forward: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int D = h->D, H = h->H, W = h->W;
+  const int D2 = D / 2, H2 = H / 2, W2 = W / 2, D3 = D / 4, H3 = H / 4, W3 = W / 4;
+  const float* P = h->params;
+  auto b = [&](int l) { return P + h->conv[l].b_off; };
+  hipLaunchKernelGGL(k_unet_input, dim3(grid1d(h->n1 * 16)), dim3(256), 0, st, in5, h->x0, h->n1);
+  int rc = HFSI_OK;
+  rc |= conv_fwd(16, 16, h->x0, h->conv[E1A].wf, b(E1A), h->a1, D, H, W, true, st);
+  rc |= conv_fwd(16, 16, h->a1, h->conv[E1B].wf, b(E1B), h->s1, D, H, W, true, st);
+  hipLaunchKernelGGL(k_pool2_fwd, dim3(grid1d(h->n2 * 16)), dim3(256), 0, st, h->s1, h->p1, D, H, W, 16);
+  rc |= conv_fwd(16, 32, h->p1, h->conv[E2A].wf, b(E2A), h->a2, D2, H2, W2, true, st);
+  rc |= conv_fwd(32, 32, h->a2, h->conv[E2B].wf, b(E2B), h->s2, D2, H2, W2, true, st);
+  hipLaunchKernelGGL(k_pool2_fwd, dim3(grid1d(h->n3 * 32)), dim3(256), 0, st, h->s2, h->p2, D2, H2, W2, 32);
+  rc |= conv_fwd(32, 64, h->p2, h->conv[B3A].wf, b(B3A), h->a3, D3, H3, W3, true, st);
+  rc |= conv_fwd(64, 64, h->a3, h->conv[B3B].wf, b(B3B), h->b3, D3, H3, W3, true, st);
+  hipLaunchKernelGGL(k_upcat_fwd, dim3(grid1d(h->n2 * 96)), dim3(256), 0, st, h->b3, h->s2, h->u2, D2, H2, W2, 64, 32);
+  rc |= conv_fwd(96, 32, h->u2, h->conv[D2A].wf, b(D2A), h->a4, D2, H2, W2, true, st);
+  rc |= conv_fwd(32, 32, h->a4, h->conv[D2B].wf, b(D2B), h->d2, D2, H2, W2, true, st);
+  hipLaunchKernelGGL(k_upcat_fwd, dim3(grid1d(h->n1 * 48)), dim3(256), 0, st, h->d2, h->s1, h->u1, D, H, W, 32, 16);
+  rc |= conv_fwd(48, 16, h->u1, h->conv[D1A].wf, b(D1A), h->a5, D, H, W, true, st);
+  rc |= conv_fwd(16, 16, h->a5, h->conv[D1B].wf, b(D1B), h->d1, D, H, W, true, st);
+  hipLaunchKernelGGL((k_out_fwd<16>), dim3(grid1d(h->n1)), dim3(256), 0, st, h->d1, P + h->w_out_off,
+                     P + h->b_out_off, out, h->n1);
+  if (rc != HFSI_OK) return rc;
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+int hfsi_unet_backward(hfsi_unet* h, const float* dout, void* stream) {
+  if (!h || !dout || !h->train) {
+    set_error("hfsi_unet_backward: invalid arguments (or handle created without training buffers)");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int D = h->D, H = h->H, W = h->W;
+  const int D2 = D / 2, H2 = H / 2, W2 = W / 2, D3 = D / 4, H3 = H / 4, W3 = W / 4;
+  const long n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  int rc = HFSI_OK;
+  // output layer
+  {
+    const int nb = grid1d(n1);
+    hipLaunchKernelGGL((k_out_bwd<16>), dim3(nb), dim3(256), 0, st, h->d1, (const float*)h->params + h->w_out_off,
+                       dout, h->g_d1, n1, h->partials);
+    hipLaunchKernelGGL((k_finish_sums<17, FinOut>), dim3(1), dim3(kFinishThreads), 0, st,
+                       (const double*)h->partials, (uint32_t)nb, h->totals,
+                       FinOut{h->grads + h->w_out_off, h->grads + h->b_out_off}, (uint64_t)0);
+  }
+  HFSI_CHECK(hipMemsetAsync(h->g_s1, 0, n1 * 16 * sizeof(bf16), st));
+  HFSI_CHECK(hipMemsetAsync(h->g_s2, 0, n2 * 32 * sizeof(bf16), st));
+  // level 1 decoder
+  relu_bwd(h->g_d1, h->d1, n1 * 16, st);
+  rc |= conv_wgrad(h, h->conv[D1B], h->a5, h->g_d1, D, H, W, st);
+  rc |= conv_fwd(16, 16, h->g_d1, h->conv[D1B].wb, nullptr, h->g_a5, D, H, W, false, st);
+  relu_bwd(h->g_a5, h->a5, n1 * 16, st);
+  rc |= conv_wgrad(h, h->conv[D1A], h->u1, h->g_a5, D, H, W, st);
+  rc |= conv_fwd(16, 48, h->g_a5, h->conv[D1A].wb, nullptr, h->g_u1, D, H, W, false, st);
+  hipLaunchKernelGGL(k_upcat_bwd_coarse, dim3(grid1d(n2 * 32)), dim3(256), 0, st, h->g_u1, h->g_d2, D, H, W, 32, 16);
+  hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(grid1d(n1 * 16)), dim3(256), 0, st, h->g_u1, h->g_s1, n1, 32, 16);
+  // level 2 decoder
+  relu_bwd(h->g_d2, h->d2, n2 * 32, st);
+  rc |= conv_wgrad(h, h->conv[D2B], h->a4, h->g_d2, D2, H2, W2, st);
+  rc |= conv_fwd(32, 32, h->g_d2, h->conv[D2B].wb, nullptr, h->g_a4, D2, H2, W2, false, st);
+  relu_bwd(h->g_a4, h->a4, n2 * 32, st);
+  rc |= conv_wgrad(h, h->conv[D2A], h->u2, h->g_a4, D2, H2, W2, st);
+  rc |= conv_fwd(32, 96, h->g_a4, h->conv[D2A].wb, nullptr, h->g_u2, D2, H2, W2, false, st);
+  hipLaunchKernelGGL(k_upcat_bwd_coarse, dim3(grid1d(n3 * 64)), dim3(256), 0, st, h->g_u2, h->g_b3, D2, H2, W2, 64,
+                     32);
+  hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(grid1d(n2 * 32)), dim3(256), 0, st, h->g_u2, h->g_s2, n2, 64, 32);
+  // bottom
+  relu_bwd(h->g_b3, h->b3, n3 * 64, st);
+  rc |= conv_wgrad(h, h->conv[B3B], h->a3, h->g_b3, D3, H3, W3, st);
+  rc |= conv_fwd(64, 64, h->g_b3, h->conv[B3B].wb, nullptr, h->g_a3, D3, H3, W3, false, st);
+  relu_bwd(h->g_a3, h->a3, n3 * 64, st);
+  rc |= conv_wgrad(h, h->conv[B3A], h->p2, h->g_a3, D3, H3, W3, st);
+  rc |= conv_fwd(64, 32, h->g_a3, h->conv[B3A].wb, nullptr, h->g_p2, D3, H3, W3, false, st);
+  hipLaunchKernelGGL(k_pool2_bwd, dim3(grid1d(n2 * 32)), dim3(256), 0, st, h->g_p2, h->g_s2, D2, H2, W2, 32, 1);
+  // level 2 encoder
+  relu_bwd(h->g_s2, h->s2, n2 * 32, st);
+  rc |= conv_wgrad(h, h->conv[E2B], h->a2, h->g_s2, D2, H2, W2, st);
+  rc |= conv_fwd(32, 32, h->g_s2, h->conv[E2B].wb, nullptr, h->g_a2, D2, H2, W2, false, st);
+  relu_bwd(h->g_a2, h->a2, n2 * 32, st);
+  rc |= conv_wgrad(h, h->conv[E2A], h->p1, h->g_a2, D2, H2, W2, st);
+  rc |= conv_fwd(32, 16, h->g_a2, h->conv[E2A].wb, nullptr, h->g_p1, D2, H2, W2, false, st);
+  hipLaunchKernelGGL(k_pool2_bwd, dim3(grid1d(n1 * 16)), dim3(256), 0, st, h->g_p1, h->g_s1, D, H, W, 16, 1);
+  // level 1 encoder
+  relu_bwd(h->g_s1, h->s1, n1 * 16, st);
+  rc |= conv_wgrad(h, h->conv[E1B], h->a1, h->g_s1, D, H, W, st);
+  rc |= conv_fwd(16, 16, h->g_s1, h->conv[E1B].wb, nullptr, h->g_a1, D, H, W, false, st);
+  relu_bwd(h->g_a1, h->a1, n1 * 16, st);
+  rc |= conv_wgrad(h, h->conv[E1A], h->x0, h->g_a1, D, H, W, st);
+  if (rc != HFSI_OK) return rc;
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+int hfsi_unet_loss(hfsi_unet* h, const float* out, const float* target, const uint8_t* mask, float* dout,
+                   double* loss, void* stream) {
+  if (!h || !out || !target || !dout) {
+    set_error("hfsi_unet_loss: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = grid1d(h->n1);
+  hipLaunchKernelGGL(k_loss_sums, dim3(nb), dim3(256), 0, st, out, target, mask, h->n1, h->partials);
+  hipLaunchKernelGGL((k_finish_sums<2, FinLoss>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)h->partials,
+                     (uint32_t)nb, h->totals, FinLoss{}, (uint64_t)0);
+  hipLaunchKernelGGL(k_loss_grad, dim3(grid1d(h->n1)), dim3(256), 0, st, out, target, mask, h->n1,
+                     (const double*)h->totals, dout);
+  HFSI_LAUNCH_CHECK();
+  if (loss) {
+    HFSI_CHECK(hipMemcpyAsync(h->host_totals, h->totals, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HFSI_CHECK(hipStreamSynchronize(st));
+    *loss = h->host_totals[1] > 0.0 ? h->host_totals[0] / h->host_totals[1] : 0.0;
+  }
+  return HFSI_OK;
+}
+
+int hfsi_unet_adam_step(hfsi_unet* h, float lr, float beta1, float beta2, float eps, float weight_decay,
+                        void* stream) {
+  if (!h || !(lr > 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f)) {
+    set_error("hfsi_unet_adam_step: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  h->adam_step += 1;
+  const float c1 = 1.0f - std::pow(beta1, (float)h->adam_step);
+  const float c2 = 1.0f - std::pow(beta2, (float)h->adam_step);
+  hipLaunchKernelGGL(k_adam, dim3(grid1d((long)h->nparams)), dim3(256), 0, st, h->params, (const float*)h->grads,
+                     h->m, h->v, (long)h->nparams, lr, beta1, beta2, eps, c1, c2, weight_decay);
+  return refresh_weights(h, st);
+}
+
+}  // extern "C"

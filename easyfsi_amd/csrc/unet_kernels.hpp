@@ -1,0 +1,421 @@
+// Kernels of the learned initial-guess network (3D U-Net, bf16 on MFMA) for gfx950.
+//
+// No reference counterpart: the network maps divergence, solid mask and cell
+// spacing to a pressure initial guess that the mixed MGPCG then finishes.
+//
+// Layout: activations are channels-last bf16, voxel index ((z * H + y) * W + x)
+// with x fastest, i.e. the fluid's cell index [i][j][k].  Channel counts are
+// multiples of 8 (the 5 input channels are padded to 16), so a lane's 8
+// consecutive K values of an MFMA operand are 8 channels of one voxel: one
+// 16-byte load.
+//
+// 3x3x3 convolution, zero padding, as an implicit GEMM
+//   y[v][co] = bias[co] + sum_{tap, ci} W[co][tap * CIN + ci] * x[v + off(tap)][ci]
+// with tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1):
+//   M = voxels (16 per MFMA row block), N = output channels, K = 27 * CIN,
+//   v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][8 (l / 16) .. + 7],
+//   B[8 (l / 16) .. + 7][l % 16] and D[4 (l / 16) + i][l % 16], i < 4.
+// The data gradient is the same convolution of dy with the flipped, transposed
+// weights W'[ci][tap' * COUT + co] = W[co][(26 - tap') * CIN + ci]; the weight
+// gradient dW[co][tap][ci] = sum_v dy[v][co] x[v + off(tap)][ci] is an MFMA
+// GEMM with K = voxels, both operands transposed through LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hfsi_common.hpp"
+
+namespace hfsi {
+
+typedef __bf16 bf16;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+constexpr int kConvThreads = 256;  // 4 waves
+constexpr int kConvVoxWave = 64;   // voxels per wave (4 MFMA row blocks)
+
+__host__ __device__ constexpr int conv_kp(int cin) { return (27 * cin + 31) / 32 * 32; }
+
+__device__ inline bf16x8 bf16x8_zero() {
+  bf16x8 z;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) z[e] = (bf16)0.0f;
+  return z;
+}
+
+__device__ inline void tap_offset(int tap, int& dz, int& dy, int& dx) {
+  dz = tap / 9 - 1;
+  dy = (tap / 3) % 3 - 1;
+  dx = tap % 3 - 1;
+}
+
+// y[v * ycs + yoff + co] = act(bias[co] + conv(x)[v][co]); bias may be null.
+template <int CIN, int COUT, bool RELU>
+__global__ void __launch_bounds__(kConvThreads) k_conv3_fwd(const bf16* __restrict__ x, const bf16* __restrict__ wt,
+                                                             const float* __restrict__ bias, bf16* __restrict__ y,
+                                                             int ycs, int yoff, int D, int H, int W) {
+  static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+  constexpr int KP = conv_kp(CIN);
+  constexpr int NT = COUT / 16;
+  constexpr int MT = kConvVoxWave / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long nvox = (long)D * H * W;
+  const long v0 = ((long)blockIdx.x * (kConvThreads / 64) + wave) * kConvVoxWave;
+  if (v0 >= nvox) return;  // no barriers in this kernel
+  const int r = lane & 15, g = lane >> 4;
+  int vz[MT], vy[MT], vx[MT];
+  bool vin[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const long v = v0 + 16 * mt + r;
+    vin[mt] = v < nvox;
+    const long vv = vin[mt] ? v : 0;
+    vx[mt] = (int)(vv % W);
+    vy[mt] = (int)((vv / W) % H);
+    vz[mt] = (int)(vv / ((long)W * H));
+  }
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int kb = 0; kb < KP; kb += 32) {
+    const int k = kb + 8 * g;
+    const int tap = k / CIN, c0 = k % CIN;
+    bf16x8 b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) b[nt] = *reinterpret_cast<const bf16x8*>(wt + (size_t)(nt * 16 + r) * KP + k);
+    int dz = 0, dy = 0, dx = 0;
+    const bool tap_ok = tap < 27;
+    if (tap_ok) tap_offset(tap, dz, dy, dx);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      bf16x8 a = bf16x8_zero();
+      const int zz = vz[mt] + dz, yy = vy[mt] + dy, xx = vx[mt] + dx;
+      if (tap_ok && vin[mt] && zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        a = *reinterpret_cast<const bf16x8*>(x + (((long)zz * H + yy) * W + xx) * CIN + c0);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int co = nt * 16 + r;
+      const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long v = v0 + 16 * mt + 4 * g + i;
+        if (v < nvox) {
+          float o = acc[mt][nt][i] + bv;
+          if (RELU) o = fmaxf(o, 0.0f);
+          y[v * ycs + yoff + co] = (bf16)o;
+        }
+      }
+    }
+}
+
+// Weight gradient, per (voxel chunk, tap): partial[chunk][tap][co][ci] =
+// sum over the chunk of dy[v][co] x[v + off(tap)][ci].  Each 128-voxel step
+// stages dy and the shifted x transposed ([channel][voxel]) in LDS.
+constexpr int kWgStep = 128;
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(kConvThreads) k_conv3_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dy,
+                                                               float* __restrict__ partial, int D, int H, int W,
+                                                               int chunk) {
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
+  constexpr int MT = COUT / 16, NTC = CIN / 16, TILES = MT * NTC;
+  constexpr int TPW = (TILES + 3) / 4;  // tiles per wave
+  __shared__ __attribute__((aligned(16))) bf16 sdy[COUT][kWgStep];
+  __shared__ __attribute__((aligned(16))) bf16 sx[CIN][kWgStep];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int tap = blockIdx.y;
+  int dz, dyo, dx;
+  tap_offset(tap, dz, dyo, dx);
+  const long nvox = (long)D * H * W;
+  const long vbeg = (long)blockIdx.x * chunk;
+  const long vend = vbeg + chunk < nvox ? vbeg + chunk : nvox;
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (long vs = vbeg; vs < vend; vs += kWgStep) {
+    __syncthreads();  // previous step's LDS reads are done
+    // stage dy: COUT/8 vectors per voxel
+    for (int idx = threadIdx.x; idx < kWgStep * (COUT / 8); idx += blockDim.x) {
+      const int t = idx / (COUT / 8), cv = idx % (COUT / 8);
+      const long v = vs + t;
+      bf16x8 val = bf16x8_zero();
+      if (v < vend) val = *reinterpret_cast<const bf16x8*>(dy + v * COUT + cv * 8);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) sdy[cv * 8 + e][t] = val[e];
+    }
+    // stage the shifted input
+    for (int idx = threadIdx.x; idx < kWgStep * (CIN / 8); idx += blockDim.x) {
+      const int t = idx / (CIN / 8), cv = idx % (CIN / 8);
+      const long v = vs + t;
+      bf16x8 val = bf16x8_zero();
+      if (v < vend) {
+        const int xx = (int)(v % W) + dx, yy = (int)((v / W) % H) + dyo, zz = (int)(v / ((long)W * H)) + dz;
+        if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+          val = *reinterpret_cast<const bf16x8*>(x + (((long)zz * H + yy) * W + xx) * CIN + cv * 8);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) sx[cv * 8 + e][t] = val[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kWgStep; kk += 32) {
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        const int tile = wave + 4 * t;
+        if (tile < TILES) {
+          const int mt = tile / NTC, nt = tile % NTC;
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdy[mt * 16 + r][kk + 8 * g]);
+          const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[nt * 16 + r][kk + 8 * g]);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float* out = partial + ((size_t)blockIdx.x * 27 + tap) * COUT * CIN;
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = wave + 4 * t;
+    if (tile < TILES) {
+      const int mt = tile / NTC, nt = tile % NTC;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[(mt * 16 + 4 * g + i) * CIN + nt * 16 + r] = acc[t][i];
+    }
+  }
+}
+
+// grad[co][tap * cin + ci] += sum over chunks of partial[chunk][tap][co][ci]
+__global__ void k_wgrad_reduce(const float* __restrict__ partial, int nchunks, int cin, int cout,
+                               float* __restrict__ grad) {
+  const int n = 27 * cin * cout;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+    const int ci = idx % cin;
+    const int co = (idx / cin) % cout;
+    const int tap = idx / (cin * cout);
+    double s = 0.0;
+    for (int c = 0; c < nchunks; ++c) s += (double)partial[((size_t)c * 27 + tap) * cout * cin + co * cin + ci];
+    grad[(size_t)co * 27 * cin + tap * cin + ci] += (float)s;
+  }
+}
+
+// Bias gradient partials: partial[block][co] = sum of dy[v][co] over the block's voxels.
+template <int COUT>
+__global__ void __launch_bounds__(256) k_bias_grad(const bf16* __restrict__ dy, long nvox, float* __restrict__ partial) {
+  __shared__ float red[256 / COUT][COUT];
+  const int co = threadIdx.x % COUT, lanegrp = threadIdx.x / COUT;
+  float s = 0.0f;
+  for (long v = (long)blockIdx.x * (256 / COUT) + lanegrp; v < nvox; v += (long)gridDim.x * (256 / COUT))
+    s += (float)dy[v * COUT + co];
+  red[lanegrp][co] = s;
+  __syncthreads();
+  if (threadIdx.x < COUT) {
+    float t = 0.0f;
+    for (int q = 0; q < 256 / COUT; ++q) t += red[q][threadIdx.x];
+    partial[(size_t)blockIdx.x * COUT + threadIdx.x] = t;
+  }
+}
+
+__global__ void k_bias_reduce(const float* __restrict__ partial, int nblocks, int cout, float* __restrict__ grad) {
+  const int co = threadIdx.x;
+  if (co >= cout) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += (double)partial[(size_t)b * cout + co];
+  grad[co] += (float)s;
+}
+
+// dy *= (y > 0)   (ReLU backward through the stored post-activation output)
+__global__ void k_relu_bwd(bf16* __restrict__ dy, const bf16* __restrict__ y, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (!((float)y[i] > 0.0f)) dy[i] = (bf16)0.0f;
+}
+
+// 2x2x2 average pooling (even fine dims): y[D/2][H/2][W/2][C]
+__global__ void k_pool2_fwd(const bf16* __restrict__ x, bf16* __restrict__ y, int D, int H, int W, int C) {
+  const int d = D / 2, hh = H / 2, w = W / 2;
+  const long n = (long)d * hh * w * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long v = i / C;
+    const int X = (int)(v % w), Y = (int)((v / w) % hh), Z = (int)(v / ((long)w * hh));
+    float s = 0.0f;
+    for (int oz = 0; oz < 2; ++oz)
+      for (int oy = 0; oy < 2; ++oy)
+        for (int ox = 0; ox < 2; ++ox)
+          s += (float)x[((((long)(2 * Z + oz) * H) + (2 * Y + oy)) * W + (2 * X + ox)) * C + c];
+    y[i] = (bf16)(0.125f * s);
+  }
+}
+
+// dx[fine] (=|+=) dy[parent] / 8
+__global__ void k_pool2_bwd(const bf16* __restrict__ dy, bf16* __restrict__ dx, int D, int H, int W, int C,
+                            int accumulate) {
+  const long n = (long)D * H * W * C;
+  const int hh = H / 2, w = W / 2;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long v = i / C;
+    const int X = (int)(v % W), Y = (int)((v / W) % H), Z = (int)(v / ((long)W * H));
+    const float g = 0.125f * (float)dy[((((long)(Z / 2) * hh) + Y / 2) * w + X / 2) * C + c];
+    dx[i] = (bf16)(accumulate ? (float)dx[i] + g : g);
+  }
+}
+
+// out[v][0:CC] = coarse[parent(v)], out[v][CC:CC+CS] = skip[v]   (fine dims D, H, W)
+__global__ void k_upcat_fwd(const bf16* __restrict__ coarse, const bf16* __restrict__ skip, bf16* __restrict__ out,
+                            int D, int H, int W, int CC, int CS) {
+  const int C = CC + CS;
+  const long n = (long)D * H * W * C;
+  const int hh = H / 2, w = W / 2;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long v = i / C;
+    if (c < CC) {
+      const int X = (int)(v % W), Y = (int)((v / W) % H), Z = (int)(v / ((long)W * H));
+      out[i] = coarse[((((long)(Z / 2) * hh) + Y / 2) * w + X / 2) * CC + c];
+    } else {
+      out[i] = skip[v * CS + (c - CC)];
+    }
+  }
+}
+
+// dcoarse[p][c] = sum of the 8 children's dout[v][c] (c < CC); dskip[v][c] += dout[v][CC + c]
+__global__ void k_upcat_bwd_coarse(const bf16* __restrict__ dout, bf16* __restrict__ dcoarse, int D, int H, int W,
+                                   int CC, int CS) {
+  const int C = CC + CS;
+  const int d = D / 2, hh = H / 2, w = W / 2;
+  const long n = (long)d * hh * w * CC;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % CC);
+    const long p = i / CC;
+    const int X = (int)(p % w), Y = (int)((p / w) % hh), Z = (int)(p / ((long)w * hh));
+    float s = 0.0f;
+    for (int oz = 0; oz < 2; ++oz)
+      for (int oy = 0; oy < 2; ++oy)
+        for (int ox = 0; ox < 2; ++ox)
+          s += (float)dout[((((long)(2 * Z + oz) * H) + (2 * Y + oy)) * W + (2 * X + ox)) * C + c];
+    dcoarse[i] = (bf16)s;
+  }
+}
+
+__global__ void k_upcat_bwd_skip(const bf16* __restrict__ dout, bf16* __restrict__ dskip, long nvox, int CC, int CS) {
+  const int C = CC + CS;
+  const long n = nvox * CS;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / CS;
+    const int c = (int)(i % CS);
+    dskip[i] = (bf16)((float)dskip[i] + (float)dout[v * C + CC + c]);
+  }
+}
+
+// Output layer: y[v] = b + sum_c w[c] u[v][c]  (CU channels, float output)
+template <int CU>
+__global__ void k_out_fwd(const bf16* __restrict__ u, const float* __restrict__ w, const float* __restrict__ b,
+                          float* __restrict__ y, long nvox) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    float s = b[0];
+#pragma unroll
+    for (int c = 0; c < CU; ++c) s += w[c] * (float)u[v * CU + c];
+    y[v] = s;
+  }
+}
+
+// Output layer backward: du[v][c] = dy[v] w[c]; partials of (sum_v dy u[v][c], sum_v dy).
+template <int CU>
+__global__ void __launch_bounds__(256) k_out_bwd(const bf16* __restrict__ u, const float* __restrict__ w,
+                                                 const float* __restrict__ dyv, bf16* __restrict__ du, long nvox,
+                                                 double* __restrict__ partials) {
+  __shared__ double lds[(CU + 1) * 4];
+  double acc[CU + 1];
+#pragma unroll
+  for (int c = 0; c <= CU; ++c) acc[c] = 0.0;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    const float g = dyv[v];
+#pragma unroll
+    for (int c = 0; c < CU; ++c) {
+      const float uc = (float)u[v * CU + c];
+      acc[c] += (double)(g * uc);
+      du[v * CU + c] = (bf16)(g * w[c]);
+    }
+    acc[CU] += (double)g;
+  }
+  block_partials<CU + 1>(acc, partials, lds);
+}
+
+// Relative squared error over the masked voxels: partials of
+// (sum m (y - t)^2, sum m t^2); the gradient is formed by k_loss_grad.
+__global__ void __launch_bounds__(256) k_loss_sums(const float* __restrict__ y, const float* __restrict__ t,
+                                                   const uint8_t* __restrict__ mask, long nvox,
+                                                   double* __restrict__ partials) {
+  __shared__ double lds[2 * 4];
+  double acc[2] = {0.0, 0.0};
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    if (mask && !mask[v]) continue;
+    const double e = (double)y[v] - (double)t[v];
+    acc[0] += e * e;
+    acc[1] += (double)t[v] * (double)t[v];
+  }
+  block_partials<2>(acc, partials, lds);
+}
+
+// dy = 2 (y - t) / sum m t^2 on masked voxels, 0 elsewhere (sums in totals[0..1])
+__global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict__ t, const uint8_t* __restrict__ mask,
+                            long nvox, const double* __restrict__ totals, float* __restrict__ dy) {
+  const double den = totals[1] > 0.0 ? totals[1] : 1.0;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x)
+    dy[v] = (mask && !mask[v]) ? 0.0f : (float)(2.0 * ((double)y[v] - (double)t[v]) / den);
+}
+
+// Adam on the float32 master parameters.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long n, float lr, float b1, float b2, float eps, float c1, float c2, float weight_decay) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i] + weight_decay * p[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// bf16 working weights of a conv layer from its float32 master [cout][27 cin]:
+// forward [cout][KP(cin)] and flipped-transposed [cin][KP(cout)], zero padded.
+__global__ void k_conv_weights(const float* __restrict__ wm, int cin, int cout, bf16* __restrict__ wf,
+                               bf16* __restrict__ wb) {
+  const int kpf = conv_kp(cin), kpb = conv_kp(cout);
+  const int nf = cout * kpf, nb = wb ? cin * kpb : 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf + nb; i += gridDim.x * blockDim.x) {
+    if (i < nf) {
+      const int co = i / kpf, k = i % kpf;
+      wf[i] = (bf16)(k < 27 * cin ? wm[(size_t)co * 27 * cin + k] : 0.0f);
+    } else {
+      const int j = i - nf;
+      const int ci = j / kpb, k = j % kpb;
+      float val = 0.0f;
+      if (k < 27 * cout) {
+        const int tapb = k / cout, co = k % cout;
+        val = wm[(size_t)co * 27 * cin + (26 - tapb) * cin + ci];
+      }
+      wb[j] = (bf16)val;
+    }
+  }
+}
+
+// Network input: the 5 float channels of each voxel -> 16 bf16 channels (zero padded).
+__global__ void k_unet_input(const float* __restrict__ in5, bf16* __restrict__ x, long nvox) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nvox * 16; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / 16;
+    const int c = (int)(i % 16);
+    x[i] = (bf16)(c < 5 ? in5[v * 5 + c] : 0.0f);
+  }
+}
+
+}  // namespace hfsi

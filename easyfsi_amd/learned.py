@@ -1,0 +1,350 @@
+"""Learned pressure initial guess for the mixed-precision MGPCG.
+
+No reference counterpart.  A 3-level 3D U-Net (libhfsi ``hfsi_unet_*``: bf16
+MFMA forward and backward kernels, Adam on float32 masters) maps
+
+    divergence / RMS,  solid fraction,  log2(cell width / min width) per axis
+
+to the pressure of the same cell in the units of the scaled operator
+``p_hat = -p kappa / ((rho / dt) RMS(div))`` (kappa = 1 / min width^2).  The
+guess is used as the PCG starting point after multiplying it by the factor
+that minimises the energy-norm error, ``gamma = <p, b>_V / <p, A p>_V``, so it
+can never start the solve further from the solution (in that norm) than
+``p = 0``; MGPCG then finishes to the requested true-residual tolerance, so
+the plain method stays both the fallback and the judge.
+
+The network runs on the 2:1-coarsened grid of the solve (features are
+volume-averaged over 2x2x2 cells, the result is interpolated back), which is
+the resolution it is trained at.  Feature assembly, averaging and the
+interpolation use torch on the device (a handful of element-wise passes); the
+network itself runs in libhfsi.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+import torch
+import torch.nn.functional as TF
+
+from . import _lib
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_guess.npz")
+
+
+class GuessNetwork:
+    """The U-Net of libhfsi for one (coarse) grid shape."""
+
+    def __init__(self, shape, *, train: bool = False, seed: int = 0):
+        self._lib = _lib.lib()
+        self.shape = tuple(int(v) for v in shape)
+        if any(v % 4 for v in self.shape):
+            raise ValueError("the guess network needs grid dimensions that are multiples of 4")
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.hfsi_unet_create(ctypes.byref(h), *self.shape, 1 if train else 0, int(seed),
+                                              _lib.stream_handle()), "hfsi_unet_create")
+        self._h = h
+        self.train_enabled = train
+        self.num_params = int(self._lib.hfsi_unet_num_params(h))
+        n = int(np.prod(self.shape))
+        self._out = torch.zeros(self.shape, dtype=torch.float32, device="cuda")
+        self._dout = torch.zeros(n, dtype=torch.float32, device="cuda") if train else None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.hfsi_unet_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- parameters --------------------------------------------------------
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        _lib.check(self._lib.hfsi_unet_get_params(self._h, p.ctypes.data, _lib.stream_handle()), "get_params")
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {p.size}")
+        _lib.check(self._lib.hfsi_unet_set_params(self._h, p.ctypes.data, _lib.stream_handle()), "set_params")
+
+    def save(self, path: str, **meta) -> None:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        np.savez(path, params=self.get_params(), **{k: np.asarray(v) for k, v in meta.items()})
+
+    def load(self, path: str) -> dict:
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"])
+            return {k: z[k] for k in z.files if k != "params"}
+
+    # -- compute -----------------------------------------------------------
+    def forward(self, features: torch.Tensor) -> torch.Tensor:
+        """features: [n, 5] float32 (cuda, contiguous); returns the [shape] float32 output (a view that the
+        next forward overwrites)."""
+        f = features.contiguous()
+        _lib.check(self._lib.hfsi_unet_forward(self._h, f.data_ptr(), self._out.data_ptr(), _lib.stream_handle()),
+                   "hfsi_unet_forward")
+        return self._out
+
+    def train_step(self, features: torch.Tensor, target: torch.Tensor, mask: torch.Tensor | None, *, lr: float,
+                   read_loss: bool = False) -> float | None:
+        """One Adam step on the relative squared error of the masked cells."""
+        if not self.train_enabled:
+            raise RuntimeError("GuessNetwork was created without training buffers")
+        st = _lib.stream_handle()
+        _lib.check(self._lib.hfsi_unet_zero_grads(self._h, st), "zero_grads")
+        out = self.forward(features)
+        loss = ctypes.c_double()
+        m = mask.contiguous() if mask is not None else None
+        _lib.check(self._lib.hfsi_unet_loss(self._h, out.data_ptr(), target.contiguous().data_ptr(),
+                                            m.data_ptr() if m is not None else None, self._dout.data_ptr(),
+                                            ctypes.byref(loss) if read_loss else None, st), "hfsi_unet_loss")
+        _lib.check(self._lib.hfsi_unet_backward(self._h, self._dout.data_ptr(), st), "hfsi_unet_backward")
+        _lib.check(self._lib.hfsi_unet_adam_step(self._h, float(lr), 0.9, 0.999, 1e-8, 0.0, st), "adam")
+        return float(loss.value) if read_loss else None
+
+
+# ---------------------------------------------------------------------------
+# Features and targets
+# ---------------------------------------------------------------------------
+
+def _volumes(widths, device="cuda"):
+    w = [torch.as_tensor(np.asarray(a, np.float64), device=device) for a in widths]
+    return w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
+
+
+def coarsen_widths(widths):
+    """Pairwise sums (2:1 coarsening of every axis; even lengths)."""
+    return [np.asarray(a, np.float64)[0::2] + np.asarray(a, np.float64)[1::2] for a in widths]
+
+
+def features(div: torch.Tensor, obstacle: torch.Tensor, widths, *, coarsen: bool):
+    """Network input [n, 5] and the divergence RMS it was normalised with.
+
+    div: float32 [nx, ny, nz]; obstacle: int32 (nonzero = not free).  With
+    ``coarsen`` the features are volume averages over 2x2x2 cells of the
+    solve grid and the network grid is the coarsened one."""
+    free = (obstacle == 0).to(torch.float64)
+    V = _volumes(widths, div.device)
+    d = div.to(torch.float64) * free
+    if coarsen:
+        vf = V * free
+        num = TF.avg_pool3d((d * V)[None, None], 2)[0, 0]
+        vol = TF.avg_pool3d(V[None, None], 2)[0, 0]
+        fvol = TF.avg_pool3d(vf[None, None], 2)[0, 0]
+        d = torch.where(fvol > 0, num / fvol.clamp_min(1e-300), torch.zeros_like(num))
+        solid = 1.0 - fvol / vol
+        widths = coarsen_widths(widths)
+        V = vol * 8.0
+        freew = fvol
+    else:
+        solid = 1.0 - free
+        freew = V * free
+    rms = math.sqrt(float((freew * d * d).sum() / freew.sum().clamp_min(1e-300)))
+    rms = rms if rms > 0 else 1.0
+    hmin = min(float(np.min(a)) for a in widths)
+    shape = d.shape
+    f = torch.empty(shape + (5,), dtype=torch.float32, device=div.device)
+    f[..., 0] = (d / rms).to(torch.float32)
+    f[..., 1] = solid.to(torch.float32)
+    for a in range(3):
+        lw = torch.as_tensor(np.log2(np.asarray(widths[a], np.float64) / hmin), dtype=torch.float32, device=div.device)
+        sh = [1, 1, 1]
+        sh[a] = shape[a]
+        f[..., 2 + a] = lw.reshape(sh).expand(shape)
+    return f.reshape(-1, 5), rms, widths
+
+
+def target(pressure: torch.Tensor, rms: float, widths, rho_over_dt: float, *, coarsen: bool) -> torch.Tensor:
+    """Normalised pressure p_hat = -p kappa / ((rho/dt) rms) on the network grid (volume-averaged when coarsened)."""
+    hmin = min(float(np.min(a)) for a in widths)
+    p = pressure.to(torch.float64)
+    if coarsen:
+        V = _volumes(widths, p.device)
+        p = TF.avg_pool3d((p * V)[None, None], 2)[0, 0] / TF.avg_pool3d(V[None, None], 2)[0, 0]
+        hmin = min(float(np.min(a)) for a in coarsen_widths(widths))
+    kappa = 1.0 / (hmin * hmin)
+    return (-p * kappa / (rho_over_dt * rms)).to(torch.float32)
+
+
+# ---------------------------------------------------------------------------
+# Guess for a solver
+# ---------------------------------------------------------------------------
+
+class LearnedPressureGuess:
+    """Initial guess for CartesianFluidSolver pressure solves from a trained network.
+
+    ``apply(solver)`` writes the guess into ``solver.pressure`` (the warm start of
+    the next PCG solve) and returns the energy-optimal factor it used."""
+
+    def __init__(self, solver, network: GuessNetwork | None = None, weights: str | None = DEFAULT_WEIGHTS):
+        shape = (solver.nx, solver.ny, solver.nz)
+        self.coarsen = all(v % 8 == 0 for v in shape)
+        net_shape = tuple(v // 2 for v in shape) if self.coarsen else shape
+        self.net = network if network is not None else GuessNetwork(net_shape)
+        if network is None and weights is not None:
+            if not os.path.exists(weights):
+                raise FileNotFoundError(f"guess network weights not found: {weights}")
+            self.net.load(weights)
+        if self.net.shape != net_shape:
+            raise ValueError(f"network shape {self.net.shape} does not match {net_shape}")
+        g = solver.grid
+        self.widths = [np.asarray(g.cell_widths_x_m), np.asarray(g.cell_widths_y_m), np.asarray(g.cell_widths_z_m)]
+        self._V = _volumes(self.widths)
+        self._Ap = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        self.last_gamma = 0.0
+
+    def apply(self, solver, *, outlet: bool = False) -> float:
+        div = solver.divergence.tensor
+        obst = solver.obstacle.tensor
+        f, rms, wnet = features(div, obst, self.widths, coarsen=self.coarsen)
+        out = self.net.forward(f)
+        hmin_net = min(float(np.min(a)) for a in wnet)
+        kappa = 1.0 / (hmin_net * hmin_net)
+        rho_over_dt = float(np.float32(solver.rho / solver.dt))
+        p_net = out.to(torch.float64) * (-(rho_over_dt * rms) / kappa)
+        if self.coarsen:
+            p = TF.interpolate(p_net[None, None], scale_factor=2, mode="trilinear", align_corners=False)[0, 0]
+        else:
+            p = p_net
+        free = obst == 0
+        p = torch.where(free, p, torch.zeros_like(p))
+        # b = -(rho/dt) (div - source) with the V-mean removed on closed domains (the CG right-hand side)
+        b = -rho_over_dt * (div.to(torch.float64) - solver.volume_source_s.tensor.to(torch.float64))
+        b = torch.where(obst == 1, torch.zeros_like(b), b)
+        if not outlet:
+            Vf = self._V * free
+            b = torch.where(free, b - (Vf * b).sum() / Vf.sum(), torch.zeros_like(b))
+            p = torch.where(free, p - (Vf * p).sum() / Vf.sum(), torch.zeros_like(p))
+        p = p.contiguous()
+        _lib.check(solver._lib.hfsi_fluid_apply(solver._handle, p.data_ptr(), self._Ap.data_ptr(), int(outlet),
+                                                solver._stream), "hfsi_fluid_apply")
+        num = float((self._V * p * b)[free].sum())
+        den = float((self._V * p * self._Ap)[free].sum())
+        gamma = num / den if den > 0 else 0.0
+        solver.pressure.tensor.copy_(p * gamma)
+        self.last_gamma = gamma
+        return gamma
+
+
+# ---------------------------------------------------------------------------
+# Training on projections the plain solver generates
+# ---------------------------------------------------------------------------
+
+def bench_axis(n: int, growth: float = 1.04) -> np.ndarray:
+    """The graded axis of the 256^3 benchmark family: n/2 core cells, n/4 per side at `growth`, unit length."""
+    from .simulation_core.grid import explicit_graded_axis
+
+    raw = np.asarray(explicit_graded_axis(n // 2, 1.0, n // 4, growth))
+    return raw / raw.sum()
+
+
+def fourier_velocity(shape, kmax: float, seed: int) -> torch.Tensor:
+    """Velocity with N(0,1) complex coefficients on 0 < |k| <= kmax (index space), unit RMS per component."""
+    nx, ny, nz = shape
+    dev = torch.device("cuda")
+    kx = torch.fft.fftfreq(nx, d=1.0 / nx, device=dev)
+    ky = torch.fft.fftfreq(ny, d=1.0 / ny, device=dev)
+    kz = torch.fft.rfftfreq(nz, d=1.0 / nz, device=dev)
+    k2 = kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2
+    mask = ((k2 > 0) & (k2 <= kmax * kmax)).to(torch.float32)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    out = torch.empty(tuple(shape) + (3,), dtype=torch.float32, device=dev)
+    for d in range(3):
+        re = torch.randn(mask.shape, generator=gen, device=dev)
+        im = torch.randn(mask.shape, generator=gen, device=dev)
+        f = torch.fft.irfftn(torch.complex(re * mask, im * mask), s=tuple(shape))
+        out[..., d] = (f / f.pow(2).mean().sqrt()).to(torch.float32)
+    return out
+
+
+def sphere_obstacle(widths, centre, radius) -> torch.Tensor:
+    c = [torch.as_tensor(np.cumsum(w) - 0.5 * np.asarray(w), device="cuda") for w in widths]
+    r2 = ((c[0][:, None, None] - centre[0]) ** 2 + (c[1][None, :, None] - centre[1]) ** 2 +
+          (c[2][None, None, :] - centre[2]) ** 2)
+    return (r2 <= radius * radius).to(torch.int32)
+
+
+def make_solver(widths, precision: str = "mixed"):
+    from .simulation_core.fluid import CartesianFluidSolver
+    from .simulation_core.grid import CartesianGrid, FluidDomainSpec
+
+    g = CartesianGrid(bounds_min_m=(0.0, 0.0, 0.0), cell_widths_x_m=tuple(widths[0]),
+                      cell_widths_y_m=tuple(widths[1]), cell_widths_z_m=tuple(widths[2]))
+    spec = FluidDomainSpec(bounds_min_m=g.bounds_min_m, bounds_max_m=g.bounds_max_m, grid_nodes=None,
+                           density_kgm3=1000.0, viscosity_pa_s=1e-3, dt_s=1e-3, cartesian_grid=g)
+    return CartesianFluidSolver(spec, solver_precision=precision)
+
+
+def solve_plain(solver, *, tolerance=1e-8, outlet=False, precision="mixed"):
+    """Pressure of the current divergence from p = 0 (the projection's solve)."""
+    solver.pressure.tensor.zero_()
+    solver._solve_pressure_poisson_fv_cg(iterations=500, rhs_scale=float(np.float32(solver.rho / solver.dt)),
+                                         pressure_outlet_zmin=outlet, tolerance=tolerance,
+                                         preconditioner="fv_multigrid", remove_nullspace_mean=not outlet,
+                                         precision=precision, true_residual_at_exit=True)
+    return solver.last_cg_stats
+
+
+def generate_samples(count: int, fine_axis: np.ndarray, *, seed: int = 0, radius=(0.06, 0.2),
+                     kmax=(8.0, 16.0)):
+    """Training pairs for the network on the 2:1-coarsened grid of `fine_axis`^3, exactly as it is used:
+    the pressure is solved on the fine grid (mixed MGPCG to a float64 true residual of 1e-8) for a random
+    sphere obstacle and the divergence of a random Fourier velocity, then features and target are the
+    2x2x2 volume averages.  Returns lists of features [n, 5], targets [shape] and masks (coarse cells with
+    a free child) on the device, and the fine widths."""
+    widths = [fine_axis, fine_axis, fine_axis]
+    solver = make_solver(widths)
+    shape = (len(fine_axis),) * 3
+    rng = np.random.default_rng(seed)
+    rho_over_dt = float(np.float32(solver.rho / solver.dt))
+    feats, targs, masks = [], [], []
+    for s in range(count):
+        r = float(rng.uniform(*radius))
+        centre = rng.uniform(0.5 - 0.2, 0.5 + 0.2, size=3)
+        obst = sphere_obstacle(widths, centre, r)
+        solver.obstacle.tensor.copy_(obst)
+        solver._set_masks()
+        km = float(rng.uniform(*kmax))
+        solver.velocity.tensor.copy_(fourier_velocity(shape, km, int(rng.integers(1 << 30))))
+        solver._divergence_only(False)
+        st = solve_plain(solver)
+        if not st["converged"]:
+            continue
+        f, rms, _ = features(solver.divergence.tensor, obst, widths, coarsen=True)
+        t = target(solver.pressure.tensor, rms, widths, rho_over_dt, coarsen=True)
+        feats.append(f)
+        targs.append(t)
+        masks.append((f[:, 1] < 1.0).to(torch.uint8).contiguous())
+    return feats, targs, masks, widths
+
+
+def train(net: GuessNetwork, feats, targs, masks, *, epochs: int, lr: float = 2e-3, seed: int = 0,
+          log=None) -> list[float]:
+    """Adam on the relative squared error, cosine learning-rate decay; returns per-epoch mean losses."""
+    n = len(feats)
+    rng = np.random.default_rng(seed)
+    total = epochs * n
+    step = 0
+    history = []
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        losses = []
+        for i in order:
+            cur = lr * 0.5 * (1.0 + math.cos(math.pi * step / max(total, 1)))
+            read = (step % 16 == 0)
+            loss = net.train_step(feats[i], targs[i], masks[i], lr=max(cur, lr * 0.02), read_loss=read)
+            if read:
+                losses.append(loss)
+            step += 1
+        history.append(float(np.mean(losses)) if losses else float("nan"))
+        if log:
+            log(f"epoch {ep}: loss {history[-1]:.4f}")
+    return history

@@ -125,6 +125,14 @@ class NeoHookeanMpmState:
                 pass
             self._handle = None
 
+    def set_p2g_mode(self, mode: str) -> None:
+        """'binned' (default): particles bucketed by 8^3-node block, accumulated in LDS; 'atomic': every
+        particle scatters its 27 node contributions with global atomics."""
+        modes = {"binned": 0, "atomic": 1}
+        if mode not in modes:
+            raise ValueError(f"unsupported p2g mode: {mode!r}")
+        _lib.check(self._lib.hfsi_mpm_set_p2g_mode(self._handle, modes[mode]), "hfsi_mpm_set_p2g_mode")
+
     # ------------------------------------------------------------------
     def initialize_box(self, *, particle_counts, box_min_m, box_max_m, density_kgm3: float) -> None:
         """Cell-centred lattice in a box (neo_hookean_mpm.py:144-229)."""

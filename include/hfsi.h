@@ -232,6 +232,15 @@ int hfsi_mpm_create(hfsi_mpm** out, int32_t particle_capacity, const double* bou
                     const int32_t* grid_nodes3, void* stream);
 int hfsi_mpm_destroy(hfsi_mpm* h);
 
+/* P2G scatter: HFSI_MPM_P2G_BINNED (default) buckets particles by 8^3-node
+ * block and accumulates each block in an LDS tile (one global atomic per
+ * touched node value); HFSI_MPM_P2G_ATOMIC scatters every particle's 27 node
+ * contributions with global float atomics.  Same arithmetic per particle;
+ * float summation order differs. */
+#define HFSI_MPM_P2G_BINNED 0
+#define HFSI_MPM_P2G_ATOMIC 1
+int hfsi_mpm_set_p2g_mode(hfsi_mpm* h, int32_t mode);
+
 /* One step: grid clear, P2G with the SVD clamp and neo-Hookean stress, grid
  * update with damping, G2P / APIC / F update, surface update and the report
  * reductions (NeoHookeanMpmState._step_kernel, neo_hookean_mpm.py:695-959).
@@ -256,6 +265,37 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* particles, int32_t coun
 int hfsi_fluid_mixed_levels(const hfsi_fluid* h, int32_t* half_levels);
 int hfsi_fluid_mixed_level_shape(const hfsi_fluid* h, int32_t level, int32_t* shape3);
 int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells);
+
+/* ---- learned pressure initial guess (no reference counterpart) -----------
+ * A 3-level 3D U-Net (3x3x3 convolutions, channels 16 / 32 / 64, two per
+ * level, 2x2x2 average pooling, nearest upsampling with skip concatenation,
+ * ReLU, 1x1 output) in bf16 on MFMA with float32 accumulation, trained with
+ * these entry points.  A handle is built for one grid shape (multiples of 4).
+ * Input: 5 float channels per cell, channels-last [n][5]; output: 1 float per
+ * cell.  Parameters (float32 masters), in order for each of the ten 3x3x3
+ * layers E1A E1B E2A E2B B3A B3B D2A D2B D1A D1B: W[cout][27][cin] (tap =
+ * (dz+1)*9 + (dy+1)*3 + (dx+1), E1A's cin padded 5 -> 16), b[cout]; then the
+ * output layer w[16], b[1]. */
+typedef struct hfsi_unet hfsi_unet;
+int hfsi_unet_create(hfsi_unet** out, int32_t nx, int32_t ny, int32_t nz, int32_t train, uint64_t seed,
+                     void* stream);
+int hfsi_unet_destroy(hfsi_unet* h);
+int64_t hfsi_unet_num_params(const hfsi_unet* h);
+/* host <-> device parameter and gradient copies (synchronise) */
+int hfsi_unet_get_params(hfsi_unet* h, float* host, void* stream);
+int hfsi_unet_set_params(hfsi_unet* h, const float* host, void* stream);
+int hfsi_unet_get_grads(hfsi_unet* h, float* host, void* stream);
+int hfsi_unet_zero_grads(hfsi_unet* h, void* stream);
+/* out[n] = net(in5[n][5]) (device pointers); keeps the activations for backward */
+int hfsi_unet_forward(hfsi_unet* h, const float* in5, float* out, void* stream);
+/* accumulate d(loss)/d(params) for d(loss)/d(out) = dout (after forward; train handles only) */
+int hfsi_unet_backward(hfsi_unet* h, const float* dout, void* stream);
+/* loss = sum_m (out - target)^2 / sum_m target^2 over mask != 0 (all cells if mask is NULL),
+ * dout = its gradient; *loss (host, may be NULL: no synchronisation) */
+int hfsi_unet_loss(hfsi_unet* h, const float* out, const float* target, const uint8_t* mask, float* dout,
+                   double* loss, void* stream);
+int hfsi_unet_adam_step(hfsi_unet* h, float lr, float beta1, float beta2, float eps, float weight_decay,
+                        void* stream);
 
 /* Live kernel timing (no reference counterpart; measurement support):
  * while enabled, HIP events are recorded on the launch stream around every

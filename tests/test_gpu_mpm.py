@@ -21,11 +21,12 @@ MAT = ecoflex_0010_material()
 MU, LAM = MAT.shear_modulus_pa, MAT.lame_lambda_pa
 
 
-def _state(nodes, lo, hi, counts, box_lo, box_hi, capacity=None):
+def _state(nodes, lo, hi, counts, box_lo, box_hi, capacity=None, p2g="binned"):
     from easyfsi_amd.simulation_core.neo_hookean_mpm import NeoHookeanMpmState
 
     P = int(np.prod(counts))
     s = NeoHookeanMpmState(capacity or P, lo, hi, nodes)
+    s.set_p2g_mode(p2g)
     s.initialize_box(particle_counts=counts, box_min_m=box_lo, box_max_m=box_hi, density_kgm3=MAT.density_kgm3)
     return s
 
@@ -86,8 +87,9 @@ def _compare(s, o: M.MpmState, rep, ref: M.StepReport, *, x_scale):
     assert rep.mean_radial_stretch == pytest.approx(v["radial_stretch_sum"] / rc, rel=1e-5)
 
 
-def test_uniform_velocity_step_matches_oracle():  # test_neo_hookean_mpm.py:102-130
-    s = _state((12, 12, 12), (-0.02,) * 3, (0.02,) * 3, (4, 4, 4), (-0.005,) * 3, (0.005,) * 3)
+@pytest.mark.parametrize("p2g", ["binned", "atomic"])
+def test_uniform_velocity_step_matches_oracle(p2g):  # test_neo_hookean_mpm.py:102-130
+    s = _state((12, 12, 12), (-0.02,) * 3, (0.02,) * 3, (4, 4, 4), (-0.005,) * 3, (0.005,) * 3, p2g=p2g)
     s.set_uniform_velocity((0.02, -0.01, 0.03))
     o = _oracle_from(s)
     ref = M.step(o, bounds_min=(-0.02,) * 3, bounds_max=(0.02,) * 3, grid_nodes=(12, 12, 12), dt=1e-5, mu=MU,
@@ -110,10 +112,10 @@ def test_non_cubic_grid_spacing_matches_oracle():  # test_neo_hookean_mpm.py:132
     assert rep.grid_spacing_m == pytest.approx(((0.02 / 12), (0.02 / 12), (0.08 / 12)))
 
 
-def _loaded_state(seed):
+def _loaded_state(seed, p2g="binned"):
     rng = np.random.default_rng(seed)
     lo, hi, nodes = (-0.03, -0.025, -0.02), (0.03, 0.025, 0.04), (20, 18, 24)
-    s = _state(nodes, lo, hi, (7, 6, 5), (-0.01, -0.008, -0.005), (0.012, 0.009, 0.02))
+    s = _state(nodes, lo, hi, (7, 6, 5), (-0.01, -0.008, -0.005), (0.012, 0.009, 0.02), p2g=p2g)
     n = s.particle_count
     F = np.eye(3)[None] + 0.06 * rng.standard_normal((n, 3, 3))
     s.F.tensor[:n] = _t(F)
@@ -140,9 +142,9 @@ def _t(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dt)
 
 
-@pytest.mark.parametrize("damping", [1.0, 0.98])
-def test_deformed_loaded_state_matches_oracle_per_step(damping):
-    s, lo, hi, nodes = _loaded_state(7)
+@pytest.mark.parametrize("damping,p2g", [(1.0, "binned"), (0.98, "binned"), (1.0, "atomic")])
+def test_deformed_loaded_state_matches_oracle_per_step(damping, p2g):
+    s, lo, hi, nodes = _loaded_state(7, p2g)
     for _ in range(3):
         o = _oracle_from(s)
         ref = M.step(o, bounds_min=lo, bounds_max=hi, grid_nodes=nodes, dt=2e-5, mu=MU, lam=LAM, damping=damping,
@@ -215,6 +217,34 @@ def test_step_without_particles_raises():
     s = NeoHookeanMpmState(8, (0.0,) * 3, (0.01,) * 3, (8, 8, 8))
     with pytest.raises(ValueError, match="initialize particles before stepping"):
         s.step(dt_s=1e-5, mu_pa=MU, lambda_pa=LAM, primary_region_id=1, secondary_region_id=2)
+
+
+def test_binned_and_atomic_scatter_agree_at_scale():
+    """64^3 particles with a rotation on a 96^3 grid: the LDS-binned P2G and the global-atomic P2G give the
+    same grid (float summation order only) and the same particle state."""
+    out = {}
+    for mode in ("binned", "atomic"):
+        s = _state((96, 96, 96), (-0.05,) * 3, (0.05,) * 3, (64, 64, 64), (-0.02,) * 3, (0.02,) * 3, p2g=mode)
+        x = s.x.tensor[: s.particle_count]
+        v = torch_zeros_like(x)
+        v[:, 0] = -3.0 * x[:, 1]
+        v[:, 1] = 3.0 * x[:, 0]
+        s.v.tensor[: s.particle_count] = v
+        rep = s.step(dt_s=1e-5, mu_pa=MU, lambda_pa=LAM, primary_region_id=1, secondary_region_id=2)
+        out[mode] = (s.grid_mass_kg.to_numpy(), s.grid_velocity_mps.to_numpy(), s.v.to_numpy(), s.C.to_numpy(), rep)
+    (gm_b, gv_b, v_b, c_b, r_b), (gm_a, gv_a, v_a, c_a, r_a) = out["binned"], out["atomic"]
+    _close("grid mass", gm_b, gm_a, 1e-5)
+    _close("grid velocity", gv_b, gv_a, 1e-4)
+    _close("v", v_b, v_a, 1e-4)
+    _close("C", c_b, c_a, 1e-4)
+    assert r_b.active_grid_nodes == r_a.active_grid_nodes
+    assert r_b.transfer_relative_error < 2e-5 and r_a.transfer_relative_error < 2e-5
+
+
+def torch_zeros_like(t):
+    import torch
+
+    return torch.zeros_like(t)
 
 
 def test_large_block_conserves_mass_and_momentum():
