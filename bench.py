@@ -104,7 +104,7 @@ def make_rhs(solver, count: int, seed0: int):
 
 
 VARIANTS = {
-    "mixed": dict(precision="mixed", true_residual=1, check=10, monitor_lag=2, dtype="f32", timer="mix_rb",
+    "mixed": dict(precision="mixed", true_residual=1, check=10, monitor_lag=1, dtype="f32", timer="mix_rb",
                   roofline_kernel="mixed V-cycle level-0 red-black colour pass (k_mix_rb<half,float>)",
                   # colour pass: e of the other colour read at full-line granularity (2 B/cell, fp16),
                   # r and flags of the pass colour (2 + 0.5 B/cell), e of the pass colour written (1 B/cell)

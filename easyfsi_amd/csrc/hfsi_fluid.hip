@@ -129,6 +129,20 @@ struct FinMeanR {  // t = {<r,r>, sum Vr}
     s->rr = t[0];
   }
 };
+// t = {<r,r>_V, sum V r} of the true residual: its norm after the mean
+// projection (sum V (r - m)^2 = <r,r>_V - (sum V r)^2 / V) into a pinned host
+// slot, the per-iteration convergence monitor of the mixed solve.
+struct FinTrueMonitor {
+  Scalars* s;
+  int remove_mean;
+  double* slot;
+  __device__ void operator()(const double* t) const {
+    const double V = s->free_vol;
+    double rr = t[0];
+    if (remove_mean && V > 0.0) rr -= t[1] * t[1] / V;
+    *slot = rr > 0.0 ? rr : 0.0;
+  }
+};
 struct FinRR {
   Scalars* s;
   __device__ void operator()(const double* t) const { s->rr = t[0]; }
@@ -307,11 +321,7 @@ constexpr int kMonRing = 8;  // slots of the convergence-monitor ring (lag < kMo
 struct FinVScale {
   Scalars* s;
   double kappa;
-  double* monitor = nullptr;  // pinned host slot receiving <r,r>_V (convergence monitor), or null
-  __device__ void operator()(const double* t) const {
-    set(t[0]);
-    if (monitor) *monitor = t[0];
-  }
+  __device__ void operator()(const double* t) const { set(t[0]); }
   __device__ void set(double rr) const {
     const double V = s->free_vol > 0.0 ? s->free_vol : 1.0;
     const double rms = sqrt(fmax(rr, 0.0) / V);
@@ -666,7 +676,8 @@ __global__ void k_wall_clamp(uint32_t nx, uint32_t ny, uint32_t nz, float* u, in
 }
 
 // Float64 true residual r = b - A x on non-obstacle rows (axpby of
-// fluid.py:4518-4531), stored at TR, with {<r,r>, sum V r} in float64.
+// fluid.py:4518-4531), stored at TR (not stored when r is null), with
+// {<r,r>, sum V r} in float64.
 template <typename TR>
 __global__ void __launch_bounds__(kBlock) k_true_residual(OpGeo<double> g, const double* x, const double* b, TR* r,
                                                            double* partials) {
@@ -679,7 +690,7 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(OpGeo<double> g, const
     uint8_t f = g.flags[c];
     double v = 0.0;
     if (!(f & kObst)) v = b[c] - apply_cell(g, c, i, j, k, P);
-    r[c] = (TR)v;
+    if (r) r[c] = (TR)v;
     if (!(f & kNotFree)) {
       double V = volume64(g, i, j, k);
       acc[0] += v * v * V;
@@ -1072,6 +1083,7 @@ struct hfsi_fluid {
   int mix_tail_first = 0;
   double mix_kappa = 1.0;
   uint32_t mix_half_min_cells = 1u << 20;  // levels with at least this many cells store fp16
+  bool mix_row_kernels = false;            // 8-cell row kernels where nz % 8 == 0
   hfsi::MixLevel ml[hfsi::kMixMaxLevels];
   void* mix_tail_args = nullptr;  // MixTailArgs (device)
   int mix_tail_outlet = -1;       // outlet switch the tail arguments were written for
@@ -1557,18 +1569,33 @@ static int write_mix_tail_args(hfsi_fluid* h, int outlet, hipStream_t st) {
   return HFSI_OK;
 }
 
+// Row kernels (8 cells of one row per thread, vector loads) on levels whose
+// rows split into 8-cell groups; the flat kernels elsewhere.
+static inline bool mix_rows(const hfsi_fluid* h, const MixLevel& m) { return h->mix_row_kernels && m.nz % 8 == 0; }
+
 template <typename TE, typename TR>
 static void mix_rb_pass(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int color,
                         bool from_zero, hipStream_t st) {
   const MixLevel& m = h->ml[l];
   const uint32_t npairs = m.nx * m.ny * ((m.nz + 1) / 2);
+  const uint32_t nrows8 = m.nx * m.ny * (m.nz / 8);
+  const bool rows = mix_rows(h, m);
   OpGeo<float> g = mix_geo(h, l, outlet);
   if (from_zero) {
-    hipLaunchKernelGGL((k_mix_rb<TE, TR, true>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs, color);
+    if (rows)
+      hipLaunchKernelGGL((k_mix_rows<TE, TR, 1>), dim3(blocks_for(nrows8)), dim3(kBlock), 0, st, g, e, rhs, rs, color,
+                         nullptr);
+    else
+      hipLaunchKernelGGL((k_mix_rb<TE, TR, true>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs,
+                         color);
   } else {
     if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
-    hipLaunchKernelGGL((k_mix_rb<TE, TR, false>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs,
-                       color);
+    if (rows)
+      hipLaunchKernelGGL((k_mix_rows<TE, TR, 0>), dim3(blocks_for(nrows8)), dim3(kBlock), 0, st, g, e, rhs, rs, color,
+                         nullptr);
+    else
+      hipLaunchKernelGGL((k_mix_rb<TE, TR, false>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs,
+                         color);
     if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
   }
 }
@@ -1592,8 +1619,12 @@ static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TE* e, const TR
   const MixLevel& f = h->ml[l];
   const MixLevel& c = h->ml[l + 1];
   OpGeo<float> g = mix_geo(h, l, outlet);
-  hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, e, rhs, rs,
-                     (TE*)f.res);
+  if (mix_rows(h, f))
+    hipLaunchKernelGGL((k_mix_rows<TE, TR, 2>), dim3(blocks_for(f.nx * f.ny * (f.nz / 8))), dim3(kBlock), 0, st, g,
+                       const_cast<TE*>(e), rhs, rs, 0, (TE*)f.res);
+  else
+    hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, e, rhs, rs,
+                       (TE*)f.res);
   hipLaunchKernelGGL((k_mix_restrict_res<TE, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, g, (const TE*)f.res,
                      mix_xfer(h, l), c.nx, c.ny, c.nz, (const uint8_t*)c.flags, (TC*)c.rhs);
 }
@@ -1601,9 +1632,14 @@ static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TE* e, const TR
 template <typename TF, typename TCo>
 static void mix_prolong(hfsi_fluid* h, int l, int outlet, TF* ef, hipStream_t st) {
   const MixLevel& c = h->ml[l + 1];
-  hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(h->ml[l].n)), dim3(kBlock), 0, st,
-                     mix_geo(h, l, outlet), ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags,
-                     (const TCo*)c.e);
+  const MixLevel& f = h->ml[l];
+  if (mix_rows(h, f))
+    hipLaunchKernelGGL((k_mix_prolong_rows<TF, TCo>), dim3(blocks_for(f.nx * f.ny * (f.nz / 8))), dim3(kBlock), 0, st,
+                       mix_geo(h, l, outlet), ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags,
+                       (const TCo*)c.e);
+  else
+    hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet),
+                       ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags, (const TCo*)c.e);
 }
 
 // Down-leg of level l (smoothing + restriction), with the storage types of l
@@ -1795,6 +1831,9 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   const int pre = light ? 1 : 2, coarse = light ? 12 : 24, post = light ? 1 : 2;
   // mixed V-cycle: float32 CG vectors, no velocity-Dirichlet or interface terms
   bool use_mix = mixed && use_mg && std::is_same<TV, float>::value && h->ML >= 1 && !h->has_vd && !h->has_diag;
+  // RB-GS sweeps on the coarsest mixed level (<= 3 cells on an axis): 4, 8 and 24
+  // give the same iteration counts
+  const int mix_coarse = light ? 4 : 8;
   const double kappa = h->mix_kappa;
   const int lag = std::min(std::max(0, o.monitor_lag), kMonRing - 1);
   auto mix_scale = [&]() -> int {
@@ -1806,8 +1845,8 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   auto precond_and_rz = [&](bool project) -> int {
     if (use_mix) {
       HFSI_TRY(mix_scale());
-      HFSI_TRY((mix_precondition<3>(h, (const float*)v.r, (const float*)v.q, (float*)v.z, pre, coarse, post, outlet,
-                                    FinDots3{sc, project ? rm : 0}, st)));
+      HFSI_TRY((mix_precondition<3>(h, (const float*)v.r, (const float*)v.q, (float*)v.z, pre, mix_coarse, post,
+                                    outlet, FinDots3{sc, project ? rm : 0}, st)));
       S.vcycles += 1;
       if (project && rm) S.mean_projection_count += 1;
       return HFSI_OK;
@@ -1836,37 +1875,53 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   }
   HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 1}, st));
 
+  // Convergence monitor (mixed solve, monitor_lag = L > 0): after each update of
+  // x the float64 true residual norm of x (mean-free, not stored) goes to a pinned
+  // ring slot.  At the end of iteration it -- its V-cycle already queued, so the
+  // device stays busy -- the host reads the value of iteration it - L + 1: with
+  // L = 1 the solve stops at the first iterate whose true residual meets the
+  // tolerance; with L > 1 a hit forces a check at the next iteration.
   const bool monitor = use_mix && lag > 0 && h->mon_ring != nullptr;
+  bool monitor_pending = false;
   int it = 0;
   while (it < max_it) {
     ++it;
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
-    if (march_ok(h, 0))
+    if (use_mix) {
+      // q = A d with float64 arithmetic on the float32 d: for smooth d the stencil
+      // sum den d - sum c d_nb cancels to a small fraction of den d, and float32
+      // arithmetic would leave q -- and through r -= alpha q the recursive residual --
+      // with errors that slow the convergence of the true residual (256^3 graded
+      // sphere case: 15 iterations to 1e-8 against 11 with r recomputed each time)
+      if (march_ok(h, 0))
+        launch_march<double, TV, TV, TV, TV, kModeApply, true>(h, 0, outlet, v.d, (const TV*)nullptr, 1.0, v.q, 0.0,
+                                                               nullptr, 0.0, (const TV*)nullptr, FinDad{sc}, st);
+      else
+        hipLaunchKernelGGL((k_spmv_dot<double, TV, TV>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, v.d, v.q,
+                           h->partials);
+    } else if (march_ok(h, 0)) {
       launch_march<C, TV, TV, TV, TV, kModeApply, true>(h, 0, outlet, v.d, (const TV*)nullptr, (C)1, v.q, (C)0,
                                                         nullptr, (C)0, (const TV*)nullptr, FinDad{sc}, st);
-    else
+    } else {
       hipLaunchKernelGGL((k_spmv_dot<C, TV, TV>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.d, v.q,
                          h->partials);
+    }
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
-    if (!march_ok(h, 0)) finish_sums<1>(h, blocks_for(g.n), FinDad{sc}, st);
+    if (!march_ok(h, 0)) finish_sums<1>(h, blocks_for(use_mix ? g64.n : g.n), FinDad{sc}, st);
     HFSI_LAUNCH_CHECK();
-    bool monitor_hit = false;
+    const bool monitor_hit = monitor_pending;
+    monitor_pending = false;
     if (use_mix) {
       timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
       hipLaunchKernelGGL(k_cg_update_rr, dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, (const float*)v.d,
                          (float*)v.r, (const float*)v.q, sc, h->partials);
       timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
-      double* slot = monitor ? h->mon_ring + (it % kMonRing) : nullptr;
-      finish_sums<1>(h, blocks_for(g64.n), FinVScale{sc, kappa, slot}, st);
+      finish_sums<1>(h, blocks_for(g64.n), FinVScale{sc, kappa}, st);
       if (monitor) {
+        hipLaunchKernelGGL((k_true_residual<float>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, h->b,
+                           (float*)nullptr, h->partials);
+        finish_sums<2>(h, blocks_for(g64.n), FinTrueMonitor{sc, rm, h->mon_ring + (it % kMonRing)}, st);
         HFSI_CHECK(hipEventRecord(h->mon_events[it % kMonRing], st));
-        if (it > lag) {
-          // the value of iteration it - lag: that iteration's work is (nearly) done, the
-          // device keeps the later iterations queued while the host waits for it
-          HFSI_CHECK(hipEventSynchronize(h->mon_events[(it - lag) % kMonRing]));
-          const double rr = h->mon_ring[(it - lag) % kMonRing];
-          monitor_hit = std::sqrt(std::max(rr, 0.0)) <= tol * b_norm;
-        }
       }
       HFSI_LAUNCH_CHECK();
     } else {
@@ -1938,8 +1993,8 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
     }
     FusedDots fd{v.q, sc, rm, use_mg ? 1 : 0, false};
     if (use_mix) {
-      HFSI_TRY((mix_precondition<5>(h, (const float*)v.r, (const float*)v.q, (float*)v.z, pre, coarse, post, outlet,
-                                    FinDots5{sc, rm, 1}, st)));
+      HFSI_TRY((mix_precondition<5>(h, (const float*)v.r, (const float*)v.q, (float*)v.z, pre, mix_coarse, post,
+                                    outlet, FinDots5{sc, rm, 1}, st)));
       fd.done = true;
     } else {
       HFSI_TRY((precondition<C, TV>(h, use_mg, pre, coarse, post, outlet, st, &fd)));
@@ -1949,6 +2004,20 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
     if (!fd.done)
       HFSI_TRY(launch_sums<5>(h, g64, OpDots5<TV, TV, TV>{v.r, v.z, v.q}, FinDots5{sc, rm, use_mg ? 1 : 0}, st));
     HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 0}, st));
+    if (monitor && it >= lag) {
+      const int m = it - lag + 1;
+      HFSI_CHECK(hipEventSynchronize(h->mon_events[m % kMonRing]));
+      const double rel_m = std::sqrt(std::max(h->mon_ring[m % kMonRing], 0.0)) / b_norm;
+      if (rel_m <= tol) {
+        if (m == it) {
+          // x is the iterate the monitor measured; the queued V-cycle does not touch it
+          S.relative_residual = rel_m;
+          S.converged = 1;
+          break;
+        }
+        monitor_pending = true;
+      }
+    }
   }
   HFSI_TRY(sync_scalars(h, st));
   S.breakdown_code = h->host_scal->code;
@@ -2098,8 +2167,8 @@ static int build_mix_hierarchy(hfsi_fluid* h, const std::vector<float> wf0[3], c
     double hmax = 0.0;
     for (int a = 0; a < 3; ++a)
       for (double w : F.w[a]) hmax = std::max(hmax, w);
-    // once every width is within 2.5x of the nominal width, plain 2:1 coarsening
-    const bool full = hmax <= 2.5 * hmin;
+    // once every width is within kMixFullRatio of the nominal width, plain 2:1 coarsening
+    const bool full = hmax <= kMixFullRatio * hmin;
     HostLevel Cn;
     bool any = false;
     for (int a = 0; a < 3; ++a) {
@@ -2108,7 +2177,7 @@ static int build_mix_hierarchy(hfsi_fluid* h, const std::vector<float> wf0[3], c
       for (size_t i = 0; i < n;) {
         const int32_t ci = (int32_t)Cn.w[a].size();
         Cn.first[a].push_back((int32_t)i);
-        if (i + 1 < n && (full || F.w[a][i] + F.w[a][i + 1] <= 3.0 * hmin)) {
+        if (i + 1 < n && (full || F.w[a][i] + F.w[a][i + 1] <= kMixMergeRatio * hmin)) {
           Cn.w[a].push_back(F.w[a][i] + F.w[a][i + 1]);
           Cn.cnt[a].push_back(2);
           F.par[a][i] = F.par[a][i + 1] = ci;
@@ -2552,6 +2621,15 @@ int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells) {
     return HFSI_ERR_ARG;
   }
   h->mix_half_min_cells = (uint32_t)std::min<int64_t>(min_cells, (int64_t)UINT32_MAX);
+  return HFSI_OK;
+}
+
+int hfsi_fluid_mixed_set_row_kernels(hfsi_fluid* h, int32_t enable) {
+  if (!h) {
+    set_error("hfsi_fluid_mixed_set_row_kernels: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  h->mix_row_kernels = enable != 0;
   return HFSI_OK;
 }
 

@@ -424,7 +424,7 @@ class CartesianFluidSolver:
         opt.restart_interval = 256 if pressure_interface_matrix_active else 512
         opt.true_residual = 1 if prec != "ref64" else 0
         opt.true_residual_at_exit = 1 if true_residual_at_exit else 0
-        opt.monitor_lag = 2 if prec == "mixed" else 0
+        opt.monitor_lag = 1 if prec == "mixed" else 0
         opt.tolerance = max(0.0, float(tolerance))
         opt.rhs_scale = float(rhs_scale)
         stats = _lib.CgStats()

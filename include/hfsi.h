@@ -84,10 +84,12 @@ typedef struct hfsi_cg_options {
   int32_t restart_interval;   /* exact-residual restart interval (reference: 512)         */
   int32_t true_residual;      /* 1: every check recomputes r = b - A x in float64        */
   int32_t true_residual_at_exit;
-  int32_t monitor_lag;        /* MIXED only, > 0: the host reads the recursive residual norm of
-                                 every iteration this many iterations behind the device (no
-                                 stall) and checks the true residual as soon as it reaches the
-                                 tolerance; 0: checks every check_interval iterations only  */
+  int32_t monitor_lag;        /* MIXED only, L > 0: every iteration computes the float64 true
+                                 residual norm of x (not stored) into a host-mapped ring; at
+                                 the end of iteration it, with its V-cycle queued, the host
+                                 reads iteration it - L + 1.  L = 1 stops at the first iterate
+                                 that meets the tolerance; L > 1 checks one iteration after a
+                                 hit.  0: checks every check_interval iterations only        */
   double tolerance;           /* relative, on ||r||_V / ||b||_V                            */
   double rhs_scale;           /* rho / dt                                                 */
 } hfsi_cg_options;
@@ -265,6 +267,11 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* particles, int32_t coun
 int hfsi_fluid_mixed_levels(const hfsi_fluid* h, int32_t* half_levels);
 int hfsi_fluid_mixed_level_shape(const hfsi_fluid* h, int32_t level, int32_t* shape3);
 int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells);
+/* Row kernels (8 cells of one row per thread, 16-byte loads) for the smoother,
+ * residual and prolongation of levels with nz % 8 == 0 (default off: slower
+ * than the cell-per-thread kernels on MI355X); both compute the same
+ * arithmetic. */
+int hfsi_fluid_mixed_set_row_kernels(hfsi_fluid* h, int32_t enable);
 
 /* ---- learned pressure initial guess (no reference counterpart) -----------
  * A 3-level 3D U-Net (3x3x3 convolutions, channels 16 / 32 / 64, two per

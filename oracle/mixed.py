@@ -11,7 +11,7 @@ cell by cell:
 
   hierarchy   with T_l = 2^l h_min (h_min: smallest level-0 width over all
               axes), along each axis cells i, i+1 of level l merge when
-              w_i + w_{i+1} <= 3 T_l, or always once max width <= 2.5 T_l;
+              w_i + w_{i+1} <= 2.5 T_l, or always once max width <= 2.5 T_l;
               stop when an axis has <= 3 cells or nothing merges.  Level
               operators are the reference's FV operator (fluid.py:4079-4221)
               rediscretised on the merged widths (level 0: the level-0
@@ -46,6 +46,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import fluid as F
+
+MERGE_RATIO = 2.5   # merge a pair while its width <= MERGE_RATIO x the nominal width T_l
+FULL_RATIO = 2.5    # plain 2:1 once every width <= FULL_RATIO x T_l
 
 
 @dataclass
@@ -105,7 +108,7 @@ def build_hierarchy(widths, obstacle) -> list[MixLevel]:
             break
         hmin = min(float(w.min()) for w in levels[0].w) * 2.0 ** (len(levels) - 1)
         hmax = float(np.concatenate(f.w).max())
-        full = hmax <= 2.5 * hmin
+        full = hmax <= FULL_RATIO * hmin
         nw, par, first, cnt = [], [], [], []
         any_merge = False
         for a in range(3):
@@ -116,7 +119,7 @@ def build_hierarchy(widths, obstacle) -> list[MixLevel]:
             i = 0
             while i < n:
                 fi.append(i)
-                if i + 1 < n and (full or wa[i] + wa[i + 1] <= 3.0 * hmin):
+                if i + 1 < n and (full or wa[i] + wa[i + 1] <= MERGE_RATIO * hmin):
                     cw.append(wa[i] + wa[i + 1])
                     ci.append(2)
                     pa[i] = pa[i + 1] = len(cw) - 1

@@ -178,14 +178,17 @@ def test_deformation_clamp_and_reflection_match_oracle():  # test_neo_hookean_mp
     assert (np.linalg.det(s.F.to_numpy()[:n].astype(np.float64)) > 0).all()
 
 
-def test_all_particles_outside_grid_raises():  # test_neo_hookean_mpm.py:225-246
-    s = _state((8, 8, 8), (0.0,) * 3, (0.01,) * 3, (2, 2, 2), (0.05,) * 3, (0.06,) * 3)
+@pytest.mark.parametrize("p2g", ["binned", "atomic"])
+def test_all_particles_outside_grid_raises(p2g):  # test_neo_hookean_mpm.py:225-246
+    s = _state((8, 8, 8), (0.0,) * 3, (0.01,) * 3, (2, 2, 2), (0.05,) * 3, (0.06,) * 3, p2g=p2g)
     with pytest.raises(RuntimeError, match="outside the background grid"):
         s.step(dt_s=1e-5, mu_pa=MU, lambda_pa=LAM, primary_region_id=1, secondary_region_id=2)
+        print(f"no raise ({p2g}): particle_count {s.particle_count}, report counts {s._report[28:34]}", flush=True)
 
 
-def test_nan_positions_count_as_out_of_bounds():
-    s = _state((8, 8, 8), (-0.01,) * 3, (0.01,) * 3, (2, 2, 2), (-0.002,) * 3, (0.002,) * 3)
+@pytest.mark.parametrize("p2g", ["binned", "atomic"])
+def test_nan_positions_count_as_out_of_bounds(p2g):
+    s = _state((8, 8, 8), (-0.01,) * 3, (0.01,) * 3, (2, 2, 2), (-0.002,) * 3, (0.002,) * 3, p2g=p2g)
     s.x.tensor[0, 1] = float("nan")
     rep = s.step(dt_s=1e-5, mu_pa=MU, lambda_pa=LAM, primary_region_id=1, secondary_region_id=2)
     assert rep.grid_out_of_bounds_particle_count == 1
