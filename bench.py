@@ -1,29 +1,37 @@
 #!/usr/bin/env python3
 """Projection-solve throughput on one MI355X.
 
-Workload (GPU scale): 256^3 graded MAC grid on the
-unit cube -- a core of 128 equal cells of width h and 64 cells on each side of
-width h*1.04^k (k = 1..64), h = 1/(128 + 2*sum 1.04^k) -- with a sphere
-obstacle of radius 0.06 at the centre; right-hand sides are the divergence of
-seeded random Fourier velocity fields (|k| <= 16 in cell-index space, one
-field per step).  A step is one pressure solve from x0 = 0 to a float64 TRUE
-volume-weighted relative residual ||b - A x||_V / ||b||_V <= 1e-8.
+Workload (GPU scale): 256^3 graded MAC grid on the unit cube -- a core of 128
+equal cells of width h and 64 cells on each side of width h*1.04^k
+(k = 1..64), h = 1/(128 + 2*sum 1.04^k) -- with a sphere obstacle of radius
+0.06 at the centre; right-hand sides are the divergence of seeded random
+Fourier velocity fields (|k| <= 16 in cell-index space, one field per step).
+A step is one pressure solve from its start to a float64 TRUE volume-weighted
+relative residual ||b - A x||_V / ||b||_V <= 1e-8.
 
 Variants (all on the GPU, same grid and right-hand sides):
-  mixed      : red-black Gauss-Seidel V-cycle on a semi-coarsened hierarchy
-               with fp16-stored large levels and float32 arithmetic, inside
-               flexible CG with float32 vectors and float64 solution, dots
-               and true-residual replacement every 10 iterations.
-  plain_fp32 : the reference's V-cycle (Jacobi 2/24/2, x0.7 prolongation)
-               in float32 inside flexible CG with float32 vectors, float64
-               solution, dots and true-residual replacement every 16 its.
-  ref64      : the same algorithm at the reference's precision (float64
-               vectors, float32 coarse levels), recursive-residual test.
-The headline value is the fastest method variant (mixed when run); the
-speed-up over plain_fp32 is reported beside it.
+  mixed         : red-black Gauss-Seidel V-cycle on a semi-coarsened
+                  hierarchy with fp16-stored large levels and float32
+                  arithmetic (fused two-colour sweeps), inside flexible CG
+                  with float32 vectors, float64 solution, SpMV arithmetic and
+                  dots, true-residual replacement every 10 iterations and a
+                  float64 true-residual convergence monitor every iteration;
+                  start p = 0.
+  learned_mixed : the same solve started from the learned U-Net guess (the
+                  guess is part of the timed solve).
+  plain_fp32    : the reference's V-cycle (Jacobi 2/24/2, x0.7 prolongation)
+                  in float32 inside flexible CG with float32 vectors, float64
+                  solution, dots and true-residual replacement every 16 its.
+  ref64         : the same algorithm at the reference's precision (float64
+                  vectors, float32 coarse levels), recursive-residual test.
+The headline value is the faster of learned_mixed and mixed; the speed-up
+over plain_fp32 is reported beside it.  Also reported: a time-coherent
+sequence (moving sphere, turning velocity) solved from p = 0, from the
+previous step's pressure and from the learned guess; the neo-Hookean MPM step
+(1M particles); the CPU baseline (C + OpenMP restatement of the plain
+algorithm on the host cores).
 
-Prints one JSON line (rank 0).  --gpus above 1 is refused: this build
-measures one GPU.
+Prints one JSON line.  --gpus above 1 is refused: this build measures one GPU.
 """
 
 from __future__ import annotations
@@ -105,10 +113,15 @@ def make_rhs(solver, count: int, seed0: int):
 
 VARIANTS = {
     "mixed": dict(precision="mixed", true_residual=1, check=10, monitor_lag=1, dtype="f32", timer="mix_rb",
-                  roofline_kernel="mixed V-cycle level-0 red-black colour pass (k_mix_rb<half,float>)",
-                  # colour pass: e of the other colour read at full-line granularity (2 B/cell, fp16),
-                  # r and flags of the pass colour (2 + 0.5 B/cell), e of the pass colour written (1 B/cell)
-                  bytes_per_cell=5.5),
+                  roofline_kernel="mixed V-cycle level-0 fused red-black sweep (k_mix_sweep<half,float>)",
+                  # one sweep (both colours): e read and written once (2 + 2 B/cell, fp16), the CG residual
+                  # r as right-hand side (4 B/cell, float32), flags (1 B/cell)
+                  bytes_per_cell=9.0),
+    # the learned guess (U-Net on the 2:1-coarsened grid, energy-optimal scaling) as the start of the mixed
+    # solve; the guess (features, network, interpolation, scaling) is inside the timed solve
+    "learned_mixed": dict(precision="mixed", true_residual=1, check=10, monitor_lag=1, dtype="f32", timer="mix_rb",
+                          roofline_kernel="mixed V-cycle level-0 fused red-black sweep (k_mix_sweep<half,float>)",
+                          bytes_per_cell=9.0, learned=True),
     "plain_fp32": dict(precision="fp32", true_residual=1, check=16, dtype="f32", timer="l0_sweep",
                        roofline_kernel="level-0 weighted-Jacobi sweep (k_march, float32)",
                        bytes_per_cell=13.0),  # p + rhs read, p write (4 B each), 1 B flags
@@ -117,7 +130,7 @@ VARIANTS = {
 }
 
 
-def run_variant(name, solver, divs, steps, warmup):
+def run_variant(name, solver, divs, steps, warmup, guess=None):
     import ctypes
 
     import torch
@@ -125,6 +138,8 @@ def run_variant(name, solver, divs, steps, warmup):
     from easyfsi_amd import _lib
 
     cfg = VARIANTS[name]
+    if cfg.get("learned") and guess is None:
+        raise ValueError("the learned variant needs a guess network")
     L = solver._lib
     opt = _lib.CgOptions()
     opt.precision = {"fp32": _lib.HFSI_PREC_FP32, "ref64": _lib.HFSI_PREC_REF64,
@@ -148,7 +163,11 @@ def run_variant(name, solver, divs, steps, warmup):
     p = solver.pressure
 
     def solve(div):
-        p.tensor.zero_()
+        if cfg.get("learned"):
+            solver.divergence.tensor.copy_(div)
+            guess.apply(solver)  # writes the scaled guess into the pressure: the solve's start
+        else:
+            p.tensor.zero_()
         _lib.check(L.hfsi_fluid_solve_cg(solver._handle, div.data_ptr(), src, None, p.ptr, ctypes.byref(opt),
                                          ctypes.byref(stats), stream), "hfsi_fluid_solve_cg")
         return stats.as_dict()
@@ -188,6 +207,61 @@ def run_variant(name, solver, divs, steps, warmup):
         "timer_ms_total": float(ms.value),
         "dtype": cfg["dtype"],
     }
+
+
+def run_sequence(solver, w, steps: int, guess=None):
+    """Time-coherent projections (the C4 comparison): a sphere r = 0.06 moving at 0.25 core cells per step
+    through a velocity field that turns between two seeded |k| <= 16 fields by 0.05 rad per step.  Each
+    right-hand side is solved by the mixed MGPCG from p = 0, from the previous step's pressure and from
+    the learned guess; medians of iterations and wall time (the guess included)."""
+    import torch
+
+    from easyfsi_amd import learned as Lm
+
+    n = solver.nx
+    ua = fourier_velocity((n, n, n), 16, 777)
+    ub = fourier_velocity((n, n, n), 16, 778)
+    h = float(np.min(w))
+    rows = {"zero": [], "previous": [], "learned": []}
+    prev = None
+    for t in range(steps + 1):
+        centre = (0.5 + 0.25 * h * (t - steps / 2), 0.5, 0.5)
+        solver.obstacle.tensor.copy_(Lm.sphere_obstacle([w, w, w], centre, 0.06))
+        solver._set_masks()
+        phi = 0.05 * t
+        solver.velocity.tensor.copy_(math.cos(phi) * ua + math.sin(phi) * ub)
+        solver._divergence_only(False)
+        starts = [("zero", None), ("previous", prev)] + ([("learned", "guess")] if guess is not None else [])
+        for name, x0 in starts:
+            if x0 is None and name == "previous":
+                continue
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name == "zero":
+                solver.pressure.tensor.zero_()
+            elif name == "previous":
+                solver.pressure.tensor.copy_(x0)
+                solver.pressure.tensor.masked_fill_(solver.obstacle.tensor != 0, 0.0)
+            else:
+                guess.apply(solver)
+            solver._solve_pressure_poisson_fv_cg(iterations=500, rhs_scale=float(np.float32(solver.rho / solver.dt)),
+                                                 pressure_outlet_zmin=False, tolerance=1e-8,
+                                                 preconditioner="fv_multigrid", remove_nullspace_mean=True,
+                                                 precision="mixed", true_residual_at_exit=True)
+            torch.cuda.synchronize()
+            st = solver.last_cg_stats
+            if t > 0:  # the first step only seeds the previous pressure
+                rows[name].append((st["iterations"], 1e3 * (time.perf_counter() - t0),
+                                   st["true_relative_residual"]))
+        prev = solver.pressure.tensor.clone()
+    out = {}
+    for name, r in rows.items():
+        if r:
+            out[name] = {"iterations_median": statistics.median(v[0] for v in r),
+                         "ms_median": round(statistics.median(v[1] for v in r), 3),
+                         "true_rel_residual_max": max(v[2] for v in r)}
+    out["steps"] = steps
+    return out
 
 
 def cpu_baseline(solver, w, div, iterations_needed: int, budget_s: float = 15.0):
@@ -281,7 +355,10 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--variants", default="mixed,plain_fp32,ref64")
+    ap.add_argument("--variants", default="mixed,learned_mixed,plain_fp32,ref64")
+    ap.add_argument("--weights", default=None, help="guess network weights (default: the packaged ones)")
+    ap.add_argument("--sequence-steps", type=int, default=10,
+                    help="time-coherent projections for the warm-start comparison (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mpm-steps", type=int, default=20, help="neo-Hookean MPM steps timed beside (0: skip)")
@@ -304,19 +381,38 @@ def main(argv=None) -> int:
     solver, w = make_solver(args.size, "mixed")
     n = solver.nx * solver.ny * solver.nz
     divs = make_rhs(solver, max(args.steps, 1), seed0=1000)
+    guess = None
+    if "learned_mixed" in variants or args.sequence_steps > 0:
+        from easyfsi_amd import learned as Lm
+
+        weights = args.weights or Lm.DEFAULT_WEIGHTS
+        if os.path.exists(weights):
+            guess = Lm.LearnedPressureGuess(solver, weights=weights)
+        else:
+            print(f"[bench] no guess network weights at {weights}: learned variant skipped", file=sys.stderr)
+            variants = [v for v in variants if v != "learned_mixed"]
     results = {}
     for v in variants:
         print(f"[bench] variant {v}: {args.warmup} warmup + {args.steps} timed solves", file=sys.stderr, flush=True)
-        results[v] = run_variant(v, solver, divs, args.steps, args.warmup)
+        results[v] = run_variant(v, solver, divs, args.steps, args.warmup, guess=guess)
         print(f"[bench] {v}: {results[v]['solves_per_s']:.2f} solves/s, "
               f"{results[v]['iterations_median']} its, true res {results[v]['true_rel_residual_max']:.2e}",
               file=sys.stderr, flush=True)
-    head_name = next(v for v in ("mixed", "plain_fp32", "ref64") if v in results)
+    # headline: the faster of learned+mixed and mixed (the guess is optional in the method: it only ever
+    # changes the start of the same solve), else the plain variants
+    cands = [v for v in ("learned_mixed", "mixed") if v in results]
+    head_name = (max(cands, key=lambda v: results[v]["solves_per_s"]) if cands
+                 else next(v for v in ("plain_fp32", "ref64") if v in results))
     head = results[head_name]
     # roofline of the level-0 smoother pass of the headline variant
     bytes_per_cell = VARIANTS[head_name]["bytes_per_cell"]
     avg_s = head["timer_ms_total"] / max(head["timer_launches"], 1) / 1e3
     achieved = bytes_per_cell * n / avg_s / 1e9 if head["timer_launches"] else 0.0
+    sequence = None
+    if args.sequence_steps > 0:
+        print(f"[bench] warm-start comparison: {args.sequence_steps} time-coherent projections", file=sys.stderr,
+              flush=True)
+        sequence = run_sequence(solver, w, args.sequence_steps, guess)
     mpm = None
     if args.mpm_steps > 0:
         print(f"[bench] neo-Hookean MPM: 1M particles, {args.mpm_steps} steps", file=sys.stderr, flush=True)
@@ -366,6 +462,11 @@ def main(argv=None) -> int:
     }
     if mpm is not None:
         line["mpm"] = mpm
+    if sequence is not None:
+        line["warm_start_sequence"] = sequence
+    if "learned_mixed" in results and "mixed" in results:
+        line["learned_vs_mixed"] = round(results["learned_mixed"]["solves_per_s"] / results["mixed"]["solves_per_s"],
+                                         3)
     if "ref64" in results and "plain_fp32" in results:
         line["speedup_plain_fp32_vs_ref64"] = round(results["plain_fp32"]["solves_per_s"]
                                                     / results["ref64"]["solves_per_s"], 3)

@@ -209,6 +209,186 @@ __global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, 
 }
 
 // ---------------------------------------------------------------------------
+// Fused red-black sweep (both colour passes in one pass over HBM), marching
+// along i.  A workgroup owns a tile of kSwTJ x kSwTK cells of the (j, k)
+// plane and a chunk of planes [ia, ib).  It keeps a ring of five planes of
+// the iterate in LDS (storage type TE, halo 2 in j and k; five so that the
+// plane written at a step is none of the four the previous step still reads
+// after its last barrier) and at step i
+//   * writes plane i + 1 (loaded during the step before) into LDS,
+//   * stores plane i - 2 (final after the step before),
+//   * runs the first colour c0 on plane i over the tile plus a halo of 1
+//     (the cells of that colour only read cells of the other colour, still
+//     old: planes i - 1, i, i + 1),
+//   * runs the second colour on plane i - 1 over the tile (its neighbours
+//     are first-colour cells of planes i - 2 .. i, all updated).
+// The first-colour results are rounded to TE in LDS as the two-pass form
+// rounds them in memory, so both forms give the same values.  The sweep is
+// out of place (ein -> eout): a tile's halo reads must see the old iterate
+// even where the neighbouring workgroup has already written.  Every chunk
+// starts two planes early (the first colour on plane ia - 1, planes ia - 2 ..
+// loaded) and recomputes what the chunk before owns.  Requires nz even
+// (2-element loads and stores).
+// ---------------------------------------------------------------------------
+
+constexpr int kSwTJ = 16, kSwTK = 64, kSwHalo = 2;
+constexpr int kSwRows = kSwTJ + 2 * kSwHalo;     // 20
+constexpr int kSwCols = kSwTK + 2 * kSwHalo;     // 68
+constexpr int kSwPlane = kSwRows * kSwCols;      // cells per LDS plane
+constexpr int kSwPairs = kSwPlane / 2;           // 2-element loads per plane
+constexpr int kSwThreads = 256;
+constexpr int kSwLoadIters = (kSwPairs + kSwThreads - 1) / kSwThreads;  // 3
+constexpr int kSwA = (kSwTJ + 2) * ((kSwTK + 2) / 2);                   // first-colour cells (tile + 1)
+constexpr int kSwB = kSwTJ * (kSwTK / 2);                               // second-colour cells (tile)
+
+struct SweepGeo {
+  uint32_t kblocks, jblocks, ichunk, nblocks;
+};
+
+template <typename TE>
+struct Pair;
+template <>
+struct Pair<__half> {
+  using raw = uint32_t;
+  __device__ static inline raw load(const __half* p) { return *reinterpret_cast<const uint32_t*>(p); }
+  __device__ static inline void put(__half* lds, raw v) { *reinterpret_cast<uint32_t*>(lds) = v; }
+  __device__ static inline void store(__half* p, const __half* lds) {
+    *reinterpret_cast<uint32_t*>(p) = *reinterpret_cast<const uint32_t*>(lds);
+  }
+  __device__ static inline raw zero() { return 0u; }
+};
+template <>
+struct Pair<float> {
+  using raw = float2;
+  __device__ static inline raw load(const float* p) { return *reinterpret_cast<const float2*>(p); }
+  __device__ static inline void put(float* lds, raw v) { *reinterpret_cast<float2*>(lds) = v; }
+  __device__ static inline void store(float* p, const float* lds) {
+    *reinterpret_cast<float2*>(p) = *reinterpret_cast<const float2*>(lds);
+  }
+  __device__ static inline raw zero() { return make_float2(0.0f, 0.0f); }
+};
+
+template <typename TE, typename TR, bool FROMZERO>
+__global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const TE* __restrict__ ein,
+                                                           TE* __restrict__ eout, const TR* __restrict__ rhs,
+                                                           const double* rs_dev, int c0, SweepGeo sg) {
+  __shared__ TE E[5][kSwPlane];
+  using P = Pair<TE>;
+  const uint32_t G = gridDim.x;
+  const uint32_t b = blockIdx.x;
+  const uint32_t L = (b % HFSI_NXCD) * (G / HFSI_NXCD) + b / HFSI_NXCD;
+  if (L >= sg.nblocks) return;  // whole workgroup, before any barrier
+  const int kb = (int)(L % sg.kblocks);
+  const int jb = (int)((L / sg.kblocks) % sg.jblocks);
+  const int ib = (int)(L / (sg.kblocks * sg.jblocks));
+  const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+  const int j0 = jb * kSwTJ - kSwHalo, k0 = kb * kSwTK - kSwHalo;  // LDS (0, 0) in grid coordinates
+  const int ia = ib * (int)sg.ichunk, iend = min(nx, ia + (int)sg.ichunk);
+  const float rs = rhs_scale(rs_dev);
+  const int tid = threadIdx.x;
+
+  // 2-element loads of plane p into registers (0 outside the grid)
+  typename P::raw pre[kSwLoadIters];
+  auto fetch = [&](int p) {
+#pragma unroll
+    for (int m = 0; m < kSwLoadIters; ++m) {
+      const int t = tid + m * kSwThreads;
+      pre[m] = P::zero();
+      if (FROMZERO || t >= kSwPairs || p < 0 || p >= nx) continue;
+      const int r = t / (kSwCols / 2), k = k0 + 2 * (t % (kSwCols / 2)), j = j0 + r;
+      if (j < 0 || j >= ny || k < 0 || k >= nz) continue;
+      pre[m] = P::load(ein + ((size_t)p * ny + j) * nz + k);
+    }
+  };
+  auto slot = [](int p) { return ((p % 5) + 5) % 5; };
+  auto put = [&](int p) {
+    TE* pl = E[slot(p)];
+#pragma unroll
+    for (int m = 0; m < kSwLoadIters; ++m) {
+      const int t = tid + m * kSwThreads;
+      if (t < kSwPairs) P::put(pl + 2 * t, pre[m]);
+    }
+  };
+  // one Gauss-Seidel update of cell (i, j, k) from LDS planes (xm, x, xp); v stays 0 on cells that are not free
+  auto update = [&](int i, int j, int k, const TE* xm, const TE* x, const TE* xp, int li) -> float {
+    const uint32_t c = ((uint32_t)i * g.ny + (uint32_t)j) * g.nz + (uint32_t)k;
+    const uint8_t f = g.flags[c];
+    const float r = ldf<TR>(rhs, c);
+    float v = 0.0f;
+    if (!(f & kNotFree)) {
+      float nb = 0.0f, den = 0.0f;
+      if (f & kXm) { const float q = g.cl[0][i]; nb += q * to_c_exact(xm[li]); den += q; }
+      if (f & kXp) { const float q = g.ch[0][i]; nb += q * to_c_exact(xp[li]); den += q; }
+      if (f & kYm) { const float q = g.cl[1][j]; nb += q * to_c_exact(x[li - kSwCols]); den += q; }
+      if (f & kYp) { const float q = g.ch[1][j]; nb += q * to_c_exact(x[li + kSwCols]); den += q; }
+      if (f & kZm) { const float q = g.cl[2][k]; nb += q * to_c_exact(x[li - 1]); den += q; }
+      if (f & kZp) { const float q = g.ch[2][k]; nb += q * to_c_exact(x[li + 1]); den += q; }
+      if (g.outlet && k == 0) den += g.outlet_add;
+      if (den > 0.0f) v = (nb + rs * r) / den;
+    }
+    return v;
+  };
+  auto store_plane = [&](int p) {
+    const TE* pl = E[slot(p)];
+    for (int t = tid; t < kSwTJ * (kSwTK / 2); t += kSwThreads) {
+      const int r = t / (kSwTK / 2), kk = kSwHalo + 2 * (t % (kSwTK / 2));
+      const int j = j0 + kSwHalo + r, k = k0 + kk;
+      if (j >= ny || k >= nz) continue;
+      P::store(eout + ((size_t)p * ny + j) * nz + k, pl + (r + kSwHalo) * kSwCols + kk);
+    }
+  };
+
+  // prologue: planes ia - 2 and ia - 1 into LDS, plane ia into registers
+  fetch(ia - 2);
+  put(ia - 2);
+  fetch(ia - 1);
+  put(ia - 1);
+  fetch(ia);
+  for (int i = ia - 1; i <= iend; ++i) {
+    put(i + 1);
+    if (i + 2 <= iend + 1) fetch(i + 2);
+    __syncthreads();
+    if (i - 2 >= ia) store_plane(i - 2);
+    if (i >= 0 && i < nx) {
+      // first colour on plane i, tile + 1
+      TE* x = E[slot(i)];
+      const TE* xm = E[slot(i - 1)];
+      const TE* xp = E[slot(i + 1)];
+      for (int t = tid; t < kSwA; t += kSwThreads) {
+        const int r = 1 + t / ((kSwTK + 2) / 2);
+        const int j = j0 + r;
+        const int par = (c0 + i + j) & 1;  // k parity of the colour in this row (k0 is even)
+        const int kk = 2 * (t % ((kSwTK + 2) / 2)) + (par ? 1 : 2);
+        const int k = k0 + kk;
+        if (j < 0 || j >= ny || k < 0 || k >= nz) continue;
+        const int li = r * kSwCols + kk;
+        x[li] = (TE)update(i, j, k, xm, x, xp, li);
+      }
+    }
+    __syncthreads();
+    const int ib2 = i - 1;
+    if (ib2 >= ia && ib2 < iend) {
+      // second colour on plane i - 1, tile
+      TE* x = E[slot(ib2)];
+      const TE* xm = E[slot(ib2 - 1)];
+      const TE* xp = E[slot(ib2 + 1)];
+      for (int t = tid; t < kSwB; t += kSwThreads) {
+        const int r = kSwHalo + t / (kSwTK / 2);
+        const int j = j0 + r;
+        const int par = (1 - c0 + ib2 + j) & 1;
+        const int kk = kSwHalo + 2 * (t % (kSwTK / 2)) + par;
+        const int k = k0 + kk;
+        if (j >= ny || k >= nz) continue;
+        const int li = r * kSwCols + kk;
+        x[li] = (TE)update(ib2, j, k, xm, x, xp, li);
+      }
+    }
+  }
+  __syncthreads();
+  if (iend - 1 >= ia) store_plane(iend - 1);
+}
+
+// ---------------------------------------------------------------------------
 // Row kernels (nz % 8 == 0): a thread owns 8 consecutive cells k0..k0+7 of
 // one (i, j) row, loads them and the four neighbouring rows as 8-vectors
 // (16 B in fp16), keeps the x / y coefficients per thread and computes the
@@ -394,9 +574,17 @@ struct MixTailLevel {
   MixXfer x;  // to the next coarser level
 };
 
+// Coarsest levels up to this many cells are solved directly in float64.
+constexpr int kMixDenseMax = 512;
+
 struct MixTailArgs {
   int first, last;
   MixTailLevel lv[kMixMaxLevels];
+  // float64 dense solve of level `last` (dense_m free cells; 0: none)
+  int dense_m;
+  const int32_t* dense_idx;
+  const double* dense_vol;
+  const double* dense_inv;  // row-major [dense_m][dense_m]
 };
 
 __device__ inline void mix_tail_rb(const OpGeo<float>& g, float* e, const float* rhs, int color, bool from_zero) {
@@ -479,11 +667,28 @@ __global__ void __launch_bounds__(kMixTailThreads) k_mix_tail(const MixTailArgs*
   }
   {
     const MixTailLevel& L = A->lv[last];
-    for (int s = 0; s < coarse; ++s) {
-      mix_tail_rb(L.g, L.e, L.rhs, 0, s == 0);
+    if (coarse <= 0 && A->dense_m > 0) {
+      // float64 direct solve: e = S^-1 (V rhs) on the free cells, 0 elsewhere
+      __shared__ double bvec[kMixDenseMax];
+      const int m = A->dense_m;
+      for (int t = threadIdx.x; t < m; t += blockDim.x) bvec[t] = A->dense_vol[t] * (double)L.rhs[A->dense_idx[t]];
+      for (uint32_t c = threadIdx.x; c < L.g.n; c += blockDim.x) L.e[c] = 0.0f;
       __syncthreads();
-      mix_tail_rb(L.g, L.e, L.rhs, 1, false);
+      for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        const double* row = A->dense_inv + (size_t)t * m;
+        double v = 0.0;
+        for (int q = 0; q < m; ++q) v += row[q] * bvec[q];
+        L.e[A->dense_idx[t]] = (float)v;
+      }
       __syncthreads();
+    } else {
+      const int sweeps = coarse > 0 ? coarse : 8;
+      for (int s = 0; s < sweeps; ++s) {
+        mix_tail_rb(L.g, L.e, L.rhs, 0, s == 0);
+        __syncthreads();
+        mix_tail_rb(L.g, L.e, L.rhs, 1, false);
+        __syncthreads();
+      }
     }
   }
   for (int l = last - 1; l >= first; --l) {

@@ -1035,6 +1035,8 @@ struct MixLevel {
   double* wd[3] = {};
   float outlet_add = 0.0f;  // 2 / w_z[0]^2 / kappa
   void* e = nullptr;
+  void* e2 = nullptr;       // second iterate buffer of the fused sweeps (grid-launched levels, nz even)
+  void* cur = nullptr;      // the level's iterate during a V-cycle: e or e2
   void* rhs = nullptr;      // unused on level 0 (the CG residual is the input)
   void* res = nullptr;      // residual scratch of the two-pass restriction (grid-launched levels)
   // transfer to level + 1 (fine side): parent, interpolation partner, parent weight
@@ -1084,9 +1086,18 @@ struct hfsi_fluid {
   double mix_kappa = 1.0;
   uint32_t mix_half_min_cells = 1u << 20;  // levels with at least this many cells store fp16
   bool mix_row_kernels = false;            // 8-cell row kernels where nz % 8 == 0
+  bool mix_fused_sweeps = true;            // fused two-colour marching sweeps (k_mix_sweep)
   hfsi::MixLevel ml[hfsi::kMixMaxLevels];
   void* mix_tail_args = nullptr;  // MixTailArgs (device)
   int mix_tail_outlet = -1;       // outlet switch the tail arguments were written for
+  // float64 dense solve of the coarsest mixed level (at most kMixDenseMax cells): host copies of its
+  // metrics, and per outlet switch the inverse of its V-weighted operator (null space shifted)
+  std::vector<float> mixd_cl[3], mixd_ch[3], mixd_wf[3];
+  float mixd_outlet_add = 0.0f;
+  int mixd_m = 0;               // free cells of the coarsest level; 0: the tail sweeps it instead
+  int32_t* mixd_idx = nullptr;  // [kMixDenseMax] cell of each free cell
+  double* mixd_vol = nullptr;   // [kMixDenseMax] its volume
+  double* mixd_inv[2] = {};     // [kMixDenseMax^2] row-major inverse, closed / z-min outlet
   // convergence monitor of the mixed solve: pinned ring of <r,r>_V values and their events
   double* mon_ring = nullptr;
   std::vector<hipEvent_t> mon_events;
@@ -1557,6 +1568,10 @@ static int write_mix_tail_args(hfsi_fluid* h, int outlet, hipStream_t st) {
   MixTailArgs A{};
   A.first = h->mix_tail_first;
   A.last = h->ML - 1;
+  A.dense_m = h->mixd_m;
+  A.dense_idx = h->mixd_idx;
+  A.dense_vol = h->mixd_vol;
+  A.dense_inv = h->mixd_inv[outlet ? 1 : 0];
   for (int l = A.first; l <= A.last; ++l) {
     A.lv[l].g = mix_geo(h, l, outlet);
     A.lv[l].e = (float*)h->ml[l].e;
@@ -1572,6 +1587,22 @@ static int write_mix_tail_args(hfsi_fluid* h, int outlet, hipStream_t st) {
 // Row kernels (8 cells of one row per thread, vector loads) on levels whose
 // rows split into 8-cell groups; the flat kernels elsewhere.
 static inline bool mix_rows(const hfsi_fluid* h, const MixLevel& m) { return h->mix_row_kernels && m.nz % 8 == 0; }
+
+// Fused two-colour sweeps (k_mix_sweep) on grid-launched levels with a second
+// iterate buffer (nz even); colour passes (k_mix_rb / k_mix_rows) elsewhere.
+static inline bool mix_fused(const hfsi_fluid* h, const MixLevel& m) { return h->mix_fused_sweeps && m.e2 != nullptr; }
+
+static SweepGeo mix_sweep_geo(const MixLevel& m) {
+  SweepGeo sg;
+  sg.kblocks = (m.nz + kSwTK - 1) / kSwTK;
+  sg.jblocks = (m.ny + kSwTJ - 1) / kSwTJ;
+  // planes per workgroup: about 1024 workgroups, 8..32 planes (each chunk re-reads 3 planes)
+  uint32_t c = (uint32_t)(((uint64_t)m.nx * sg.kblocks * sg.jblocks + 512) / 1024);
+  c = c < 8 ? 8 : (c > 32 ? 32 : c);
+  sg.ichunk = c > m.nx ? m.nx : c;
+  sg.nblocks = sg.kblocks * sg.jblocks * ((m.nx + sg.ichunk - 1) / sg.ichunk);
+  return sg;
+}
 
 template <typename TE, typename TR>
 static void mix_rb_pass(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int color,
@@ -1600,93 +1631,116 @@ static void mix_rb_pass(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, 
   }
 }
 
-// pre / post smoothing on level l: red then black (pre), black then red (post)
+// pre / post smoothing on level l: red then black (pre), black then red
+// (post); the first pre-smoothing sweep starts from zero.  The iterate of the
+// level is ml[l].cur (the fused sweeps alternate between e and e2).
 template <typename TE, typename TR>
-static void mix_smooth(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int sweeps, bool pre,
+static void mix_smooth(hfsi_fluid* h, int l, int outlet, const TR* rhs, const double* rs, int sweeps, bool pre,
                        hipStream_t st) {
+  MixLevel& m = h->ml[l];
+  const int c0 = pre ? 0 : 1;
+  if (mix_fused(h, m)) {
+    const SweepGeo sg = mix_sweep_geo(m);
+    const uint32_t grid = (sg.nblocks + HFSI_NXCD - 1) / HFSI_NXCD * HFSI_NXCD;
+    OpGeo<float> g = mix_geo(h, l, outlet);
+    for (int s = 0; s < sweeps; ++s) {
+      TE* in = (TE*)m.cur;
+      TE* out = (TE*)(m.cur == m.e ? m.e2 : m.e);
+      if (pre && s == 0) {
+        hipLaunchKernelGGL((k_mix_sweep<TE, TR, true>), dim3(grid), dim3(kSwThreads), 0, st, g, in, out, rhs, rs, c0,
+                           sg);
+      } else {
+        if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
+        hipLaunchKernelGGL((k_mix_sweep<TE, TR, false>), dim3(grid), dim3(kSwThreads), 0, st, g, in, out, rhs, rs,
+                           c0, sg);
+        if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
+      }
+      m.cur = out;
+    }
+    return;
+  }
   for (int s = 0; s < sweeps; ++s) {
-    const int c0 = pre ? 0 : 1;
-    mix_rb_pass<TE, TR>(h, l, outlet, e, rhs, rs, c0, pre && s == 0, st);
-    mix_rb_pass<TE, TR>(h, l, outlet, e, rhs, rs, 1 - c0, false, st);
+    mix_rb_pass<TE, TR>(h, l, outlet, (TE*)m.cur, rhs, rs, c0, pre && s == 0, st);
+    mix_rb_pass<TE, TR>(h, l, outlet, (TE*)m.cur, rhs, rs, 1 - c0, false, st);
   }
 }
 
 // Residual of level l (stored at the level's own precision), then its
 // volume-weighted restriction into the rhs of level l + 1.
 template <typename TE, typename TR, typename TC>
-static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TE* e, const TR* rhs, const double* rs,
-                         hipStream_t st) {
+static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TR* rhs, const double* rs, hipStream_t st) {
   const MixLevel& f = h->ml[l];
   const MixLevel& c = h->ml[l + 1];
+  TE* e = (TE*)f.cur;
   OpGeo<float> g = mix_geo(h, l, outlet);
   if (mix_rows(h, f))
     hipLaunchKernelGGL((k_mix_rows<TE, TR, 2>), dim3(blocks_for(f.nx * f.ny * (f.nz / 8))), dim3(kBlock), 0, st, g,
-                       const_cast<TE*>(e), rhs, rs, 0, (TE*)f.res);
+                       e, rhs, rs, 0, (TE*)f.res);
   else
-    hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, e, rhs, rs,
-                       (TE*)f.res);
+    hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, (const TE*)e, rhs,
+                       rs, (TE*)f.res);
   hipLaunchKernelGGL((k_mix_restrict_res<TE, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, g, (const TE*)f.res,
                      mix_xfer(h, l), c.nx, c.ny, c.nz, (const uint8_t*)c.flags, (TC*)c.rhs);
 }
 
 template <typename TF, typename TCo>
-static void mix_prolong(hfsi_fluid* h, int l, int outlet, TF* ef, hipStream_t st) {
+static void mix_prolong(hfsi_fluid* h, int l, int outlet, hipStream_t st) {
   const MixLevel& c = h->ml[l + 1];
   const MixLevel& f = h->ml[l];
+  TF* ef = (TF*)f.cur;
   if (mix_rows(h, f))
     hipLaunchKernelGGL((k_mix_prolong_rows<TF, TCo>), dim3(blocks_for(f.nx * f.ny * (f.nz / 8))), dim3(kBlock), 0, st,
                        mix_geo(h, l, outlet), ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags,
-                       (const TCo*)c.e);
+                       (const TCo*)c.cur);
   else
     hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet),
-                       ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags, (const TCo*)c.e);
+                       ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags, (const TCo*)c.cur);
 }
 
 // Down-leg of level l (smoothing + restriction), with the storage types of l
 // and l + 1 resolved at run time.
 template <typename TE, typename TR>
-static void mix_down(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int pre,
-                     hipStream_t st) {
-  mix_smooth<TE, TR>(h, l, outlet, e, rhs, rs, pre, true, st);
+static void mix_down(hfsi_fluid* h, int l, int outlet, const TR* rhs, const double* rs, int pre, hipStream_t st) {
+  mix_smooth<TE, TR>(h, l, outlet, rhs, rs, pre, true, st);
   if (mix_half(h, l + 1))
-    mix_restrict<TE, TR, __half>(h, l, outlet, e, rhs, rs, st);
+    mix_restrict<TE, TR, __half>(h, l, outlet, rhs, rs, st);
   else
-    mix_restrict<TE, TR, float>(h, l, outlet, e, rhs, rs, st);
+    mix_restrict<TE, TR, float>(h, l, outlet, rhs, rs, st);
 }
 
 template <typename TE, typename TR>
-static void mix_up(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, const double* rs, int post,
-                   hipStream_t st) {
+static void mix_up(hfsi_fluid* h, int l, int outlet, const TR* rhs, const double* rs, int post, hipStream_t st) {
   if (mix_half(h, l + 1))
-    mix_prolong<TE, __half>(h, l, outlet, e, st);
+    mix_prolong<TE, __half>(h, l, outlet, st);
   else
-    mix_prolong<TE, float>(h, l, outlet, e, st);
-  mix_smooth<TE, TR>(h, l, outlet, e, rhs, rs, post, false, st);
+    mix_prolong<TE, float>(h, l, outlet, st);
+  mix_smooth<TE, TR>(h, l, outlet, rhs, rs, post, false, st);
 }
 
 // One mixed V-cycle on A e = r0 * vin (scaled level operators); leaves the
-// level-0 iterate in ml[0].e (fp16 or float per mix_half(h, 0)).
+// level-0 iterate in ml[0].cur (fp16 or float per mix_half(h, 0)).
 static int run_mix_vcycle(hfsi_fluid* h, const float* r0, int pre, int coarse, int post, int outlet, hipStream_t st) {
   HFSI_TRY(write_mix_tail_args(h, outlet, st));
   const double* vin = (const double*)((const char*)h->scal + offsetof(Scalars, vin));
+  for (int l = 0; l < h->ML; ++l) h->ml[l].cur = h->ml[l].e;
   const int T = h->ML > 1 ? h->mix_tail_first : 1;
   if (h->ML == 1) {
     if (mix_half(h, 0))
-      mix_smooth<__half, float>(h, 0, outlet, (__half*)h->ml[0].e, r0, vin, coarse, true, st);
+      mix_smooth<__half, float>(h, 0, outlet, r0, vin, coarse, true, st);
     else
-      mix_smooth<float, float>(h, 0, outlet, (float*)h->ml[0].e, r0, vin, coarse, true, st);
+      mix_smooth<float, float>(h, 0, outlet, r0, vin, coarse, true, st);
     HFSI_LAUNCH_CHECK();
     return HFSI_OK;
   }
   for (int l = 0; l < T; ++l) {
     const bool he = mix_half(h, l);
     if (l == 0) {
-      if (he) mix_down<__half, float>(h, 0, outlet, (__half*)h->ml[0].e, r0, vin, pre, st);
-      else mix_down<float, float>(h, 0, outlet, (float*)h->ml[0].e, r0, vin, pre, st);
+      if (he) mix_down<__half, float>(h, 0, outlet, r0, vin, pre, st);
+      else mix_down<float, float>(h, 0, outlet, r0, vin, pre, st);
     } else if (he) {
-      mix_down<__half, __half>(h, l, outlet, (__half*)h->ml[l].e, (const __half*)h->ml[l].rhs, nullptr, pre, st);
+      mix_down<__half, __half>(h, l, outlet, (const __half*)h->ml[l].rhs, nullptr, pre, st);
     } else {
-      mix_down<float, float>(h, l, outlet, (float*)h->ml[l].e, (const float*)h->ml[l].rhs, nullptr, pre, st);
+      mix_down<float, float>(h, l, outlet, (const float*)h->ml[l].rhs, nullptr, pre, st);
     }
   }
   hipLaunchKernelGGL(k_mix_tail, dim3(1), dim3(kMixTailThreads), 0, st, (const MixTailArgs*)h->mix_tail_args, pre,
@@ -1694,12 +1748,12 @@ static int run_mix_vcycle(hfsi_fluid* h, const float* r0, int pre, int coarse, i
   for (int l = T - 1; l >= 0; --l) {
     const bool he = mix_half(h, l);
     if (l == 0) {
-      if (he) mix_up<__half, float>(h, 0, outlet, (__half*)h->ml[0].e, r0, vin, post, st);
-      else mix_up<float, float>(h, 0, outlet, (float*)h->ml[0].e, r0, vin, post, st);
+      if (he) mix_up<__half, float>(h, 0, outlet, r0, vin, post, st);
+      else mix_up<float, float>(h, 0, outlet, r0, vin, post, st);
     } else if (he) {
-      mix_up<__half, __half>(h, l, outlet, (__half*)h->ml[l].e, (const __half*)h->ml[l].rhs, nullptr, post, st);
+      mix_up<__half, __half>(h, l, outlet, (const __half*)h->ml[l].rhs, nullptr, post, st);
     } else {
-      mix_up<float, float>(h, l, outlet, (float*)h->ml[l].e, (const float*)h->ml[l].rhs, nullptr, post, st);
+      mix_up<float, float>(h, l, outlet, (const float*)h->ml[l].rhs, nullptr, post, st);
     }
   }
   HFSI_LAUNCH_CHECK();
@@ -1714,11 +1768,11 @@ static int mix_precondition(hfsi_fluid* h, const float* r, const float* q, float
   OpGeo<double> g64 = geo<double>(h, 0, outlet);
   const uint32_t nb = blocks_for(g64.n);
   if (mix_half(h, 0))
-    hipLaunchKernelGGL((k_mix_finish<NS, __half>), dim3(nb), dim3(kBlock), 0, st, g64, (const __half*)h->ml[0].e, r,
-                       q, z, h->scal, h->partials);
+    hipLaunchKernelGGL((k_mix_finish<NS, __half>), dim3(nb), dim3(kBlock), 0, st, g64, (const __half*)h->ml[0].cur,
+                       r, q, z, h->scal, h->partials);
   else
-    hipLaunchKernelGGL((k_mix_finish<NS, float>), dim3(nb), dim3(kBlock), 0, st, g64, (const float*)h->ml[0].e, r, q,
-                       z, h->scal, h->partials);
+    hipLaunchKernelGGL((k_mix_finish<NS, float>), dim3(nb), dim3(kBlock), 0, st, g64, (const float*)h->ml[0].cur, r,
+                       q, z, h->scal, h->partials);
   finish_sums<NS>(h, nb, fin, st);
   HFSI_LAUNCH_CHECK();
   return HFSI_OK;
@@ -1831,9 +1885,9 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   const int pre = light ? 1 : 2, coarse = light ? 12 : 24, post = light ? 1 : 2;
   // mixed V-cycle: float32 CG vectors, no velocity-Dirichlet or interface terms
   bool use_mix = mixed && use_mg && std::is_same<TV, float>::value && h->ML >= 1 && !h->has_vd && !h->has_diag;
-  // RB-GS sweeps on the coarsest mixed level (<= 3 cells on an axis): 4, 8 and 24
-  // give the same iteration counts
-  const int mix_coarse = light ? 4 : 8;
+  // coarsest mixed level: the float64 dense solve when it is set up (0), else
+  // RB-GS sweeps (4, 8 and 24 sweeps give the same iteration counts)
+  const int mix_coarse = h->mixd_m > 0 ? 0 : (light ? 4 : 8);
   const double kappa = h->mix_kappa;
   const int lag = std::min(std::max(0, o.monitor_lag), kMonRing - 1);
   auto mix_scale = [&]() -> int {
@@ -2223,7 +2277,9 @@ static int build_mix_hierarchy(hfsi_fluid* h, const std::vector<float> wf0[3], c
       rc |= dalloc(h, (float**)&m.rhs, m.n);
     }
     rc |= dalloc(h, (float**)&m.e, m.n);
+    m.cur = m.e;
     if (l + 1 < ML && (l < h->mix_tail_first || ML == 1)) rc |= dalloc(h, (float**)&m.res, m.n);
+    if ((l < h->mix_tail_first || ML == 1) && m.nz % 2 == 0) rc |= dalloc(h, (float**)&m.e2, m.n);
     for (int a = 0; a < 3 && rc == HFSI_OK; ++a) {
       const size_t n = L.w[a].size();
       std::vector<float> cl(n, 0.0f), ch(n, 0.0f), wf(n);
@@ -2240,6 +2296,12 @@ static int build_mix_hierarchy(hfsi_fluid* h, const std::vector<float> wf0[3], c
         }
       }
       if (a == 2) m.outlet_add = (float)(2.0 / (L.w[2][0] * L.w[2][0]) / kappa);
+      if (l == ML - 1 && ML > 1) {
+        h->mixd_cl[a] = cl;
+        h->mixd_ch[a] = ch;
+        h->mixd_wf[a] = wf;
+        if (a == 2) h->mixd_outlet_add = m.outlet_add;
+      }
       rc |= dalloc(h, &m.cl[a], n);
       rc |= dalloc(h, &m.ch[a], n);
       rc |= dalloc(h, &m.wf[a], n);
@@ -2289,7 +2351,146 @@ static int build_mix_hierarchy(hfsi_fluid* h, const std::vector<float> wf0[3], c
     }
   }
   if (rc == HFSI_OK && ML > 1) rc |= dalloc(h, (char**)&h->mix_tail_args, sizeof(MixTailArgs));
+  if (rc == HFSI_OK && ML > 1 && h->ml[ML - 1].n <= (uint32_t)kMixDenseMax) {
+    rc |= dalloc(h, &h->mixd_idx, (size_t)kMixDenseMax);
+    rc |= dalloc(h, &h->mixd_vol, (size_t)kMixDenseMax);
+    rc |= dalloc(h, &h->mixd_inv[0], (size_t)kMixDenseMax * kMixDenseMax);
+    rc |= dalloc(h, &h->mixd_inv[1], (size_t)kMixDenseMax * kMixDenseMax);
+  }
   return rc;
+}
+
+// Inverse (row-major, float64) of the V-weighted operator S = V L of the
+// coarsest mixed level over its free cells.  S is symmetric positive
+// semidefinite; on every connected set of free cells without a z-min outlet
+// row it has the constants as null space, shifted out by adding
+// alpha 1 1^T (alpha = mean diagonal of the set / its size), which leaves
+// the solution of a consistent right-hand side unchanged (the mean-free
+// one).  Cholesky factorisation, then the inverse column by column.
+static bool mix_dense_inverse(const hfsi_fluid* h, const std::vector<uint8_t>& fl, const std::vector<int32_t>& idx,
+                              int outlet, std::vector<double>& inv) {
+  const MixLevel& m = h->ml[h->ML - 1];
+  const int nfree = (int)idx.size();
+  std::vector<int32_t> pos(m.n, -1);
+  for (int t = 0; t < nfree; ++t) pos[idx[t]] = t;
+  std::vector<double> S((size_t)nfree * nfree, 0.0);
+  auto vol = [&](uint32_t i, uint32_t j, uint32_t k) {
+    return (double)h->mixd_wf[0][i] * (double)h->mixd_wf[1][j] * (double)h->mixd_wf[2][k];
+  };
+  for (int t = 0; t < nfree; ++t) {
+    const uint32_t c = (uint32_t)idx[t];
+    const uint32_t k = c % m.nz, j = (c / m.nz) % m.ny, i = c / (m.ny * m.nz);
+    const uint8_t f = fl[c];
+    const double V = vol(i, j, k);
+    double den = 0.0;
+    auto face = [&](uint8_t bit, double coef, int64_t nb) {
+      if (!(f & bit)) return;
+      den += coef;
+      S[(size_t)t * nfree + pos[nb]] -= V * coef;
+    };
+    const int64_t sx = (int64_t)m.ny * m.nz, sy = m.nz;
+    face(kXm, h->mixd_cl[0][i], (int64_t)c - sx);
+    face(kXp, h->mixd_ch[0][i], (int64_t)c + sx);
+    face(kYm, h->mixd_cl[1][j], (int64_t)c - sy);
+    face(kYp, h->mixd_ch[1][j], (int64_t)c + sy);
+    face(kZm, h->mixd_cl[2][k], (int64_t)c - 1);
+    face(kZp, h->mixd_ch[2][k], (int64_t)c + 1);
+    if (outlet && k == 0) den += h->mixd_outlet_add;
+    S[(size_t)t * nfree + t] += V * den;
+  }
+  for (int a = 0; a < nfree; ++a)
+    for (int b = a + 1; b < nfree; ++b) {
+      const double v = 0.5 * (S[(size_t)a * nfree + b] + S[(size_t)b * nfree + a]);
+      S[(size_t)a * nfree + b] = S[(size_t)b * nfree + a] = v;
+    }
+  // connected sets of free cells (open faces), shifted unless they hold an outlet row
+  std::vector<int> comp(nfree, -1);
+  int ncomp = 0;
+  for (int t0 = 0; t0 < nfree; ++t0) {
+    if (comp[t0] >= 0) continue;
+    std::vector<int> stack{t0}, members;
+    comp[t0] = ncomp;
+    bool has_outlet = false;
+    while (!stack.empty()) {
+      const int t = stack.back();
+      stack.pop_back();
+      members.push_back(t);
+      if (outlet && idx[t] % (int32_t)m.nz == 0) has_outlet = true;
+      for (int u = 0; u < nfree; ++u)
+        if (u != t && comp[u] < 0 && S[(size_t)t * nfree + u] != 0.0) {
+          comp[u] = ncomp;
+          stack.push_back(u);
+        }
+    }
+    if (!has_outlet) {
+      double dsum = 0.0;
+      for (int t : members) dsum += S[(size_t)t * nfree + t];
+      const double alpha = dsum / (double)members.size() / (double)members.size();
+      for (int a : members)
+        for (int b : members) S[(size_t)a * nfree + b] += alpha;
+    }
+    ++ncomp;
+  }
+  // Cholesky S = L L^T (lower triangle in place)
+  for (int j = 0; j < nfree; ++j) {
+    double d = S[(size_t)j * nfree + j];
+    for (int q = 0; q < j; ++q) d -= S[(size_t)j * nfree + q] * S[(size_t)j * nfree + q];
+    if (!(d > 0.0) || !std::isfinite(d)) return false;
+    const double ljj = std::sqrt(d);
+    S[(size_t)j * nfree + j] = ljj;
+    for (int i = j + 1; i < nfree; ++i) {
+      double v = S[(size_t)i * nfree + j];
+      for (int q = 0; q < j; ++q) v -= S[(size_t)i * nfree + q] * S[(size_t)j * nfree + q];
+      S[(size_t)i * nfree + j] = v / ljj;
+    }
+  }
+  inv.assign((size_t)nfree * nfree, 0.0);
+  std::vector<double> y(nfree);
+  for (int col = 0; col < nfree; ++col) {
+    for (int i = 0; i < nfree; ++i) {  // L y = e_col
+      double v = i == col ? 1.0 : 0.0;
+      for (int q = 0; q < i; ++q) v -= S[(size_t)i * nfree + q] * y[q];
+      y[i] = v / S[(size_t)i * nfree + i];
+    }
+    for (int i = nfree - 1; i >= 0; --i) {  // L^T x = y
+      double v = y[i];
+      for (int q = i + 1; q < nfree; ++q) v -= S[(size_t)q * nfree + i] * y[q];
+      y[i] = v / S[(size_t)i * nfree + i];
+    }
+    for (int i = 0; i < nfree; ++i) inv[(size_t)i * nfree + col] = y[i];
+  }
+  return true;
+}
+
+// After the masks change: the free cells of the coarsest mixed level and the
+// two inverses (closed, outlet) of its dense solve.  mixd_m = 0 (the tail
+// sweeps the level) when the level is too large or has no free cell.
+static int build_mix_dense(hfsi_fluid* h, hipStream_t st) {
+  h->mixd_m = 0;
+  if (h->ML < 2 || !h->mixd_idx) return HFSI_OK;
+  const MixLevel& m = h->ml[h->ML - 1];
+  std::vector<uint8_t> fl(m.n);
+  HFSI_CHECK(hipMemcpyAsync(fl.data(), m.flags, m.n, hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  std::vector<int32_t> idx;
+  std::vector<double> vol;
+  for (uint32_t c = 0; c < m.n; ++c) {
+    if (fl[c] & kNotFree) continue;
+    idx.push_back((int32_t)c);
+    const uint32_t k = c % m.nz, j = (c / m.nz) % m.ny, i = c / (m.ny * m.nz);
+    vol.push_back((double)h->mixd_wf[0][i] * (double)h->mixd_wf[1][j] * (double)h->mixd_wf[2][k]);
+  }
+  if (idx.empty()) return HFSI_OK;
+  std::vector<double> inv0, inv1;
+  if (!mix_dense_inverse(h, fl, idx, 0, inv0) || !mix_dense_inverse(h, fl, idx, 1, inv1)) return HFSI_OK;
+  const size_t nf = idx.size();
+  HFSI_CHECK(hipMemcpyAsync(h->mixd_idx, idx.data(), nf * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HFSI_CHECK(hipMemcpyAsync(h->mixd_vol, vol.data(), nf * sizeof(double), hipMemcpyHostToDevice, st));
+  HFSI_CHECK(hipMemcpyAsync(h->mixd_inv[0], inv0.data(), nf * nf * sizeof(double), hipMemcpyHostToDevice, st));
+  HFSI_CHECK(hipMemcpyAsync(h->mixd_inv[1], inv1.data(), nf * nf * sizeof(double), hipMemcpyHostToDevice, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  h->mixd_m = (int)nf;
+  return HFSI_OK;
 }
 
 static void axis_metrics(const std::vector<double>& w, double origin, std::vector<float>& wf,
@@ -2583,6 +2784,7 @@ int hfsi_fluid_set_masks(hfsi_fluid* h, const int32_t* obstacle, const int32_t* 
   }
   h->mix_tail_outlet = -1;
   HFSI_LAUNCH_CHECK();
+  HFSI_TRY(build_mix_dense(h, st));
   // free volume (V over free cells) into the device scalars
   OpGeo<double> g = geo<double>(h, 0, 0);
   HFSI_TRY(launch_sums<1>(h, g, OpVolume{}, FinFreeVol{h->scal}, st));
@@ -2621,6 +2823,15 @@ int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells) {
     return HFSI_ERR_ARG;
   }
   h->mix_half_min_cells = (uint32_t)std::min<int64_t>(min_cells, (int64_t)UINT32_MAX);
+  return HFSI_OK;
+}
+
+int hfsi_fluid_mixed_set_fused_sweeps(hfsi_fluid* h, int32_t enable) {
+  if (!h) {
+    set_error("hfsi_fluid_mixed_set_fused_sweeps: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  h->mix_fused_sweeps = enable != 0;
   return HFSI_OK;
 }
 
@@ -2685,7 +2896,7 @@ int hfsi_fluid_weighted_dot(hfsi_fluid* h, const double* a, const double* b, dou
 
 int hfsi_fluid_vcycle(hfsi_fluid* h, const double* r, double* z, int32_t precision, int32_t pre, int32_t coarse,
                       int32_t post, int32_t outlet_zmin, void* stream) {
-  if (!h || !r || !z || pre < 1 || coarse < 1 || post < 1) {
+  if (!h || !r || !z || pre < 1 || coarse < (precision == HFSI_PREC_MIXED ? 0 : 1) || post < 1) {
     set_error("hfsi_fluid_vcycle: invalid arguments");
     return HFSI_ERR_ARG;
   }

@@ -143,7 +143,9 @@ int hfsi_fluid_diagonal(hfsi_fluid* h, double* diag, int32_t outlet_zmin, void* 
 int hfsi_fluid_weighted_dot(hfsi_fluid* h, const double* a, const double* b, double* out, void* stream);
 
 /* One V-cycle z = M^-1 r (_apply_fv_multigrid_preconditioner, fluid.py:6085-6262)
- * at the given precision; r and z are float64 level-0 arrays. */
+ * at the given precision; r and z are float64 level-0 arrays.  With
+ * HFSI_PREC_MIXED, coarse = 0 solves the coarsest level directly in float64
+ * (levels of at most 512 cells; 8 RB sweeps otherwise). */
 int hfsi_fluid_vcycle(hfsi_fluid* h, const double* r, double* z, int32_t precision, int32_t pre,
                       int32_t coarse, int32_t post, int32_t outlet_zmin, void* stream);
 
@@ -254,11 +256,12 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* particles, int32_t coun
 
 /* ---- mixed-precision V-cycle (HFSI_PREC_MIXED; no reference counterpart) ---
  * The MIXED solve keeps the reference's flexible PCG driver with float32
- * vectors and float64 solution, dots and true residual, and replaces its
- * V-cycle by a red-black Gauss-Seidel V-cycle on a semi-coarsened hierarchy
- * (cells merge along an axis while the merged width is <= 3x the level's
- * smallest width), trilinear graded prolongation, and fp16 storage on the
- * levels with at least half_min_cells cells (default 2^20).  Grids with
+ * vectors and float64 solution, SpMV arithmetic, dots and true residual, and
+ * replaces its V-cycle by a red-black Gauss-Seidel V-cycle on a
+ * semi-coarsened hierarchy (cells merge along an axis while the merged width
+ * is <= 2.5x the level's nominal width), trilinear graded prolongation, a
+ * float64 direct solve of the coarsest level, and fp16 storage on the levels
+ * with at least half_min_cells cells (default 2^20).  Grids with
  * velocity-Dirichlet rows or an interface diagonal fall back to the FP32
  * V-cycle (hfsi_cg_stats.used_multigrid = 1; 2 when the mixed V-cycle ran).
  * hfsi_fluid_vcycle with HFSI_PREC_MIXED applies it to r (scale from r). */
@@ -272,6 +275,10 @@ int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells);
  * than the cell-per-thread kernels on MI355X); both compute the same
  * arithmetic. */
 int hfsi_fluid_mixed_set_row_kernels(hfsi_fluid* h, int32_t enable);
+/* Fused sweeps (default on): both colour passes of a red-black sweep in one
+ * marching pass over memory, on grid-launched levels with nz even; 0 selects
+ * one launch per colour pass.  Both give identical results. */
+int hfsi_fluid_mixed_set_fused_sweeps(hfsi_fluid* h, int32_t enable);
 
 /* ---- learned pressure initial guess (no reference counterpart) -----------
  * A 3-level 3D U-Net (3x3x3 convolutions, channels 16 / 32 / 64, two per

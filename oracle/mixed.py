@@ -274,8 +274,56 @@ def prolong_add(f: MixLevel, c: MixLevel, ef, ec):
     return np.where(f.notfree, ef, ef + add)
 
 
+def dense_solve(lvl: MixLevel, rhs, outlet=False):
+    """Direct solve of L e = rhs on a (small) level, e = 0 on cells that are not
+    free: S = V L over the free cells is symmetric positive semidefinite, with
+    the constants of every connected set of free cells without an outlet row
+    as null space; adding alpha 1 1^T on such a set (alpha = mean diagonal of
+    the set / its size) makes it definite and leaves the solution of a
+    consistent right-hand side (the mean-free one) unchanged."""
+    shape = lvl.shape
+    free = ~lvl.notfree
+    cells = np.flatnonzero(free.ravel())
+    m = cells.size
+    e = np.zeros(shape)
+    if m == 0:
+        return e
+    V = (lvl.w[0][:, None, None] * lvl.w[1][None, :, None] * lvl.w[2][None, None, :]).ravel()[cells]
+    S = np.zeros((m, m))
+    unit = np.zeros(int(np.prod(shape)))
+    for t, c in enumerate(cells):
+        unit[c] = 1.0
+        S[:, t] = V * apply(lvl, unit.reshape(shape), outlet).ravel()[cells]
+        unit[c] = 0.0
+    S = 0.5 * (S + S.T)
+    # connected sets of free cells through open faces
+    comp = -np.ones(m, np.int64)
+    ncomp = 0
+    for t0 in range(m):
+        if comp[t0] >= 0:
+            continue
+        stack, members = [t0], []
+        comp[t0] = ncomp
+        while stack:
+            t = stack.pop()
+            members.append(t)
+            for u in np.flatnonzero((S[t] != 0.0) & (comp < 0)):
+                comp[u] = ncomp
+                stack.append(int(u))
+        members = np.asarray(members)
+        has_outlet = outlet and bool(np.any(cells[members] % shape[2] == 0))
+        if not has_outlet:
+            alpha = S[members, members].sum() / members.size / members.size
+            S[np.ix_(members, members)] += alpha
+        ncomp += 1
+    b = V * np.asarray(rhs, np.float64).ravel()[cells]
+    e.ravel()[cells] = np.linalg.solve(S, b)
+    return e
+
+
 def vcycle(levels: list[MixLevel], r, *, pre=2, coarse=24, post=2, outlet=False):
-    """z = M r (float64), the V-cycle of the mixed solve."""
+    """z = M r (float64), the V-cycle of the mixed solve; coarse <= 0 solves
+    the coarsest level directly (dense_solve), coarse > 0 gives its RB sweeps."""
     L = len(levels)
     rhs = [np.where(levels[0].notfree, 0.0, np.asarray(r, np.float64))] + [None] * (L - 1)
     e = [None] * L
@@ -293,6 +341,8 @@ def vcycle(levels: list[MixLevel], r, *, pre=2, coarse=24, post=2, outlet=False)
         e[l] = x
         rhs[l + 1] = restrict(levels[l], levels[l + 1], x, rhs[l], outlet)
     x = np.zeros(levels[-1].shape)
+    if coarse <= 0:
+        x = dense_solve(levels[-1], rhs[-1], outlet)
     for s in range(coarse):
         x = rb_pass(levels[-1], x, rhs[-1], 0, outlet, from_zero=(s == 0))
         x = rb_pass(levels[-1], x, rhs[-1], 1, outlet)

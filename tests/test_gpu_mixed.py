@@ -85,12 +85,31 @@ def test_mixed_hierarchy_matches_oracle():
     assert _shapes(s) == [lv.shape for lv in X.build_hierarchy(u, np.zeros((17, 12, 9), np.int32))]
 
 
-def _vcycle(s, r, outlet):
+def _vcycle(s, r, outlet, coarse=24):
     rt = torch.from_numpy(np.ascontiguousarray(r)).cuda()
     z = torch.zeros_like(rt)
-    _ok(s, s._lib.hfsi_fluid_vcycle(s._handle, rt.data_ptr(), z.data_ptr(), PREC_MIXED, 2, 24, 2, int(outlet),
+    _ok(s, s._lib.hfsi_fluid_vcycle(s._handle, rt.data_ptr(), z.data_ptr(), PREC_MIXED, 2, coarse, 2, int(outlet),
                                     _stream()))
     return z.cpu().numpy()
+
+
+@pytest.mark.parametrize("outlet", [False, True])
+@pytest.mark.parametrize("half", [False, True])
+def test_mixed_vcycle_dense_coarsest_solve_matches_oracle(outlet, half):
+    """coarse = 0: the coarsest level (3 x 4 x 4 here) is solved in float64 through the inverse of its
+    null-space-shifted V-weighted operator, against oracle.mixed.dense_solve inside the same V-cycle."""
+    _say(f"mixed vcycle dense coarsest outlet={outlet} half={half}")
+    w, obst = _sphere_case(16, 8, 1.2, 0.12)
+    rng = np.random.default_rng(61)
+    r = np.where(obst == 0, rng.normal(size=obst.shape), 0.0)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    if half:
+        _ok(s, s._lib.hfsi_fluid_mixed_set_half_min_cells(s._handle, 0))
+    got = _vcycle(s, r, outlet, coarse=0)
+    want = X.vcycle(X.build_hierarchy(w, obst), r, coarse=0, outlet=outlet)
+    np.testing.assert_allclose(got, want, rtol=0, atol=(5e-3 if half else 1e-4) * np.abs(want).max())
 
 
 @pytest.mark.parametrize("outlet", [False, True])
@@ -139,6 +158,39 @@ def test_mixed_row_kernels_match_oracle_and_cell_kernels(outlet, half):
     scale = np.abs(want).max()
     np.testing.assert_allclose(rows, want, rtol=0, atol=(5e-3 if half else 1e-4) * scale)
     np.testing.assert_allclose(rows, cells, rtol=0, atol=(2e-3 if half else 1e-6) * scale)
+
+
+@pytest.mark.parametrize("case", ["sphere", "wide_k"])
+@pytest.mark.parametrize("half", [False, True])
+def test_mixed_fused_sweeps_equal_colour_passes(case, half):
+    """The fused marching sweep computes both colour passes of a sweep with
+    the first colour's results rounded to the storage type in LDS, as the
+    two-pass form rounds them in memory: the V-cycles agree bit for bit.
+    Cases cover partial (j, k) tiles, several i-chunks and, in wide_k,
+    several k-tiles (nz = 130)."""
+    _say(f"mixed fused sweeps {case} half={half}")
+    if case == "sphere":
+        w, obst = _sphere_case(16, 8, 1.2, 0.12, order=(1, 2, 0))
+    else:
+        rng = np.random.default_rng(5)
+        w = [rng.uniform(0.5, 1.5, n) for n in (24, 20, 130)]
+        w = [a / a.sum() for a in w]
+        obst = (rng.random((24, 20, 130)) < 0.05).astype(np.int32)
+    rng = np.random.default_rng(41)
+    r = np.where(obst == 0, rng.normal(size=obst.shape), 0.0)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    s._set_masks()
+    if half:
+        _ok(s, s._lib.hfsi_fluid_mixed_set_half_min_cells(s._handle, 0))
+    for outlet in (False, True):
+        _ok(s, s._lib.hfsi_fluid_mixed_set_fused_sweeps(s._handle, 1))
+        fused = _vcycle(s, r, outlet)
+        _ok(s, s._lib.hfsi_fluid_mixed_set_fused_sweeps(s._handle, 0))
+        passes = _vcycle(s, r, outlet)
+        np.testing.assert_array_equal(fused, passes)
+        want = X.vcycle(X.build_hierarchy(w, obst), r, outlet=outlet)
+        np.testing.assert_allclose(fused, want, rtol=0, atol=(5e-3 if half else 1e-4) * np.abs(want).max())
 
 
 def _cg(s, div, precision, tol, max_it, outlet, remove_mean, check=10, monitor_lag=0):

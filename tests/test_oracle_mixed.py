@@ -132,3 +132,77 @@ def test_vcycle_contracts_energy_error(outlet):
         prev = e
     # asymptotic contraction per cycle (0.23 here; 0.73 with the "any child solid" coarse rule)
     assert max(rates[4:]) < 0.3, rates
+
+
+def _walled_case(seed):
+    """Small graded level whose free cells form two sets separated by a solid plane (i = 3)."""
+    rng = np.random.default_rng(seed)
+    shape = (7, 5, 6)
+    w = [rng.uniform(0.5, 1.5, n) / n for n in shape]
+    obst = (rng.random(shape) < 0.08).astype(np.int32)
+    obst[3] = 1
+    return w, obst
+
+
+def test_dense_solve_with_outlet_solves_the_level_and_equals_converged_rbgs():
+    w, obst = _walled_case(3)
+    lvl = X.build_hierarchy(w, obst)[0]
+    rng = np.random.default_rng(4)
+    free = ~lvl.notfree
+    b = np.where(free, rng.normal(size=lvl.shape), 0.0)
+    # with the outlet only the set touching k = 0 is Dirichlet; make b consistent on any closed set
+    e = X.dense_solve(lvl, b, outlet=True)
+    r = np.where(free, b - X.apply(lvl, e, True), 0.0)
+    assert np.abs(r).max() <= 1e-10 * np.abs(b).max()
+    g = np.zeros(lvl.shape)
+    for s in range(4000):
+        g = X.rb_pass(lvl, g, b, 0, True, from_zero=(s == 0))
+        g = X.rb_pass(lvl, g, b, 1, True)
+    np.testing.assert_allclose(g, e, rtol=0, atol=1e-9 * np.abs(e).max())
+
+
+def test_dense_solve_closed_sets_give_the_mean_free_solution():
+    """Without an outlet every connected set of free cells is singular (constants); for a right-hand side
+    with zero V-weighted sum on each set the solve returns a solution whose plain sum vanishes on each set
+    (the shift alpha 1 1^T acts only on that direction)."""
+    w, obst = _walled_case(5)
+    lvl = X.build_hierarchy(w, obst)[0]
+    rng = np.random.default_rng(6)
+    free = ~lvl.notfree
+    # the level's own widths (level-0 metrics are rounded through float32, like the reference's fields)
+    V = lvl.w[0][:, None, None] * lvl.w[1][None, :, None] * lvl.w[2][None, None, :]
+    b = np.where(free, rng.normal(size=lvl.shape), 0.0)
+    sets = [np.zeros(lvl.shape, bool), np.zeros(lvl.shape, bool)]
+    sets[0][:3] = True
+    sets[1][4:] = True
+    for m in sets:
+        m &= free
+        b = np.where(m, b - (V * b)[m].sum() / V[m].sum(), b)
+    e = X.dense_solve(lvl, b, outlet=False)
+    r = np.where(free, b - X.apply(lvl, e, False), 0.0)
+    assert np.abs(r).max() <= 1e-10 * np.abs(b).max()
+    for m in sets:
+        assert abs(e[m].sum()) <= 1e-10 * np.abs(e).max() * m.sum()
+    assert np.all(e[~free] == 0.0)
+
+
+def test_vcycle_with_dense_coarsest_solve_contracts_better_than_sweeps():
+    wa = _graded(12, 6, 1.2)
+    shape = (len(wa),) * 3
+    w = [wa, wa, wa]
+    obst = np.zeros(shape, np.int32)
+    levels = X.build_hierarchy(w, obst)
+    lvl = levels[0]
+    rng = np.random.default_rng(7)
+    x_true = rng.normal(size=shape)
+    b = X.apply(lvl, x_true, True)
+    V = w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
+
+    def energy(e):
+        return float(np.sqrt((V * e * X.apply(lvl, e, True)).sum()))
+
+    errs = {}
+    for coarse in (0, 1):
+        x = X.vcycle(levels, b, coarse=coarse, outlet=True)
+        errs[coarse] = energy(x_true - x) / energy(x_true)
+    assert errs[0] < errs[1] < 1.0
