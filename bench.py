@@ -209,6 +209,73 @@ def run_variant(name, solver, divs, steps, warmup, guess=None):
     }
 
 
+def run_project(solver, steps: int, warmup: int):
+    """Full projections through the facade (divergence, mixed MGPCG to 1e-8 from p = 0, gradient
+    subtraction, wall clamp; report not read): velocity fields are seeded |k| <= 16 fields."""
+    import torch
+
+    n = solver.nx
+    fields = [fourier_velocity((n, n, n), 16, 5000 + s) for s in range(2)]
+    kw = dict(iterations=500, pressure_solver="fv_cg", cg_preconditioner="fv_multigrid", cg_tolerance=1e-8,
+              reset_pressure=True, read_report=False)
+    for s in range(warmup):
+        solver.velocity.tensor.copy_(fields[s % 2])
+        solver.project(**kw)
+    torch.cuda.synchronize()
+    times = []
+    for s in range(steps):
+        solver.velocity.tensor.copy_(fields[s % 2])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.project(**kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    st = solver.last_cg_stats
+    return {"projections_per_s": round(1.0 / statistics.median(times), 3),
+            "median_ms": round(1e3 * statistics.median(times), 3), "steps": steps,
+            "last_iterations": st["iterations"], "last_true_rel_residual": st.get("true_relative_residual")}
+
+
+UNET_LAYERS = [(16, 16, 1), (16, 16, 1), (16, 32, 2), (32, 32, 2), (32, 64, 3), (64, 64, 3), (96, 32, 2),
+               (32, 32, 2), (48, 16, 1), (16, 16, 1)]  # (cin, cout, level) of the 3x3x3 convolutions
+
+
+def run_unet(shape, steps: int, warmup: int = 2):
+    """Forward passes and training steps (forward, loss, backward, Adam) of the guess network at `shape`
+    (the inference shape of the 256^3 solve is 128^3), bf16 MFMA; FLOPs counted as 2*27*cin*cout per voxel
+    and convolution (forward), 3x that for a training step (data and weight gradients)."""
+    import torch
+
+    from easyfsi_amd import learned as Lm
+
+    net = Lm.GuessNetwork(shape, train=True, seed=1)
+    n = int(np.prod(shape))
+    feats = torch.randn(n, 5, device="cuda")
+    target = torch.randn(shape, device="cuda")
+    fwd_flops = sum(2.0 * 27 * ci * co * n / 8 ** (lvl - 1) for ci, co, lvl in UNET_LAYERS) + 2.0 * 16 * n
+    for _ in range(warmup):
+        net.forward(feats)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward(feats)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / steps
+    for _ in range(warmup):
+        net.train_step(feats, target, None, lr=1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.train_step(feats, target, None, lr=1e-4)
+    torch.cuda.synchronize()
+    t_train = (time.perf_counter() - t0) / steps
+    peak = 2500.0
+    return {"shape": list(shape), "forward_ms": round(1e3 * t_fwd, 3), "train_step_ms": round(1e3 * t_train, 3),
+            "forward_tflops": round(fwd_flops / t_fwd / 1e12, 2),
+            "train_tflops": round(3 * fwd_flops / t_train / 1e12, 2), "peak_tflops_bf16": peak,
+            "forward_frac": round(fwd_flops / t_fwd / 1e12 / peak, 4)}
+
+
 def run_sequence(solver, w, steps: int, guess=None):
     """Time-coherent projections (the C4 comparison): a sphere r = 0.06 moving at 0.25 core cells per step
     through a velocity field that turns between two seeded |k| <= 16 fields by 0.05 rad per step.  Each
@@ -359,6 +426,12 @@ def main(argv=None) -> int:
     ap.add_argument("--weights", default=None, help="guess network weights (default: the packaged ones)")
     ap.add_argument("--sequence-steps", type=int, default=10,
                     help="time-coherent projections for the warm-start comparison (0: skip)")
+    ap.add_argument("--project-steps", type=int, default=10, help="full facade projections timed beside (0: skip)")
+    ap.add_argument("--unet-steps", type=int, default=10,
+                    help="guess-network forward passes and training steps timed at the inference shape (0: skip)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB: the known-size kernel that "
+                         "scales hardware byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mpm-steps", type=int, default=20, help="neo-Hookean MPM steps timed beside (0: skip)")
@@ -377,6 +450,13 @@ def main(argv=None) -> int:
         if v not in VARIANTS:
             print(f"unknown variant {v!r}; choose from {sorted(VARIANTS)}", file=sys.stderr)
             return 2
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copied {args.copy_mb} MiB device to device", file=sys.stderr, flush=True)
+        del src, dst
     print(f"[bench] building {args.size}^3 graded grid", file=sys.stderr, flush=True)
     solver, w = make_solver(args.size, "mixed")
     n = solver.nx * solver.ny * solver.nz
@@ -408,6 +488,17 @@ def main(argv=None) -> int:
     bytes_per_cell = VARIANTS[head_name]["bytes_per_cell"]
     avg_s = head["timer_ms_total"] / max(head["timer_launches"], 1) / 1e3
     achieved = bytes_per_cell * n / avg_s / 1e9 if head["timer_launches"] else 0.0
+    unet = None
+    if args.unet_steps > 0:
+        print(f"[bench] guess network: {args.unet_steps} forward passes / training steps", file=sys.stderr,
+              flush=True)
+        unet = run_unet((args.size // 2,) * 3, args.unet_steps)
+    project = None
+    if args.project_steps > 0:
+        print(f"[bench] full projections: {args.project_steps}", file=sys.stderr, flush=True)
+        solver.mark_sphere_obstacle((0.5, 0.5, 0.5), 0.06)
+        solver._set_masks()
+        project = run_project(solver, args.project_steps, 1)
     sequence = None
     if args.sequence_steps > 0:
         print(f"[bench] warm-start comparison: {args.sequence_steps} time-coherent projections", file=sys.stderr,
@@ -464,6 +555,10 @@ def main(argv=None) -> int:
         line["mpm"] = mpm
     if sequence is not None:
         line["warm_start_sequence"] = sequence
+    if project is not None:
+        line["project"] = project
+    if unet is not None:
+        line["unet"] = unet
     if "learned_mixed" in results and "mixed" in results:
         line["learned_vs_mixed"] = round(results["learned_mixed"]["solves_per_s"] / results["mixed"]["solves_per_s"],
                                          3)

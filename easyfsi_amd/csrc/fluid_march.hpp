@@ -149,7 +149,14 @@ __device__ inline MarchPos march_pos(const MarchGeo& m) {
 // Weighted-Jacobi sweep / apply / residual in marching form
 // ---------------------------------------------------------------------------
 
-enum MarchMode { kModeNormal = 0, kModeFromZero = 1, kModeProlong = 2, kModeApply = 3 };
+enum MarchMode { kModeNormal = 0, kModeFromZero = 1, kModeProlong = 2, kModeApply = 3, kModeResidual = 4 };
+
+// partial sums per mode: <d, A d>_V (apply); {<r,r>_V, sum V r} of r = rhs - A p (residual, nothing
+// stored); the five CG sums (sweeps)
+template <int MODE>
+constexpr int march_sums() {
+  return MODE == kModeApply ? 1 : (MODE == kModeResidual ? 2 : 5);
+}
 
 template <typename C, typename TP, typename TR, typename TO, typename TQ>
 struct MarchArgs {
@@ -196,10 +203,11 @@ __device__ inline void march_load(const MarchArgs<C, TP, TR, TO, TQ>& a, uint32_
 
 // One marching kernel for: weighted-Jacobi sweep (normal / from zero /
 // prolongation fused), operator apply with <d, A d>_V (kModeApply + DOTS),
-// and the final level-0 post-sweep with the five CG sums (DOTS).
+// the norm of the residual rhs_sign * rhs - A p (kModeResidual + DOTS, no
+// store) and the final level-0 post-sweep with the five CG sums (DOTS).
 template <typename C, typename TP, typename TR, typename TO, typename TQ, int MODE, bool DOTS>
 __global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a) {
-  constexpr int NS = (MODE == kModeApply) ? 1 : 5;
+  constexpr int NS = march_sums<MODE>();
   __shared__ double lds[DOTS ? NS * 4 : 1];
   double acc[NS];
 #pragma unroll
@@ -260,7 +268,7 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a) {
       C rhs[4];
       if (MODE != kModeApply) Pack4<TR>::load(a.rhs + c, rhs);
       C qv[4];
-      if (DOTS && MODE != kModeApply) Pack4<TQ>::load(a.q + c, qv);
+      if (DOTS && MODE != kModeApply && MODE != kModeResidual) Pack4<TQ>::load(a.q + c, qv);
       const C cxl = a.cl[0][i], cxh = a.ch[0][i];
       C out[4];
 #pragma unroll
@@ -290,6 +298,8 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a) {
         C v;
         if (MODE == kModeApply) {
           v = (f & kObst) ? (C)0 : den * center - nb;
+        } else if (MODE == kModeResidual) {
+          v = (f & kObst) ? (C)0 : a.rhs_sign * rhs[m] - (den * center - nb);
         } else if (f & kObst) {
           v = (C)0;
         } else if (!(den > (C)0)) {
@@ -300,7 +310,7 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a) {
         }
         out[m] = v;
       }
-      Pack4<TO>::store(a.out + c, out);
+      if (MODE != kModeResidual) Pack4<TO>::store(a.out + c, out);
       if (DOTS) {
         const double wx = a.wd[0][i];
         C stored[4];  // the rounded values actually kept
@@ -313,6 +323,10 @@ __global__ void __launch_bounds__(256) k_march(MarchArgs<C, TP, TR, TO, TQ> a) {
           const double V = wx * wyz[m];
           if (MODE == kModeApply) {
             acc[0] += (double)pc_r[m] * (double)stored[m] * V;
+          } else if (MODE == kModeResidual) {
+            const double rv = (double)out[m];
+            acc[0] += rv * rv * V;
+            acc[1] += rv * V;
           } else {
             const double zv = (double)stored[m], rv = -(double)(a.rhs_sign * rhs[m]), qd = (double)qv[m];
             acc[0] += rv * zv * V;

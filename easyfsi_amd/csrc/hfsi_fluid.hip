@@ -1276,7 +1276,7 @@ static void launch_march(hfsi_fluid* h, int l, int outlet, const TP* p, const TR
   a.counter = h->counter;
   a.totals = h->totals;
   hipLaunchKernelGGL((k_march<C, TP, TR, TO, TQ, MODE, DOTS>), dim3(march_grid(a.m)), dim3(256), 0, st, a);
-  if (DOTS) finish_sums<(MODE == kModeApply) ? 1 : 5>(h, march_grid(a.m), fin, st);
+  if (DOTS) finish_sums<march_sums<MODE>()>(h, march_grid(a.m), fin, st);
 }
 
 template <typename C, typename TP, typename TR, typename TRes>
@@ -1972,9 +1972,15 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
       timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
       finish_sums<1>(h, blocks_for(g64.n), FinVScale{sc, kappa}, st);
       if (monitor) {
-        hipLaunchKernelGGL((k_true_residual<float>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, h->b,
-                           (float*)nullptr, h->partials);
-        finish_sums<2>(h, blocks_for(g64.n), FinTrueMonitor{sc, rm, h->mon_ring + (it % kMonRing)}, st);
+        const FinTrueMonitor fin{sc, rm, h->mon_ring + (it % kMonRing)};
+        if (march_ok(h, 0)) {
+          launch_march<double, double, double, double, double, kModeResidual, true>(
+              h, 0, outlet, x, h->b, 1.0, (double*)nullptr, 0.0, nullptr, 0.0, (const double*)nullptr, fin, st);
+        } else {
+          hipLaunchKernelGGL((k_true_residual<float>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, h->b,
+                             (float*)nullptr, h->partials);
+          finish_sums<2>(h, blocks_for(g64.n), fin, st);
+        }
         HFSI_CHECK(hipEventRecord(h->mon_events[it % kMonRing], st));
       }
       HFSI_LAUNCH_CHECK();

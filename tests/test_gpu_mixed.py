@@ -245,16 +245,17 @@ def test_mixed_pcg_matches_reference_solution(outlet):
     assert st.iterations < stf.iterations, (st.iterations, stf.iterations)
 
 
+@pytest.mark.parametrize("order", [(0, 1, 2), (1, 2, 0)])  # nz = 34: flat monitor kernel; nz = 32: marching
 @pytest.mark.parametrize("outlet", [False, True])
-def test_mixed_monitor_stops_at_first_converged_iterate(outlet):
+def test_mixed_monitor_stops_at_first_converged_iterate(outlet, order):
     """monitor_lag = 1 stops at the first iterate whose float64 true residual
     meets the tolerance: the same solve without the monitor, capped at the
     monitor's iteration count, ends with the same pressure (the monitor does
     not change the iteration) and a true residual within the tolerance; capped
     one iteration earlier it ends above it.  monitor_lag = 2 reads one
     iteration late and stops at most two iterations later."""
-    _say(f"mixed monitor outlet={outlet}")
-    w, obst = _sphere_case(16, 8, 1.2, 0.12)
+    _say(f"mixed monitor outlet={outlet} order={order}")
+    w, obst = _sphere_case(16, 8, 1.2, 0.12, order=order)
     shape = obst.shape
     rng = np.random.default_rng(53)
     div = np.where(obst == 0, rng.normal(size=shape), 0.0).astype(np.float32)
