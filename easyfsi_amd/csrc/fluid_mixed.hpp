@@ -116,21 +116,27 @@ __global__ void __launch_bounds__(kBlock) k_mix_residual(OpGeo<float> g, const T
                                                           const double* rs_dev, TRes* res) {
   const float rs = rhs_scale(rs_dev);
   HFSI_CELL_LOOP(g.n) {
+    // every load is issued before the flag test (neighbours guarded by the grid bounds only), so a
+    // cell costs one memory round trip
+    const uint32_t k = c % g.nz;
+    const uint32_t t = c / g.nz;
+    const uint32_t j = t % g.ny, i = t / g.ny;
     const uint8_t f = g.flags[c];
+    const float ec = ldf<TE>(e, c), r = ldf<TR>(rhs, c);
+    const float exm = i > 0 ? ldf<TE>(e, c - g.sx) : 0.0f, exp_ = i + 1 < g.nx ? ldf<TE>(e, c + g.sx) : 0.0f;
+    const float eym = j > 0 ? ldf<TE>(e, c - g.sy) : 0.0f, eyp = j + 1 < g.ny ? ldf<TE>(e, c + g.sy) : 0.0f;
+    const float ezm = k > 0 ? ldf<TE>(e, c - 1) : 0.0f, ezp = k + 1 < g.nz ? ldf<TE>(e, c + 1) : 0.0f;
     float out = 0.0f;
     if (!(f & kNotFree)) {
-      const uint32_t k = c % g.nz;
-      const uint32_t t = c / g.nz;
-      const uint32_t j = t % g.ny, i = t / g.ny;
       float nb = 0.0f, den = 0.0f;
-      if (f & kXm) { const float q = g.cl[0][i]; nb += q * ldf<TE>(e, c - g.sx); den += q; }
-      if (f & kXp) { const float q = g.ch[0][i]; nb += q * ldf<TE>(e, c + g.sx); den += q; }
-      if (f & kYm) { const float q = g.cl[1][j]; nb += q * ldf<TE>(e, c - g.sy); den += q; }
-      if (f & kYp) { const float q = g.ch[1][j]; nb += q * ldf<TE>(e, c + g.sy); den += q; }
-      if (f & kZm) { const float q = g.cl[2][k]; nb += q * ldf<TE>(e, c - 1); den += q; }
-      if (f & kZp) { const float q = g.ch[2][k]; nb += q * ldf<TE>(e, c + 1); den += q; }
+      if (f & kXm) { const float q = g.cl[0][i]; nb += q * exm; den += q; }
+      if (f & kXp) { const float q = g.ch[0][i]; nb += q * exp_; den += q; }
+      if (f & kYm) { const float q = g.cl[1][j]; nb += q * eym; den += q; }
+      if (f & kYp) { const float q = g.ch[1][j]; nb += q * eyp; den += q; }
+      if (f & kZm) { const float q = g.cl[2][k]; nb += q * ezm; den += q; }
+      if (f & kZp) { const float q = g.ch[2][k]; nb += q * ezp; den += q; }
       if (g.outlet && k == 0) den += g.outlet_add;
-      out = rs * ldf<TR>(rhs, c) - (den * ldf<TE>(e, c) - nb);
+      out = rs * r - (den * ec - nb);
     }
     stf<TRes>(res, c, out);
   }
@@ -147,23 +153,41 @@ __global__ void __launch_bounds__(kBlock) k_mix_restrict_res(OpGeo<float> g, con
     const uint32_t K = c % cnz;
     const uint32_t t = c / cnz;
     const uint32_t J = t % cny, I = t / cny;
+    const uint8_t cf = cflags[c];
+    const int32_t i0 = x.first[0][I], j0 = x.first[1][J], k0 = x.first[2][K];
+    const int32_t ni = x.cnt[0][I], nj = x.cnt[1][J], nk = x.cnt[2][K];
+    // the (up to) 8 children's residuals, flags and widths, loaded together (a missing second child
+    // along an axis repeats the first and is skipped below)
+    float rv[2][2][2], wx[2], wy[2], wz[2];
+    uint8_t fv[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      wx[a] = g.wf[0][(uint32_t)(i0 + (a < ni ? a : 0))];
+      wy[a] = g.wf[1][(uint32_t)(j0 + (a < nj ? a : 0))];
+      wz[a] = g.wf[2][(uint32_t)(k0 + (a < nk ? a : 0))];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const uint32_t cc = (uint32_t)(i0 + (a < ni ? a : 0)) * g.sx + (uint32_t)(j0 + (b < nj ? b : 0)) * g.sy +
+                              (uint32_t)(k0 + (d < nk ? d : 0));
+          fv[a][b][d] = g.flags[cc];
+          rv[a][b][d] = ldf<TRes>(res, cc);
+        }
     float out = 0.0f;
-    if (!(cflags[c] & kNotFree)) {
-      const int32_t i0 = x.first[0][I], j0 = x.first[1][J], k0 = x.first[2][K];
-      const int32_t ni = x.cnt[0][I], nj = x.cnt[1][J], nk = x.cnt[2][K];
+    if (!(cf & kNotFree)) {
       float s = 0.0f, vs = 0.0f;
       for (int32_t oi = 0; oi < ni; ++oi) {
-        const uint32_t i = (uint32_t)(i0 + oi);
-        const float wi = g.wf[0][i];
+        const float wi = wx[oi];
         for (int32_t oj = 0; oj < nj; ++oj) {
-          const uint32_t j = (uint32_t)(j0 + oj);
-          const float wij = wi * g.wf[1][j];
-          const uint32_t row = i * g.sx + j * g.sy;
+          const float wij = wi * wy[oj];
           for (int32_t ok = 0; ok < nk; ++ok) {
-            const uint32_t k = (uint32_t)(k0 + ok);
-            if (g.flags[row + k] & kNotFree) continue;
-            const float V = wij * g.wf[2][k];
-            s += V * ldf<TRes>(res, row + k);
+            if (fv[oi][oj][ok] & kNotFree) continue;
+            const float V = wij * wz[ok];
+            s += V * rv[oi][oj][ok];
             vs += V;
           }
         }
@@ -181,14 +205,29 @@ __global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, 
                                                          const uint8_t* cflags, const TCo* ec) {
   const uint32_t csx = cny * cnz, csy = cnz;
   HFSI_CELL_LOOP(g.n) {
-    if (g.flags[c] & kNotFree) continue;
     const uint32_t k = c % g.nz;
     const uint32_t t = c / g.nz;
     const uint32_t j = t % g.ny, i = t / g.ny;
+    const uint8_t f = g.flags[c];
+    const float efc = ldf<TF>(ef, c);
     const int32_t I[2] = {x.par[0][i], x.pnb[0][i]};
     const int32_t J[2] = {x.par[1][j], x.pnb[1][j]};
     const int32_t K[2] = {x.par[2][k], x.pnb[2][k]};
     const float wI = x.pw[0][i], wJ = x.pw[1][j], wK = x.pw[2][k];
+    // the 8 coarse flags and values, loaded together
+    uint8_t cf[2][2][2];
+    float cv[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const uint32_t cc = (uint32_t)I[a] * csx + (uint32_t)J[b] * csy + (uint32_t)K[d];
+          cf[a][b][d] = cflags[cc];
+          cv[a][b][d] = ldf<TCo>(ec, cc);
+        }
+    if (f & kNotFree) continue;
     const float wi[2] = {wI, 1.0f - wI}, wj[2] = {wJ, 1.0f - wJ}, wk[2] = {wK, 1.0f - wK};
     float s = 0.0f, ws = 0.0f;
 #pragma unroll
@@ -199,12 +238,11 @@ __global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, 
         for (int d = 0; d < 2; ++d) {
           const float w = wi[a] * wj[b] * wk[d];
           if (w == 0.0f) continue;
-          const uint32_t cc = (uint32_t)I[a] * csx + (uint32_t)J[b] * csy + (uint32_t)K[d];
-          if (cflags[cc] & kNotFree) continue;
-          s += w * ldf<TCo>(ec, cc);
+          if (cf[a][b][d] & kNotFree) continue;
+          s += w * cv[a][b][d];
           ws += w;
         }
-    if (ws > 0.0f) stf<TF>(ef, c, ldf<TF>(ef, c) + s / ws);
+    if (ws > 0.0f) stf<TF>(ef, c, efc + s / ws);
   }
 }
 
@@ -268,12 +306,27 @@ struct Pair<float> {
   __device__ static inline raw zero() { return make_float2(0.0f, 0.0f); }
 };
 
+// A thread's cells are fixed pairs of adjacent k cells ("slots"): the first
+// colour uses slots covering the tile plus a halo of 1 (18 rows x 33 pairs),
+// the second colour slots covering the tile (16 rows x 32 pairs); the colour
+// of a plane picks one cell of each pair.  A slot packs its LDS row, its
+// first LDS column, the parity of its grid row and whether each of its two
+// cells lies in the grid; `off` is the in-plane grid offset of its first cell.
+struct SweepSlot {
+  uint32_t bits;  // row | col << 5 | jpar << 12 | in0 << 13 | in1 << 14 | (k0 == 0) << 15 | (k1 == 0) << 16
+  uint32_t off;   // grid offset of the first cell within its plane (modulo 2^32 when k = -1)
+};
+
 template <typename TE, typename TR, bool FROMZERO>
 __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const TE* __restrict__ ein,
                                                            TE* __restrict__ eout, const TR* __restrict__ rhs,
                                                            const double* rs_dev, int c0, SweepGeo sg) {
   __shared__ TE E[5][kSwPlane];
+  __shared__ float cy[2][kSwRows], cz[2][kSwCols];  // y / z coefficients of the tile (0 outside the grid)
   using P = Pair<TE>;
+  constexpr int kPairsA = (kSwTK + 2) / 2, kPairsB = kSwTK / 2;
+  constexpr int kA = (kSwA + kSwThreads - 1) / kSwThreads;  // first-colour slots per thread
+  constexpr int kB = (kSwB + kSwThreads - 1) / kSwThreads;  // second-colour slots per thread
   const uint32_t G = gridDim.x;
   const uint32_t b = blockIdx.x;
   const uint32_t L = (b % HFSI_NXCD) * (G / HFSI_NXCD) + b / HFSI_NXCD;
@@ -282,22 +335,86 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
   const int jb = (int)((L / sg.kblocks) % sg.jblocks);
   const int ib = (int)(L / (sg.kblocks * sg.jblocks));
   const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+  const uint32_t sxy = g.ny * g.nz;
   const int j0 = jb * kSwTJ - kSwHalo, k0 = kb * kSwTK - kSwHalo;  // LDS (0, 0) in grid coordinates
   const int ia = ib * (int)sg.ichunk, iend = min(nx, ia + (int)sg.ichunk);
   const float rs = rhs_scale(rs_dev);
   const int tid = threadIdx.x;
+  for (int t = tid; t < kSwRows + kSwCols; t += kSwThreads) {
+    if (t < kSwRows) {
+      const int j = j0 + t;
+      const bool in = j >= 0 && j < ny;
+      cy[0][t] = in ? g.cl[1][j] : 0.0f;
+      cy[1][t] = in ? g.ch[1][j] : 0.0f;
+    } else {
+      const int kk = t - kSwRows, k = k0 + kk;
+      const bool in = k >= 0 && k < nz;
+      cz[0][kk] = in ? g.cl[2][k] : 0.0f;
+      cz[1][kk] = in ? g.ch[2][k] : 0.0f;
+    }
+  }
+  auto make_slot = [&](int r, int col) -> SweepSlot {
+    const int j = j0 + r, k = k0 + col;
+    const bool jin = j >= 0 && j < ny;
+    const bool in0 = jin && k >= 0 && k < nz, in1 = jin && k + 1 >= 0 && k + 1 < nz;
+    SweepSlot q;
+    q.bits = (uint32_t)r | ((uint32_t)col << 5) | ((uint32_t)(j & 1) << 12) | ((uint32_t)in0 << 13) |
+             ((uint32_t)in1 << 14) | ((uint32_t)(k == 0) << 15) | ((uint32_t)(k + 1 == 0) << 16);
+    q.off = jin ? (uint32_t)(j * nz + k) : 0u;
+    return q;
+  };
+  SweepSlot sa[kA], sb[kB];
+#pragma unroll
+  for (int m = 0; m < kA; ++m) {
+    const int t = tid + m * kSwThreads;
+    sa[m] = t < kSwA ? make_slot(1 + t / kPairsA, 1 + 2 * (t % kPairsA)) : SweepSlot{0u, 0u};
+  }
+#pragma unroll
+  for (int m = 0; m < kB; ++m) {
+    const int t = tid + m * kSwThreads;
+    sb[m] = t < kSwB ? make_slot(kSwHalo + t / kPairsB, kSwHalo + 2 * (t % kPairsB)) : SweepSlot{0u, 0u};
+  }
+  // the cell of a slot on plane p for the colour `color`: d = 0 / 1 picks the pair's first / second cell
+  auto pick = [&](const SweepSlot& q, int p, int color) -> int {
+    const int kpar = ((q.bits >> 5) & 1) ^ (k0 & 1);  // parity of the first cell's k (k0 is even)
+    return ((color + p + (int)((q.bits >> 12) & 1) + kpar) & 1);  // 1: the colour is on the second cell
+  };
+  auto in_grid = [](const SweepSlot& q, int d) -> bool { return (q.bits >> (13 + d)) & 1u; };
 
-  // 2-element loads of plane p into registers (0 outside the grid)
-  typename P::raw pre[kSwLoadIters];
-  auto fetch = [&](int p) {
+  // flags and right-hand sides of this step's cells, of the next step's and of the one after
+  uint32_t fa[kA], fb[kB], fa_n[kA], fb_n[kB], fa_nn[kA], fb_nn[kB];
+  float ra[kA], rb[kB], ra_n[kA], rb_n[kB], ra_nn[kA], rb_nn[kB];
+  auto fetch_cells = [&](int pa, int pb, uint32_t* fA, float* rA, uint32_t* fB, float* rB) {
+#pragma unroll
+    for (int m = 0; m < kA; ++m) {
+      const int d = pick(sa[m], pa, c0);
+      const bool ok = pa >= 0 && pa < nx && in_grid(sa[m], d);
+      const uint32_t c = (uint32_t)pa * sxy + sa[m].off + (uint32_t)d;
+      fA[m] = ok ? g.flags[c] : (uint32_t)kNotFree;
+      rA[m] = ok ? ldf<TR>(rhs, c) : 0.0f;
+    }
+#pragma unroll
+    for (int m = 0; m < kB; ++m) {
+      const int d = pick(sb[m], pb, 1 - c0);
+      const bool ok = pb >= 0 && pb < nx && in_grid(sb[m], d);
+      const uint32_t c = (uint32_t)pb * sxy + sb[m].off + (uint32_t)d;
+      fB[m] = ok ? g.flags[c] : (uint32_t)kNotFree;
+      rB[m] = ok ? ldf<TR>(rhs, c) : 0.0f;
+    }
+  };
+
+  // 2-element loads of plane p into registers (0 outside the grid); planes are fetched two steps
+  // before they go to LDS, so a load has two steps of work to arrive
+  typename P::raw pre[kSwLoadIters], pre_n[kSwLoadIters];
+  auto fetch = [&](int p, typename P::raw* dst) {
 #pragma unroll
     for (int m = 0; m < kSwLoadIters; ++m) {
       const int t = tid + m * kSwThreads;
-      pre[m] = P::zero();
+      dst[m] = P::zero();
       if (FROMZERO || t >= kSwPairs || p < 0 || p >= nx) continue;
       const int r = t / (kSwCols / 2), k = k0 + 2 * (t % (kSwCols / 2)), j = j0 + r;
       if (j < 0 || j >= ny || k < 0 || k >= nz) continue;
-      pre[m] = P::load(ein + ((size_t)p * ny + j) * nz + k);
+      dst[m] = P::load(ein + ((size_t)p * ny + j) * nz + k);
     }
   };
   auto slot = [](int p) { return ((p % 5) + 5) % 5; };
@@ -309,24 +426,36 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
       if (t < kSwPairs) P::put(pl + 2 * t, pre[m]);
     }
   };
-  // one Gauss-Seidel update of cell (i, j, k) from LDS planes (xm, x, xp); v stays 0 on cells that are not free
-  auto update = [&](int i, int j, int k, const TE* xm, const TE* x, const TE* xp, int li) -> float {
-    const uint32_t c = ((uint32_t)i * g.ny + (uint32_t)j) * g.nz + (uint32_t)k;
-    const uint8_t f = g.flags[c];
-    const float r = ldf<TR>(rhs, c);
+  // one Gauss-Seidel update of the cell (row r, column kk) from LDS planes (xm, x, xp); 0 when not free
+  auto update = [&](int r, int kk, bool kzero, uint32_t f, float rv, float cxl, float cxh, const TE* xm,
+                    const TE* x, const TE* xp) -> float {
     float v = 0.0f;
     if (!(f & kNotFree)) {
+      const int li = r * kSwCols + kk;
       float nb = 0.0f, den = 0.0f;
-      if (f & kXm) { const float q = g.cl[0][i]; nb += q * to_c_exact(xm[li]); den += q; }
-      if (f & kXp) { const float q = g.ch[0][i]; nb += q * to_c_exact(xp[li]); den += q; }
-      if (f & kYm) { const float q = g.cl[1][j]; nb += q * to_c_exact(x[li - kSwCols]); den += q; }
-      if (f & kYp) { const float q = g.ch[1][j]; nb += q * to_c_exact(x[li + kSwCols]); den += q; }
-      if (f & kZm) { const float q = g.cl[2][k]; nb += q * to_c_exact(x[li - 1]); den += q; }
-      if (f & kZp) { const float q = g.ch[2][k]; nb += q * to_c_exact(x[li + 1]); den += q; }
-      if (g.outlet && k == 0) den += g.outlet_add;
-      if (den > 0.0f) v = (nb + rs * r) / den;
+      if (f & kXm) { nb += cxl * to_c_exact(xm[li]); den += cxl; }
+      if (f & kXp) { nb += cxh * to_c_exact(xp[li]); den += cxh; }
+      if (f & kYm) { const float c = cy[0][r]; nb += c * to_c_exact(x[li - kSwCols]); den += c; }
+      if (f & kYp) { const float c = cy[1][r]; nb += c * to_c_exact(x[li + kSwCols]); den += c; }
+      if (f & kZm) { const float c = cz[0][kk]; nb += c * to_c_exact(x[li - 1]); den += c; }
+      if (f & kZp) { const float c = cz[1][kk]; nb += c * to_c_exact(x[li + 1]); den += c; }
+      if (g.outlet && kzero) den += g.outlet_add;
+      if (den > 0.0f) v = (nb + rs * rv) / den;
     }
     return v;
+  };
+  auto run_colour = [&](const SweepSlot* sl, int ns, const uint32_t* fl, const float* rv, int p, int color) {
+    TE* x = E[slot(p)];
+    const TE* xm = E[slot(p - 1)];
+    const TE* xp = E[slot(p + 1)];
+    const float cxl = g.cl[0][p], cxh = g.ch[0][p];
+    for (int m = 0; m < ns; ++m) {
+      const int d = pick(sl[m], p, color);
+      if (!in_grid(sl[m], d)) continue;
+      const int r = (int)(sl[m].bits & 31u), kk = (int)((sl[m].bits >> 5) & 127u) + d;
+      const bool kzero = (sl[m].bits >> (15 + d)) & 1u;
+      x[r * kSwCols + kk] = (TE)update(r, kk, kzero, fl[m], rv[m], cxl, cxh, xm, x, xp);
+    }
   };
   auto store_plane = [&](int p) {
     const TE* pl = E[slot(p)];
@@ -338,224 +467,40 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
     }
   };
 
-  // prologue: planes ia - 2 and ia - 1 into LDS, plane ia into registers
-  fetch(ia - 2);
+  // prologue: planes ia - 2 and ia - 1 into LDS, planes ia and ia + 1 into registers, the cells of the
+  // first two steps
+  fetch(ia - 2, pre);
   put(ia - 2);
-  fetch(ia - 1);
+  fetch(ia - 1, pre);
   put(ia - 1);
-  fetch(ia);
+  fetch(ia, pre);
+  fetch(ia + 1, pre_n);
+  fetch_cells(ia - 1, ia - 2, fa, ra, fb, rb);
+  fetch_cells(ia, ia - 1, fa_n, ra_n, fb_n, rb_n);
   for (int i = ia - 1; i <= iend; ++i) {
     put(i + 1);
-    if (i + 2 <= iend + 1) fetch(i + 2);
-    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kSwLoadIters; ++m) pre[m] = pre_n[m];
+    if (i + 3 <= iend + 1) fetch(i + 3, pre_n);
+    if (i + 2 <= iend) fetch_cells(i + 2, i + 1, fa_nn, ra_nn, fb_nn, rb_nn);
+    lds_barrier();
     if (i - 2 >= ia) store_plane(i - 2);
-    if (i >= 0 && i < nx) {
-      // first colour on plane i, tile + 1
-      TE* x = E[slot(i)];
-      const TE* xm = E[slot(i - 1)];
-      const TE* xp = E[slot(i + 1)];
-      for (int t = tid; t < kSwA; t += kSwThreads) {
-        const int r = 1 + t / ((kSwTK + 2) / 2);
-        const int j = j0 + r;
-        const int par = (c0 + i + j) & 1;  // k parity of the colour in this row (k0 is even)
-        const int kk = 2 * (t % ((kSwTK + 2) / 2)) + (par ? 1 : 2);
-        const int k = k0 + kk;
-        if (j < 0 || j >= ny || k < 0 || k >= nz) continue;
-        const int li = r * kSwCols + kk;
-        x[li] = (TE)update(i, j, k, xm, x, xp, li);
-      }
+    if (i >= 0 && i < nx) run_colour(sa, kA, fa, ra, i, c0);  // first colour on plane i, tile + 1
+    lds_barrier();
+    if (i - 1 >= ia && i - 1 < iend) run_colour(sb, kB, fb, rb, i - 1, 1 - c0);  // second colour, plane i - 1
+#pragma unroll
+    for (int m = 0; m < kA; ++m) {
+      fa[m] = fa_n[m]; ra[m] = ra_n[m];
+      fa_n[m] = fa_nn[m]; ra_n[m] = ra_nn[m];
     }
-    __syncthreads();
-    const int ib2 = i - 1;
-    if (ib2 >= ia && ib2 < iend) {
-      // second colour on plane i - 1, tile
-      TE* x = E[slot(ib2)];
-      const TE* xm = E[slot(ib2 - 1)];
-      const TE* xp = E[slot(ib2 + 1)];
-      for (int t = tid; t < kSwB; t += kSwThreads) {
-        const int r = kSwHalo + t / (kSwTK / 2);
-        const int j = j0 + r;
-        const int par = (1 - c0 + ib2 + j) & 1;
-        const int kk = kSwHalo + 2 * (t % (kSwTK / 2)) + par;
-        const int k = k0 + kk;
-        if (j >= ny || k >= nz) continue;
-        const int li = r * kSwCols + kk;
-        x[li] = (TE)update(ib2, j, k, xm, x, xp, li);
-      }
+#pragma unroll
+    for (int m = 0; m < kB; ++m) {
+      fb[m] = fb_n[m]; rb[m] = rb_n[m];
+      fb_n[m] = fb_nn[m]; rb_n[m] = rb_nn[m];
     }
   }
-  __syncthreads();
+  lds_barrier();
   if (iend - 1 >= ia) store_plane(iend - 1);
-}
-
-// ---------------------------------------------------------------------------
-// Row kernels (nz % 8 == 0): a thread owns 8 consecutive cells k0..k0+7 of
-// one (i, j) row, loads them and the four neighbouring rows as 8-vectors
-// (16 B in fp16), keeps the x / y coefficients per thread and computes the
-// index of the row once.
-// ---------------------------------------------------------------------------
-
-template <typename T>
-struct Vec8;
-template <>
-struct Vec8<__half> {
-  __device__ static inline void load(const __half* p, float v[8]) {
-    const uint4 raw = *reinterpret_cast<const uint4*>(p);
-    const __half2* h = reinterpret_cast<const __half2*>(&raw);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float2 f = __half22float2(h[q]);
-      v[2 * q] = f.x;
-      v[2 * q + 1] = f.y;
-    }
-  }
-  __device__ static inline void store(__half* p, const float v[8]) {
-    uint4 raw;
-    __half2* h = reinterpret_cast<__half2*>(&raw);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) h[q] = __floats2half2_rn(v[2 * q], v[2 * q + 1]);
-    *reinterpret_cast<uint4*>(p) = raw;
-  }
-};
-template <>
-struct Vec8<float> {
-  __device__ static inline void load(const float* p, float v[8]) {
-    const float4 a = *reinterpret_cast<const float4*>(p);
-    const float4 b = *reinterpret_cast<const float4*>(p + 4);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-  }
-  __device__ static inline void store(float* p, const float v[8]) {
-    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-    *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
-  }
-};
-
-__device__ inline void load_flags8(const uint8_t* p, uint8_t f[8]) {
-  const uint2 raw = *reinterpret_cast<const uint2*>(p);
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    f[m] = (uint8_t)(raw.x >> (8 * m));
-    f[4 + m] = (uint8_t)(raw.y >> (8 * m));
-  }
-}
-
-// Row form of k_mix_rb.  MODE 0: colour pass; MODE 1: colour pass from zero;
-// MODE 2: residual rs * rhs - L e of all 8 cells into `res`.
-template <typename TE, typename TR, int MODE>
-__global__ void __launch_bounds__(kBlock) k_mix_rows(OpGeo<float> g, TE* e, const TR* rhs, const double* rs_dev,
-                                                      int color, TE* res) {
-  const uint32_t kc = g.nz >> 3;
-  const uint32_t nthr = g.nx * g.ny * kc;
-  const float rs = rhs_scale(rs_dev);
-  HFSI_CELL_LOOP(nthr) {
-    const uint32_t q = c % kc;
-    const uint32_t row = c / kc;
-    const uint32_t j = row % g.ny, i = row / g.ny;
-    const uint32_t k0 = q * 8;
-    const uint32_t c0 = row * g.nz + k0;
-    uint8_t f[8];
-    load_flags8(g.flags + c0, f);
-    float rr[8];
-    Vec8<TR>::load(rhs + c0, rr);
-    float ec[8], xm[8], xp[8], ym[8], yp[8];
-    float kprev = 0.0f, knext = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) ec[m] = xm[m] = xp[m] = ym[m] = yp[m] = 0.0f;
-    if (MODE != 1) {
-      Vec8<TE>::load(e + c0, ec);
-      if (i > 0) Vec8<TE>::load(e + c0 - g.sx, xm);
-      if (i + 1 < g.nx) Vec8<TE>::load(e + c0 + g.sx, xp);
-      if (j > 0) Vec8<TE>::load(e + c0 - g.sy, ym);
-      if (j + 1 < g.ny) Vec8<TE>::load(e + c0 + g.sy, yp);
-      if (k0 > 0) kprev = ldf<TE>(e, c0 - 1);
-      if (k0 + 8 < g.nz) knext = ldf<TE>(e, c0 + 8);
-    }
-    const float cxl = g.cl[0][i], cxh = g.ch[0][i], cyl = g.cl[1][j], cyh = g.ch[1][j];
-    float out[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      out[m] = ec[m];
-      const bool mine = MODE == 2 || (((i + j + k0 + m) & 1u) == (uint32_t)color);
-      if (!mine) continue;
-      const uint8_t fm = f[m];
-      float v = 0.0f;
-      if (!(fm & kNotFree)) {
-        const uint32_t k = k0 + m;
-        const float zl = m == 0 ? kprev : ec[m - 1];
-        const float zh = m == 7 ? knext : ec[m + 1];
-        float nb = 0.0f, den = 0.0f;
-        if (fm & kXm) { nb += cxl * xm[m]; den += cxl; }
-        if (fm & kXp) { nb += cxh * xp[m]; den += cxh; }
-        if (fm & kYm) { nb += cyl * ym[m]; den += cyl; }
-        if (fm & kYp) { nb += cyh * yp[m]; den += cyh; }
-        if (fm & kZm) { const float cz = g.cl[2][k]; nb += cz * zl; den += cz; }
-        if (fm & kZp) { const float cz = g.ch[2][k]; nb += cz * zh; den += cz; }
-        if (g.outlet && k == 0) den += g.outlet_add;
-        if (MODE == 2)
-          v = rs * rr[m] - (den * ec[m] - nb);
-        else if (den > 0.0f)
-          v = (nb + rs * rr[m]) / den;
-      }
-      out[m] = v;
-    }
-    if (MODE == 2)
-      Vec8<TE>::store(res + c0, out);
-    else
-      Vec8<TE>::store(e + c0, out);
-  }
-}
-
-// Row form of k_mix_prolong (fine level with nz % 8 == 0).
-template <typename TF, typename TCo>
-__global__ void __launch_bounds__(kBlock) k_mix_prolong_rows(OpGeo<float> g, TF* ef, MixXfer x, uint32_t cny,
-                                                              uint32_t cnz, const uint8_t* cflags, const TCo* ec) {
-  const uint32_t kc = g.nz >> 3;
-  const uint32_t nthr = g.nx * g.ny * kc;
-  const uint32_t csx = cny * cnz, csy = cnz;
-  HFSI_CELL_LOOP(nthr) {
-    const uint32_t q = c % kc;
-    const uint32_t row = c / kc;
-    const uint32_t j = row % g.ny, i = row / g.ny;
-    const uint32_t k0 = q * 8;
-    const uint32_t c0 = row * g.nz + k0;
-    uint8_t f[8];
-    load_flags8(g.flags + c0, f);
-    bool any = false;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) any |= !(f[m] & kNotFree);
-    if (!any) continue;
-    float v[8];
-    Vec8<TF>::load(ef + c0, v);
-    const uint32_t I[2] = {(uint32_t)x.par[0][i], (uint32_t)x.pnb[0][i]};
-    const uint32_t J[2] = {(uint32_t)x.par[1][j], (uint32_t)x.pnb[1][j]};
-    const float wI = x.pw[0][i], wJ = x.pw[1][j];
-    const float wab[4] = {wI * wJ, wI * (1.0f - wJ), (1.0f - wI) * wJ, (1.0f - wI) * (1.0f - wJ)};
-    const uint32_t rb[4] = {I[0] * csx + J[0] * csy, I[0] * csx + J[1] * csy, I[1] * csx + J[0] * csy,
-                            I[1] * csx + J[1] * csy};
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      if (f[m] & kNotFree) continue;
-      const uint32_t k = k0 + m;
-      const uint32_t K[2] = {(uint32_t)x.par[2][k], (uint32_t)x.pnb[2][k]};
-      const float wK = x.pw[2][k];
-      const float wk[2] = {wK, 1.0f - wK};
-      float s = 0.0f, ws = 0.0f;
-#pragma unroll
-      for (int ab = 0; ab < 4; ++ab)
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          const float w = wab[ab] * wk[d];
-          if (w == 0.0f) continue;
-          const uint32_t cc = rb[ab] + K[d];
-          if (cflags[cc] & kNotFree) continue;
-          s += w * ldf<TCo>(ec, cc);
-          ws += w;
-        }
-      if (ws > 0.0f) v[m] += s / ws;
-    }
-    Vec8<TF>::store(ef + c0, v);
-  }
 }
 
 // ---------------------------------------------------------------------------

@@ -46,6 +46,16 @@ void set_error(const std::string& msg);
   } while (0)
 
 // ---------------------------------------------------------------------------
+// Workgroup barrier for LDS-only exchange.  __syncthreads() also waits for
+// every outstanding global load and store of the wave (it is a workgroup
+// memory fence), which defeats loads issued a step ahead of their use; this
+// barrier waits only for the wave's LDS accesses.  Use it only where the
+// threads of the workgroup exchange data through LDS alone.
+// ---------------------------------------------------------------------------
+
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// ---------------------------------------------------------------------------
 // XCD-aware chunk iteration
 // ---------------------------------------------------------------------------
 

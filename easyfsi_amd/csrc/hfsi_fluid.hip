@@ -1085,8 +1085,7 @@ struct hfsi_fluid {
   int mix_tail_first = 0;
   double mix_kappa = 1.0;
   uint32_t mix_half_min_cells = 1u << 20;  // levels with at least this many cells store fp16
-  bool mix_row_kernels = false;            // 8-cell row kernels where nz % 8 == 0
-  bool mix_fused_sweeps = true;            // fused two-colour marching sweeps (k_mix_sweep)
+  uint32_t mix_fused_min_cells = 1u << 23;  // fused two-colour sweeps (k_mix_sweep) on levels this large
   hfsi::MixLevel ml[hfsi::kMixMaxLevels];
   void* mix_tail_args = nullptr;  // MixTailArgs (device)
   int mix_tail_outlet = -1;       // outlet switch the tail arguments were written for
@@ -1584,13 +1583,12 @@ static int write_mix_tail_args(hfsi_fluid* h, int outlet, hipStream_t st) {
   return HFSI_OK;
 }
 
-// Row kernels (8 cells of one row per thread, vector loads) on levels whose
-// rows split into 8-cell groups; the flat kernels elsewhere.
-static inline bool mix_rows(const hfsi_fluid* h, const MixLevel& m) { return h->mix_row_kernels && m.nz % 8 == 0; }
-
 // Fused two-colour sweeps (k_mix_sweep) on grid-launched levels with a second
-// iterate buffer (nz even); colour passes (k_mix_rb / k_mix_rows) elsewhere.
-static inline bool mix_fused(const hfsi_fluid* h, const MixLevel& m) { return h->mix_fused_sweeps && m.e2 != nullptr; }
+// iterate buffer (nz even) and at least mix_fused_min_cells cells; one launch
+// per colour pass (k_mix_rb) elsewhere.
+static inline bool mix_fused(const hfsi_fluid* h, const MixLevel& m) {
+  return m.e2 != nullptr && m.n >= h->mix_fused_min_cells;
+}
 
 static SweepGeo mix_sweep_geo(const MixLevel& m) {
   SweepGeo sg;
@@ -1609,24 +1607,13 @@ static void mix_rb_pass(hfsi_fluid* h, int l, int outlet, TE* e, const TR* rhs, 
                         bool from_zero, hipStream_t st) {
   const MixLevel& m = h->ml[l];
   const uint32_t npairs = m.nx * m.ny * ((m.nz + 1) / 2);
-  const uint32_t nrows8 = m.nx * m.ny * (m.nz / 8);
-  const bool rows = mix_rows(h, m);
   OpGeo<float> g = mix_geo(h, l, outlet);
   if (from_zero) {
-    if (rows)
-      hipLaunchKernelGGL((k_mix_rows<TE, TR, 1>), dim3(blocks_for(nrows8)), dim3(kBlock), 0, st, g, e, rhs, rs, color,
-                         nullptr);
-    else
-      hipLaunchKernelGGL((k_mix_rb<TE, TR, true>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs,
-                         color);
+    hipLaunchKernelGGL((k_mix_rb<TE, TR, true>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs, color);
   } else {
     if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
-    if (rows)
-      hipLaunchKernelGGL((k_mix_rows<TE, TR, 0>), dim3(blocks_for(nrows8)), dim3(kBlock), 0, st, g, e, rhs, rs, color,
-                         nullptr);
-    else
-      hipLaunchKernelGGL((k_mix_rb<TE, TR, false>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs,
-                         color);
+    hipLaunchKernelGGL((k_mix_rb<TE, TR, false>), dim3(blocks_for(npairs)), dim3(kBlock), 0, st, g, e, rhs, rs,
+                       color);
     if (l == 0) timer_mark(h, HFSI_TIMER_MIX_RB, st);
   }
 }
@@ -1673,12 +1660,8 @@ static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TR* rhs, const 
   const MixLevel& c = h->ml[l + 1];
   TE* e = (TE*)f.cur;
   OpGeo<float> g = mix_geo(h, l, outlet);
-  if (mix_rows(h, f))
-    hipLaunchKernelGGL((k_mix_rows<TE, TR, 2>), dim3(blocks_for(f.nx * f.ny * (f.nz / 8))), dim3(kBlock), 0, st, g,
-                       e, rhs, rs, 0, (TE*)f.res);
-  else
-    hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, (const TE*)e, rhs,
-                       rs, (TE*)f.res);
+  hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, (const TE*)e, rhs,
+                     rs, (TE*)f.res);
   hipLaunchKernelGGL((k_mix_restrict_res<TE, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, g, (const TE*)f.res,
                      mix_xfer(h, l), c.nx, c.ny, c.nz, (const uint8_t*)c.flags, (TC*)c.rhs);
 }
@@ -1688,13 +1671,8 @@ static void mix_prolong(hfsi_fluid* h, int l, int outlet, hipStream_t st) {
   const MixLevel& c = h->ml[l + 1];
   const MixLevel& f = h->ml[l];
   TF* ef = (TF*)f.cur;
-  if (mix_rows(h, f))
-    hipLaunchKernelGGL((k_mix_prolong_rows<TF, TCo>), dim3(blocks_for(f.nx * f.ny * (f.nz / 8))), dim3(kBlock), 0, st,
-                       mix_geo(h, l, outlet), ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags,
-                       (const TCo*)c.cur);
-  else
-    hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet),
-                       ef, mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags, (const TCo*)c.cur);
+  hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet), ef,
+                     mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags, (const TCo*)c.cur);
 }
 
 // Down-leg of level l (smoothing + restriction), with the storage types of l
@@ -2832,21 +2810,12 @@ int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells) {
   return HFSI_OK;
 }
 
-int hfsi_fluid_mixed_set_fused_sweeps(hfsi_fluid* h, int32_t enable) {
-  if (!h) {
-    set_error("hfsi_fluid_mixed_set_fused_sweeps: invalid arguments");
+int hfsi_fluid_mixed_set_fused_min_cells(hfsi_fluid* h, int64_t min_cells) {
+  if (!h || min_cells < 0) {
+    set_error("hfsi_fluid_mixed_set_fused_min_cells: invalid arguments");
     return HFSI_ERR_ARG;
   }
-  h->mix_fused_sweeps = enable != 0;
-  return HFSI_OK;
-}
-
-int hfsi_fluid_mixed_set_row_kernels(hfsi_fluid* h, int32_t enable) {
-  if (!h) {
-    set_error("hfsi_fluid_mixed_set_row_kernels: invalid arguments");
-    return HFSI_ERR_ARG;
-  }
-  h->mix_row_kernels = enable != 0;
+  h->mix_fused_min_cells = (uint32_t)std::min<int64_t>(min_cells, (int64_t)UINT32_MAX);
   return HFSI_OK;
 }
 

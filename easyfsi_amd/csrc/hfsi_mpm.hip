@@ -473,23 +473,29 @@ __global__ void k_bin_clear(BinArgs b) {
   }
 }
 
+// Key, rank and list pushes of every particle.  The two list pushes (an
+// out-of-bounds particle to `oob`, the first particle of a block to `active`)
+// are one atomic on per-lane selected pointers: written as two branches, the
+// ROCm 7.2 compiler tail-merged them into one block that used the active-list
+// pointers for both (out-of-bounds particles then went to the active list).
 __global__ void __launch_bounds__(kBlock) k_bin_count(MpmArgs a, BinArgs b) {
   const uint32_t n = (uint32_t)a.count;
   HFSI_CELL_LOOP(n) {
     float X[3];
     grid_coord(a, a.x + 3 * (size_t)c, X);
-    if (stencil_oob(a, X)) {
-      b.key[c] = -1;
-      b.oob[atomicAdd(b.noob, 1)] = (int32_t)c;
-      continue;
+    const bool out = stencil_oob(a, X);
+    int32_t k = -1, r = 0;
+    if (!out) {
+      const int bx = (int)floorf(X[0] - 0.5f) / kBinEdge, by = (int)floorf(X[1] - 0.5f) / kBinEdge,
+                bz = (int)floorf(X[2] - 0.5f) / kBinEdge;
+      k = (bx * b.nby + by) * b.nbz + bz;
+      r = atomicAdd(&b.count[k], 1);
+      b.rank[c] = r;
     }
-    const int bx = (int)floorf(X[0] - 0.5f) / kBinEdge, by = (int)floorf(X[1] - 0.5f) / kBinEdge,
-              bz = (int)floorf(X[2] - 0.5f) / kBinEdge;
-    const int32_t k = (bx * b.nby + by) * b.nbz + bz;
     b.key[c] = k;
-    const int32_t r = atomicAdd(&b.count[k], 1);
-    b.rank[c] = r;
-    if (r == 0) b.active[atomicAdd(b.nactive, 1)] = k;
+    int32_t* list = out ? b.oob : b.active;
+    int32_t* counter = out ? b.noob : b.nactive;
+    if (out || r == 0) list[atomicAdd(counter, 1)] = out ? (int32_t)c : k;
   }
 }
 

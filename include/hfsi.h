@@ -270,15 +270,11 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* particles, int32_t coun
 int hfsi_fluid_mixed_levels(const hfsi_fluid* h, int32_t* half_levels);
 int hfsi_fluid_mixed_level_shape(const hfsi_fluid* h, int32_t level, int32_t* shape3);
 int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells);
-/* Row kernels (8 cells of one row per thread, 16-byte loads) for the smoother,
- * residual and prolongation of levels with nz % 8 == 0 (default off: slower
- * than the cell-per-thread kernels on MI355X); both compute the same
- * arithmetic. */
-int hfsi_fluid_mixed_set_row_kernels(hfsi_fluid* h, int32_t enable);
-/* Fused sweeps (default on): both colour passes of a red-black sweep in one
- * marching pass over memory, on grid-launched levels with nz even; 0 selects
- * one launch per colour pass.  Both give identical results. */
-int hfsi_fluid_mixed_set_fused_sweeps(hfsi_fluid* h, int32_t enable);
+/* Fused sweeps: both colour passes of a red-black sweep in one marching pass
+ * over memory, on grid-launched levels with nz even and at least min_cells
+ * cells (default 2^23); other levels run one launch per colour pass.  Both
+ * forms give identical results. */
+int hfsi_fluid_mixed_set_fused_min_cells(hfsi_fluid* h, int64_t min_cells);
 
 /* ---- learned pressure initial guess (no reference counterpart) -----------
  * A 3-level 3D U-Net (3x3x3 convolutions, channels 16 / 32 / 64, two per
