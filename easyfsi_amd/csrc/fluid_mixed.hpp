@@ -70,6 +70,30 @@ struct MixXfer {
 // when rs_dev is set, else 1).
 __device__ inline float rhs_scale(const double* rs_dev) { return rs_dev ? (float)(*rs_dev) : 1.0f; }
 
+// Gauss-Seidel value of one cell, (sum_open c e_nb + rs rhs) / sum_open c
+// (+ the outlet term), 0 on cells that are not free.  The colour passes and
+// the fused sweep both call it, with explicit fused multiply-adds, so their
+// results agree bit for bit whatever the surrounding code looks like (closed
+// faces are skipped, their neighbour values are never used).
+__device__ inline float rb_value(uint32_t f, float cxl, float cxh, float cyl, float cyh, float czl, float czh,
+                                 float exm, float exp_, float eym, float eyp, float ezm, float ezp, float den_extra,
+                                 float rs, float r) {
+  float nb = 0.0f, den = 0.0f;
+  if (f & kXm) { nb = fmaf(cxl, exm, nb); den += cxl; }
+  if (f & kXp) { nb = fmaf(cxh, exp_, nb); den += cxh; }
+  if (f & kYm) { nb = fmaf(cyl, eym, nb); den += cyl; }
+  if (f & kYp) { nb = fmaf(cyh, eyp, nb); den += cyh; }
+  if (f & kZm) { nb = fmaf(czl, ezm, nb); den += czl; }
+  if (f & kZp) { nb = fmaf(czh, ezp, nb); den += czh; }
+  den += den_extra;
+  // hardware reciprocal (1 ulp) rather than an IEEE division: the smoother needs no exact quotient
+  float v = fmaf(rs, r, nb) * __builtin_amdgcn_rcpf(den > 0.0f ? den : 1.0f);
+  // keep the float32 rounding of v: otherwise the compiler may fuse the product with the fp16 store
+  // conversion of one caller (one rounding, v_fma_mixlo_f16) and not of the other
+  asm("" : "+v"(v));
+  return (!(f & kNotFree) && den > 0.0f) ? v : 0.0f;
+}
+
 // One colour pass of red-black Gauss-Seidel on L e = rhs, L = den e - nb.
 // A thread owns the k-pair (2kk, 2kk+1) of row (i, j) and updates the cell of
 // the pass colour, (i + j + k) % 2 == color.  FROMZERO: the iterate starts at
@@ -95,15 +119,17 @@ __global__ void __launch_bounds__(kBlock) k_mix_rb(OpGeo<float> g, TE* e, const 
     const uint8_t f = g.flags[cc];
     float v = 0.0f;
     if (!(f & kNotFree)) {
-      float nb = 0.0f, den = 0.0f;
-      if (f & kXm) { const float q = g.cl[0][i]; if (!FROMZERO) nb += q * ldf<TE>(e, cc - g.sx); den += q; }
-      if (f & kXp) { const float q = g.ch[0][i]; if (!FROMZERO) nb += q * ldf<TE>(e, cc + g.sx); den += q; }
-      if (f & kYm) { const float q = g.cl[1][j]; if (!FROMZERO) nb += q * ldf<TE>(e, cc - g.sy); den += q; }
-      if (f & kYp) { const float q = g.ch[1][j]; if (!FROMZERO) nb += q * ldf<TE>(e, cc + g.sy); den += q; }
-      if (f & kZm) { const float q = g.cl[2][k]; if (!FROMZERO) nb += q * ldf<TE>(e, cc - 1); den += q; }
-      if (f & kZp) { const float q = g.ch[2][k]; if (!FROMZERO) nb += q * ldf<TE>(e, cc + 1); den += q; }
-      if (g.outlet && k == 0) den += g.outlet_add;
-      if (den > 0.0f) v = (nb + rs * ldf<TR>(rhs, cc)) / den;
+      float exm = 0.0f, exp_ = 0.0f, eym = 0.0f, eyp = 0.0f, ezm = 0.0f, ezp = 0.0f;
+      if (!FROMZERO) {
+        if (f & kXm) exm = ldf<TE>(e, cc - g.sx);
+        if (f & kXp) exp_ = ldf<TE>(e, cc + g.sx);
+        if (f & kYm) eym = ldf<TE>(e, cc - g.sy);
+        if (f & kYp) eyp = ldf<TE>(e, cc + g.sy);
+        if (f & kZm) ezm = ldf<TE>(e, cc - 1);
+        if (f & kZp) ezp = ldf<TE>(e, cc + 1);
+      }
+      v = rb_value(f, g.cl[0][i], g.ch[0][i], g.cl[1][j], g.ch[1][j], g.cl[2][k], g.ch[2][k], exm, exp_, eym, eyp,
+                   ezm, ezp, (g.outlet && k == 0) ? g.outlet_add : 0.0f, rs, ldf<TR>(rhs, cc));
     }
     stf<TE>(e, cc, v);
   }
@@ -381,40 +407,50 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
   };
   auto in_grid = [](const SweepSlot& q, int d) -> bool { return (q.bits >> (13 + d)) & 1u; };
 
-  // flags and right-hand sides of this step's cells, of the next step's and of the one after
-  uint32_t fa[kA], fb[kB], fa_n[kA], fb_n[kB], fa_nn[kA], fb_nn[kB];
-  float ra[kA], rb[kB], ra_n[kA], rb_n[kB], ra_nn[kA], rb_nn[kB];
+  // flags and right-hand sides of this step's cells and of the next step's
+  uint32_t fa[kA], fb[kB], fa_n[kA], fb_n[kB];
+  float ra[kA], rb[kB], ra_n[kA], rb_n[kB];
+  // unconditional loads from a clamped index (cell 0 for cells outside the grid), then a select: no
+  // branches around the loads
   auto fetch_cells = [&](int pa, int pb, uint32_t* fA, float* rA, uint32_t* fB, float* rB) {
 #pragma unroll
     for (int m = 0; m < kA; ++m) {
       const int d = pick(sa[m], pa, c0);
       const bool ok = pa >= 0 && pa < nx && in_grid(sa[m], d);
-      const uint32_t c = (uint32_t)pa * sxy + sa[m].off + (uint32_t)d;
-      fA[m] = ok ? g.flags[c] : (uint32_t)kNotFree;
-      rA[m] = ok ? ldf<TR>(rhs, c) : 0.0f;
+      const uint32_t c = ok ? (uint32_t)pa * sxy + sa[m].off + (uint32_t)d : 0u;
+      const uint32_t f = g.flags[c];
+      const float r = ldf<TR>(rhs, c);
+      fA[m] = ok ? f : (uint32_t)kNotFree;
+      rA[m] = ok ? r : 0.0f;
     }
 #pragma unroll
     for (int m = 0; m < kB; ++m) {
       const int d = pick(sb[m], pb, 1 - c0);
       const bool ok = pb >= 0 && pb < nx && in_grid(sb[m], d);
-      const uint32_t c = (uint32_t)pb * sxy + sb[m].off + (uint32_t)d;
-      fB[m] = ok ? g.flags[c] : (uint32_t)kNotFree;
-      rB[m] = ok ? ldf<TR>(rhs, c) : 0.0f;
+      const uint32_t c = ok ? (uint32_t)pb * sxy + sb[m].off + (uint32_t)d : 0u;
+      const uint32_t f = g.flags[c];
+      const float r = ldf<TR>(rhs, c);
+      fB[m] = ok ? f : (uint32_t)kNotFree;
+      rB[m] = ok ? r : 0.0f;
     }
   };
 
-  // 2-element loads of plane p into registers (0 outside the grid); planes are fetched two steps
-  // before they go to LDS, so a load has two steps of work to arrive
-  typename P::raw pre[kSwLoadIters], pre_n[kSwLoadIters];
+  // 2-element loads of plane p into registers (0 outside the grid), a step before they go to LDS; the
+  // in-plane offset of each of this thread's pairs is computed once (-1: outside the grid)
+  int32_t poff[kSwLoadIters];
+#pragma unroll
+  for (int m = 0; m < kSwLoadIters; ++m) {
+    const int t = tid + m * kSwThreads;
+    const int r = t / (kSwCols / 2), k = k0 + 2 * (t % (kSwCols / 2)), j = j0 + r;
+    poff[m] = (t < kSwPairs && j >= 0 && j < ny && k >= 0 && k < nz) ? j * nz + k : -1;
+  }
+  typename P::raw pre[kSwLoadIters];
   auto fetch = [&](int p, typename P::raw* dst) {
 #pragma unroll
     for (int m = 0; m < kSwLoadIters; ++m) {
-      const int t = tid + m * kSwThreads;
       dst[m] = P::zero();
-      if (FROMZERO || t >= kSwPairs || p < 0 || p >= nx) continue;
-      const int r = t / (kSwCols / 2), k = k0 + 2 * (t % (kSwCols / 2)), j = j0 + r;
-      if (j < 0 || j >= ny || k < 0 || k >= nz) continue;
-      dst[m] = P::load(ein + ((size_t)p * ny + j) * nz + k);
+      if (FROMZERO || poff[m] < 0 || p < 0 || p >= nx) continue;
+      dst[m] = P::load(ein + (size_t)p * sxy + (size_t)poff[m]);
     }
   };
   auto slot = [](int p) { return ((p % 5) + 5) % 5; };
@@ -426,23 +462,14 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
       if (t < kSwPairs) P::put(pl + 2 * t, pre[m]);
     }
   };
-  // one Gauss-Seidel update of the cell (row r, column kk) from LDS planes (xm, x, xp); 0 when not free
+  // one Gauss-Seidel update of the cell (row r, column kk) from LDS planes (xm, x, xp); all six
+  // neighbours are read (LDS addresses inside the tile), closed faces are skipped by rb_value
   auto update = [&](int r, int kk, bool kzero, uint32_t f, float rv, float cxl, float cxh, const TE* xm,
                     const TE* x, const TE* xp) -> float {
-    float v = 0.0f;
-    if (!(f & kNotFree)) {
-      const int li = r * kSwCols + kk;
-      float nb = 0.0f, den = 0.0f;
-      if (f & kXm) { nb += cxl * to_c_exact(xm[li]); den += cxl; }
-      if (f & kXp) { nb += cxh * to_c_exact(xp[li]); den += cxh; }
-      if (f & kYm) { const float c = cy[0][r]; nb += c * to_c_exact(x[li - kSwCols]); den += c; }
-      if (f & kYp) { const float c = cy[1][r]; nb += c * to_c_exact(x[li + kSwCols]); den += c; }
-      if (f & kZm) { const float c = cz[0][kk]; nb += c * to_c_exact(x[li - 1]); den += c; }
-      if (f & kZp) { const float c = cz[1][kk]; nb += c * to_c_exact(x[li + 1]); den += c; }
-      if (g.outlet && kzero) den += g.outlet_add;
-      if (den > 0.0f) v = (nb + rs * rv) / den;
-    }
-    return v;
+    const int li = r * kSwCols + kk;
+    return rb_value(f, cxl, cxh, cy[0][r], cy[1][r], cz[0][kk], cz[1][kk], to_c_exact(xm[li]), to_c_exact(xp[li]),
+                    to_c_exact(x[li - kSwCols]), to_c_exact(x[li + kSwCols]), to_c_exact(x[li - 1]),
+                    to_c_exact(x[li + 1]), (g.outlet && kzero) ? g.outlet_add : 0.0f, rs, rv);
   };
   auto run_colour = [&](const SweepSlot* sl, int ns, const uint32_t* fl, const float* rv, int p, int color) {
     TE* x = E[slot(p)];
@@ -451,53 +478,51 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
     const float cxl = g.cl[0][p], cxh = g.ch[0][p];
     for (int m = 0; m < ns; ++m) {
       const int d = pick(sl[m], p, color);
-      if (!in_grid(sl[m], d)) continue;
       const int r = (int)(sl[m].bits & 31u), kk = (int)((sl[m].bits >> 5) & 127u) + d;
       const bool kzero = (sl[m].bits >> (15 + d)) & 1u;
-      x[r * kSwCols + kk] = (TE)update(r, kk, kzero, fl[m], rv[m], cxl, cxh, xm, x, xp);
+      const TE v = (TE)update(r < 1 ? 1 : r, kk < 1 ? 1 : kk, kzero, fl[m], rv[m], cxl, cxh, xm, x, xp);
+      if (in_grid(sl[m], d)) x[r * kSwCols + kk] = v;
     }
   };
+  // the tile's pairs this thread stores: in-plane grid offset (-1: outside the grid) and LDS index
+  constexpr int kSt = kSwTJ * (kSwTK / 2) / kSwThreads;
+  int32_t soff[kSt];
+  int sli[kSt];
+#pragma unroll
+  for (int m = 0; m < kSt; ++m) {
+    const int t = tid + m * kSwThreads;
+    const int r = t / (kSwTK / 2), kk = kSwHalo + 2 * (t % (kSwTK / 2));
+    const int j = j0 + kSwHalo + r, k = k0 + kk;
+    soff[m] = (j < ny && k < nz) ? j * nz + k : -1;
+    sli[m] = (r + kSwHalo) * kSwCols + kk;
+  }
   auto store_plane = [&](int p) {
     const TE* pl = E[slot(p)];
-    for (int t = tid; t < kSwTJ * (kSwTK / 2); t += kSwThreads) {
-      const int r = t / (kSwTK / 2), kk = kSwHalo + 2 * (t % (kSwTK / 2));
-      const int j = j0 + kSwHalo + r, k = k0 + kk;
-      if (j >= ny || k >= nz) continue;
-      P::store(eout + ((size_t)p * ny + j) * nz + k, pl + (r + kSwHalo) * kSwCols + kk);
-    }
+#pragma unroll
+    for (int m = 0; m < kSt; ++m)
+      if (soff[m] >= 0) P::store(eout + (size_t)p * sxy + (size_t)soff[m], pl + sli[m]);
   };
 
-  // prologue: planes ia - 2 and ia - 1 into LDS, planes ia and ia + 1 into registers, the cells of the
-  // first two steps
+  // prologue: planes ia - 2 and ia - 1 into LDS, plane ia into registers, the cells of the first step
   fetch(ia - 2, pre);
   put(ia - 2);
   fetch(ia - 1, pre);
   put(ia - 1);
   fetch(ia, pre);
-  fetch(ia + 1, pre_n);
   fetch_cells(ia - 1, ia - 2, fa, ra, fb, rb);
-  fetch_cells(ia, ia - 1, fa_n, ra_n, fb_n, rb_n);
   for (int i = ia - 1; i <= iend; ++i) {
     put(i + 1);
-#pragma unroll
-    for (int m = 0; m < kSwLoadIters; ++m) pre[m] = pre_n[m];
-    if (i + 3 <= iend + 1) fetch(i + 3, pre_n);
-    if (i + 2 <= iend) fetch_cells(i + 2, i + 1, fa_nn, ra_nn, fb_nn, rb_nn);
+    if (i + 2 <= iend + 1) fetch(i + 2, pre);
+    if (i + 1 <= iend) fetch_cells(i + 1, i, fa_n, ra_n, fb_n, rb_n);
     lds_barrier();
     if (i - 2 >= ia) store_plane(i - 2);
     if (i >= 0 && i < nx) run_colour(sa, kA, fa, ra, i, c0);  // first colour on plane i, tile + 1
     lds_barrier();
     if (i - 1 >= ia && i - 1 < iend) run_colour(sb, kB, fb, rb, i - 1, 1 - c0);  // second colour, plane i - 1
 #pragma unroll
-    for (int m = 0; m < kA; ++m) {
-      fa[m] = fa_n[m]; ra[m] = ra_n[m];
-      fa_n[m] = fa_nn[m]; ra_n[m] = ra_nn[m];
-    }
+    for (int m = 0; m < kA; ++m) { fa[m] = fa_n[m]; ra[m] = ra_n[m]; }
 #pragma unroll
-    for (int m = 0; m < kB; ++m) {
-      fb[m] = fb_n[m]; rb[m] = rb_n[m];
-      fb_n[m] = fb_nn[m]; rb_n[m] = rb_nn[m];
-    }
+    for (int m = 0; m < kB; ++m) { fb[m] = fb_n[m]; rb[m] = rb_n[m]; }
   }
   lds_barrier();
   if (iend - 1 >= ia) store_plane(iend - 1);
