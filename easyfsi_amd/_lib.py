@@ -122,6 +122,7 @@ SIGNATURES = {
     "hfsi_fluid_mixed_level_shape": (ctypes.c_int, [_P, _I, _P]),
     "hfsi_fluid_mixed_set_half_min_cells": (ctypes.c_int, [_P, ctypes.c_int64]),
     "hfsi_fluid_mixed_set_fused_min_cells": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "hfsi_fluid_mixed_set_sweep_workgroups": (ctypes.c_int, [_P, ctypes.c_int64]),
     "hfsi_unet_create": (ctypes.c_int, [ctypes.POINTER(_P), _I, _I, _I, _I, ctypes.c_uint64, _P]),
     "hfsi_unet_destroy": (ctypes.c_int, [_P]),
     "hfsi_unet_num_params": (ctypes.c_int64, [_P]),

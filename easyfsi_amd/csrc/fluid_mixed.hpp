@@ -74,17 +74,19 @@ __device__ inline float rhs_scale(const double* rs_dev) { return rs_dev ? (float
 // (+ the outlet term), 0 on cells that are not free.  The colour passes and
 // the fused sweep both call it, with explicit fused multiply-adds, so their
 // results agree bit for bit whatever the surrounding code looks like (closed
-// faces are skipped, their neighbour values are never used).
-__device__ inline float rb_value(uint32_t f, float cxl, float cxh, float cyl, float cyh, float czl, float czh,
-                                 float exm, float exp_, float eym, float eyp, float ezm, float ezp, float den_extra,
-                                 float rs, float r) {
+// faces are skipped, their neighbour values are never used).  Neighbour
+// values come in their storage type TN and are widened exactly inside (for
+// fp16 the widening folds into a mixed-precision fused multiply-add).
+template <typename TN>
+__device__ inline float rb_value(uint32_t f, float cxl, float cxh, float cyl, float cyh, float czl, float czh, TN exm,
+                                 TN exp_, TN eym, TN eyp, TN ezm, TN ezp, float den_extra, float rs, float r) {
   float nb = 0.0f, den = 0.0f;
-  if (f & kXm) { nb = fmaf(cxl, exm, nb); den += cxl; }
-  if (f & kXp) { nb = fmaf(cxh, exp_, nb); den += cxh; }
-  if (f & kYm) { nb = fmaf(cyl, eym, nb); den += cyl; }
-  if (f & kYp) { nb = fmaf(cyh, eyp, nb); den += cyh; }
-  if (f & kZm) { nb = fmaf(czl, ezm, nb); den += czl; }
-  if (f & kZp) { nb = fmaf(czh, ezp, nb); den += czh; }
+  if (f & kXm) { nb = fmaf(cxl, to_c_exact(exm), nb); den += cxl; }
+  if (f & kXp) { nb = fmaf(cxh, to_c_exact(exp_), nb); den += cxh; }
+  if (f & kYm) { nb = fmaf(cyl, to_c_exact(eym), nb); den += cyl; }
+  if (f & kYp) { nb = fmaf(cyh, to_c_exact(eyp), nb); den += cyh; }
+  if (f & kZm) { nb = fmaf(czl, to_c_exact(ezm), nb); den += czl; }
+  if (f & kZp) { nb = fmaf(czh, to_c_exact(ezp), nb); den += czh; }
   den += den_extra;
   // hardware reciprocal (1 ulp) rather than an IEEE division: the smoother needs no exact quotient
   float v = fmaf(rs, r, nb) * __builtin_amdgcn_rcpf(den > 0.0f ? den : 1.0f);
@@ -93,6 +95,32 @@ __device__ inline float rb_value(uint32_t f, float cxl, float cxh, float cyl, fl
   asm("" : "+v"(v));
   return (!(f & kNotFree) && den > 0.0f) ? v : 0.0f;
 }
+
+// Flag byte of a free cell whose six faces are all open.
+constexpr uint32_t kOpenFree = kXm | kXp | kYm | kYp | kZm | kZp;
+__device__ inline bool all_open(uint32_t f) { return (f & ~(uint32_t)kObst) == kOpenFree; }
+
+// rb_value of a free cell with all six faces open: the same operations in the
+// same order without the face tests, so both give the same bits.
+template <typename TN>
+__device__ inline float rb_value_open(float cxl, float cxh, float cyl, float cyh, float czl, float czh, TN exm,
+                                      TN exp_, TN eym, TN eyp, TN ezm, TN ezp, float den_extra, float rs, float r) {
+  float nb = 0.0f, den = 0.0f;
+  nb = fmaf(cxl, to_c_exact(exm), nb); den += cxl;
+  nb = fmaf(cxh, to_c_exact(exp_), nb); den += cxh;
+  nb = fmaf(cyl, to_c_exact(eym), nb); den += cyl;
+  nb = fmaf(cyh, to_c_exact(eyp), nb); den += cyh;
+  nb = fmaf(czl, to_c_exact(ezm), nb); den += czl;
+  nb = fmaf(czh, to_c_exact(ezp), nb); den += czh;
+  den += den_extra;
+  float v = fmaf(rs, r, nb) * __builtin_amdgcn_rcpf(den > 0.0f ? den : 1.0f);
+  asm("" : "+v"(v));
+  return den > 0.0f ? v : 0.0f;
+}
+
+// True when every active lane of the wavefront has a free cell with all faces
+// open (a wave-uniform value: the branch on it is a scalar branch).
+__device__ inline bool wave_all_open(uint32_t f) { return __builtin_amdgcn_ballot_w64(!all_open(f)) == 0; }
 
 // One colour pass of red-black Gauss-Seidel on L e = rhs, L = den e - nb.
 // A thread owns the k-pair (2kk, 2kk+1) of row (i, j) and updates the cell of
@@ -119,17 +147,25 @@ __global__ void __launch_bounds__(kBlock) k_mix_rb(OpGeo<float> g, TE* e, const 
     const uint8_t f = g.flags[cc];
     float v = 0.0f;
     if (!(f & kNotFree)) {
-      float exm = 0.0f, exp_ = 0.0f, eym = 0.0f, eyp = 0.0f, ezm = 0.0f, ezp = 0.0f;
+      TE exm = (TE)0.0f, exp_ = (TE)0.0f, eym = (TE)0.0f, eyp = (TE)0.0f, ezm = (TE)0.0f, ezp = (TE)0.0f;
       if (!FROMZERO) {
-        if (f & kXm) exm = ldf<TE>(e, cc - g.sx);
-        if (f & kXp) exp_ = ldf<TE>(e, cc + g.sx);
-        if (f & kYm) eym = ldf<TE>(e, cc - g.sy);
-        if (f & kYp) eyp = ldf<TE>(e, cc + g.sy);
-        if (f & kZm) ezm = ldf<TE>(e, cc - 1);
-        if (f & kZp) ezp = ldf<TE>(e, cc + 1);
+        // six unconditional loads (one memory round trip); a closed face reads the cell itself, which is
+        // in the grid and whose value is not used
+        exm = e[(f & kXm) ? cc - g.sx : cc];
+        exp_ = e[(f & kXp) ? cc + g.sx : cc];
+        eym = e[(f & kYm) ? cc - g.sy : cc];
+        eyp = e[(f & kYp) ? cc + g.sy : cc];
+        ezm = e[(f & kZm) ? cc - 1 : cc];
+        ezp = e[(f & kZp) ? cc + 1 : cc];
       }
-      v = rb_value(f, g.cl[0][i], g.ch[0][i], g.cl[1][j], g.ch[1][j], g.cl[2][k], g.ch[2][k], exm, exp_, eym, eyp,
-                   ezm, ezp, (g.outlet && k == 0) ? g.outlet_add : 0.0f, rs, ldf<TR>(rhs, cc));
+      const float extra = (g.outlet && k == 0) ? g.outlet_add : 0.0f, r = ldf<TR>(rhs, cc);
+      const float cxl = g.cl[0][i], cxh = g.ch[0][i], cyl = g.cl[1][j], cyh = g.ch[1][j];
+      const float czl = g.cl[2][k], czh = g.ch[2][k];
+      if (wave_all_open(f)) {
+        v = rb_value_open(cxl, cxh, cyl, cyh, czl, czh, exm, exp_, eym, eyp, ezm, ezp, extra, rs, r);
+      } else {
+        v = rb_value(f, cxl, cxh, cyl, cyh, czl, czh, exm, exp_, eym, eyp, ezm, ezp, extra, rs, r);
+      }
     }
     stf<TE>(e, cc, v);
   }
@@ -149,9 +185,11 @@ __global__ void __launch_bounds__(kBlock) k_mix_residual(OpGeo<float> g, const T
     const uint32_t j = t % g.ny, i = t / g.ny;
     const uint8_t f = g.flags[c];
     const float ec = ldf<TE>(e, c), r = ldf<TR>(rhs, c);
-    const float exm = i > 0 ? ldf<TE>(e, c - g.sx) : 0.0f, exp_ = i + 1 < g.nx ? ldf<TE>(e, c + g.sx) : 0.0f;
-    const float eym = j > 0 ? ldf<TE>(e, c - g.sy) : 0.0f, eyp = j + 1 < g.ny ? ldf<TE>(e, c + g.sy) : 0.0f;
-    const float ezm = k > 0 ? ldf<TE>(e, c - 1) : 0.0f, ezp = k + 1 < g.nz ? ldf<TE>(e, c + 1) : 0.0f;
+    // a neighbour outside the grid reads the cell itself (its face is closed, the value is not used), so
+    // the loads need no branches
+    const float exm = ldf<TE>(e, i > 0 ? c - g.sx : c), exp_ = ldf<TE>(e, i + 1 < g.nx ? c + g.sx : c);
+    const float eym = ldf<TE>(e, j > 0 ? c - g.sy : c), eyp = ldf<TE>(e, j + 1 < g.ny ? c + g.sy : c);
+    const float ezm = ldf<TE>(e, k > 0 ? c - 1 : c), ezp = ldf<TE>(e, k + 1 < g.nz ? c + 1 : c);
     float out = 0.0f;
     if (!(f & kNotFree)) {
       float nb = 0.0f, den = 0.0f;
@@ -275,10 +313,9 @@ __global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, 
 // ---------------------------------------------------------------------------
 // Fused red-black sweep (both colour passes in one pass over HBM), marching
 // along i.  A workgroup owns a tile of kSwTJ x kSwTK cells of the (j, k)
-// plane and a chunk of planes [ia, ib).  It keeps a ring of five planes of
-// the iterate in LDS (storage type TE, halo 2 in j and k; five so that the
-// plane written at a step is none of the four the previous step still reads
-// after its last barrier) and at step i
+// plane and a chunk of planes [ia, ib).  It keeps a ring of kSwRing planes of
+// the iterate in LDS (storage type TE, halo 2 in j and k; five are live) and
+// at step i
 //   * writes plane i + 1 (loaded during the step before) into LDS,
 //   * stores plane i - 2 (final after the step before),
 //   * runs the first colour c0 on plane i over the tile plus a halo of 1
@@ -293,6 +330,14 @@ __global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, 
 // starts two planes early (the first colour on plane ia - 1, planes ia - 2 ..
 // loaded) and recomputes what the chunk before owns.  Requires nz even
 // (2-element loads and stores).
+//
+// Work layout: a thread owns up to kSwSlotIters "slots", k-pairs (2q, 2q + 1)
+// of LDS rows 1 .. kSwTJ + 2.  On plane i the first colour is the pair's cell
+// with k parity (c0 + i + j) % 2, and the second colour on plane i - 1 is the
+// same cell of the pair, so one slot serves both colours of a step and the
+// parity of i selects the cell: the per-parity LDS offsets, grid offsets and
+// update masks are computed once and the march is unrolled by two.  Cells outside the grid are never updated and never read (their faces
+// are closed), so their LDS entries need no defined value.
 // ---------------------------------------------------------------------------
 
 constexpr int kSwTJ = 16, kSwTK = 64, kSwHalo = 2;
@@ -301,9 +346,16 @@ constexpr int kSwCols = kSwTK + 2 * kSwHalo;     // 68
 constexpr int kSwPlane = kSwRows * kSwCols;      // cells per LDS plane
 constexpr int kSwPairs = kSwPlane / 2;           // 2-element loads per plane
 constexpr int kSwThreads = 256;
-constexpr int kSwLoadIters = (kSwPairs + kSwThreads - 1) / kSwThreads;  // 3
-constexpr int kSwA = (kSwTJ + 2) * ((kSwTK + 2) / 2);                   // first-colour cells (tile + 1)
-constexpr int kSwB = kSwTJ * (kSwTK / 2);                               // second-colour cells (tile)
+constexpr int kSwLoadIters = (kSwPairs + kSwThreads - 1) / kSwThreads;   // 3
+constexpr int kSwSlotPairs = kSwCols / 2;                                 // 34 slots per LDS row
+constexpr int kSwSlots = (kSwTJ + 2) * kSwSlotPairs;                      // 612 (LDS rows 1 .. 18)
+constexpr int kSwSlotIters = (kSwSlots + kSwThreads - 1) / kSwThreads;   // 3
+constexpr int kSwStoreIters = kSwTJ * (kSwTK / 2) / kSwThreads;           // 2
+constexpr int kSwRing = 8;  // LDS planes (5 are live; a power of two keeps the slot arithmetic to a mask)
+// slot word of one parity: byte offset in an LDS plane of the cell one row above (so that all six
+// neighbour reads have non-negative immediate offsets), and whether the first (bit 31) and the second
+// (bit 30) colour update the cell
+constexpr uint32_t kSwUpdA = 1u << 31, kSwUpdB = 1u << 30, kSwLdsMask = 0xffffu;
 
 struct SweepGeo {
   uint32_t kblocks, jblocks, ichunk, nblocks;
@@ -332,27 +384,19 @@ struct Pair<float> {
   __device__ static inline raw zero() { return make_float2(0.0f, 0.0f); }
 };
 
-// A thread's cells are fixed pairs of adjacent k cells ("slots"): the first
-// colour uses slots covering the tile plus a halo of 1 (18 rows x 33 pairs),
-// the second colour slots covering the tile (16 rows x 32 pairs); the colour
-// of a plane picks one cell of each pair.  A slot packs its LDS row, its
-// first LDS column, the parity of its grid row and whether each of its two
-// cells lies in the grid; `off` is the in-plane grid offset of its first cell.
-struct SweepSlot {
-  uint32_t bits;  // row | col << 5 | jpar << 12 | in0 << 13 | in1 << 14 | (k0 == 0) << 15 | (k1 == 0) << 16
-  uint32_t off;   // grid offset of the first cell within its plane (modulo 2^32 when k = -1)
+template <int V>
+struct SwC {
+  static constexpr int value = V;
 };
 
 template <typename TE, typename TR, bool FROMZERO>
 __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const TE* __restrict__ ein,
                                                            TE* __restrict__ eout, const TR* __restrict__ rhs,
                                                            const double* rs_dev, int c0, SweepGeo sg) {
-  __shared__ TE E[5][kSwPlane];
-  __shared__ float cy[2][kSwRows], cz[2][kSwCols];  // y / z coefficients of the tile (0 outside the grid)
+  __shared__ TE E[kSwRing][kSwPlane];
+  __shared__ float2 cyp[kSwRows];  // (low, high) y coefficients of each LDS row (0 outside the grid)
+  __shared__ float4 czp[kSwCols];  // (low, high) z coefficients and the outlet term of each LDS column
   using P = Pair<TE>;
-  constexpr int kPairsA = (kSwTK + 2) / 2, kPairsB = kSwTK / 2;
-  constexpr int kA = (kSwA + kSwThreads - 1) / kSwThreads;  // first-colour slots per thread
-  constexpr int kB = (kSwB + kSwThreads - 1) / kSwThreads;  // second-colour slots per thread
   const uint32_t G = gridDim.x;
   const uint32_t b = blockIdx.x;
   const uint32_t L = (b % HFSI_NXCD) * (G / HFSI_NXCD) + b / HFSI_NXCD;
@@ -362,7 +406,7 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
   const int ib = (int)(L / (sg.kblocks * sg.jblocks));
   const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
   const uint32_t sxy = g.ny * g.nz;
-  const int j0 = jb * kSwTJ - kSwHalo, k0 = kb * kSwTK - kSwHalo;  // LDS (0, 0) in grid coordinates
+  const int j0 = jb * kSwTJ - kSwHalo, k0 = kb * kSwTK - kSwHalo;  // LDS (0, 0) in grid coordinates (k0 even)
   const int ia = ib * (int)sg.ichunk, iend = min(nx, ia + (int)sg.ichunk);
   const float rs = rhs_scale(rs_dev);
   const int tid = threadIdx.x;
@@ -370,90 +414,92 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
     if (t < kSwRows) {
       const int j = j0 + t;
       const bool in = j >= 0 && j < ny;
-      cy[0][t] = in ? g.cl[1][j] : 0.0f;
-      cy[1][t] = in ? g.ch[1][j] : 0.0f;
+      cyp[t] = make_float2(in ? g.cl[1][j] : 0.0f, in ? g.ch[1][j] : 0.0f);
     } else {
       const int kk = t - kSwRows, k = k0 + kk;
       const bool in = k >= 0 && k < nz;
-      cz[0][kk] = in ? g.cl[2][k] : 0.0f;
-      cz[1][kk] = in ? g.ch[2][k] : 0.0f;
+      czp[kk] = make_float4(in ? g.cl[2][k] : 0.0f, in ? g.ch[2][k] : 0.0f,
+                            (in && g.outlet && k == 0) ? g.outlet_add : 0.0f, 0.0f);
     }
   }
-  auto make_slot = [&](int r, int col) -> SweepSlot {
-    const int j = j0 + r, k = k0 + col;
-    const bool jin = j >= 0 && j < ny;
-    const bool in0 = jin && k >= 0 && k < nz, in1 = jin && k + 1 >= 0 && k + 1 < nz;
-    SweepSlot q;
-    q.bits = (uint32_t)r | ((uint32_t)col << 5) | ((uint32_t)(j & 1) << 12) | ((uint32_t)in0 << 13) |
-             ((uint32_t)in1 << 14) | ((uint32_t)(k == 0) << 15) | ((uint32_t)(k + 1 == 0) << 16);
-    q.off = jin ? (uint32_t)(j * nz + k) : 0u;
-    return q;
-  };
-  SweepSlot sa[kA], sb[kB];
-#pragma unroll
-  for (int m = 0; m < kA; ++m) {
-    const int t = tid + m * kSwThreads;
-    sa[m] = t < kSwA ? make_slot(1 + t / kPairsA, 1 + 2 * (t % kPairsA)) : SweepSlot{0u, 0u};
-  }
-#pragma unroll
-  for (int m = 0; m < kB; ++m) {
-    const int t = tid + m * kSwThreads;
-    sb[m] = t < kSwB ? make_slot(kSwHalo + t / kPairsB, kSwHalo + 2 * (t % kPairsB)) : SweepSlot{0u, 0u};
-  }
-  // the cell of a slot on plane p for the colour `color`: d = 0 / 1 picks the pair's first / second cell
-  auto pick = [&](const SweepSlot& q, int p, int color) -> int {
-    const int kpar = ((q.bits >> 5) & 1) ^ (k0 & 1);  // parity of the first cell's k (k0 is even)
-    return ((color + p + (int)((q.bits >> 12) & 1) + kpar) & 1);  // 1: the colour is on the second cell
-  };
-  auto in_grid = [](const SweepSlot& q, int d) -> bool { return (q.bits >> (13 + d)) & 1u; };
 
-  // flags and right-hand sides of this step's cells and of the next step's
-  uint32_t fa[kA], fb[kB], fa_n[kA], fb_n[kB];
-  float ra[kA], rb[kB], ra_n[kA], rb_n[kB];
-  // unconditional loads from a clamped index (cell 0 for cells outside the grid), then a select: no
-  // branches around the loads
-  auto fetch_cells = [&](int pa, int pb, uint32_t* fA, float* rA, uint32_t* fB, float* rB) {
+  // update slots; the workgroup index rotates the thread -> slot map by whole wavefronts, so the partly
+  // filled last round of slots lands on a different SIMD in neighbouring workgroups
+  const int trot = (tid + 64 * (int)(L & 3u)) & (kSwThreads - 1);
+  // per parity of i: the slot's cell for the first colour on plane i and the second on plane i - 1
+  uint32_t W[2][kSwSlotIters];    // LDS byte offset of the row above | kSwUpdA | kSwUpdB
+  uint32_t cza[2][kSwSlotIters];  // byte offset of the cell's column in czp
+  uint32_t gof[2][kSwSlotIters];  // in-plane grid offset of the cell (0 outside the grid)
+  uint32_t cya[kSwSlotIters];     // byte offset of the slot's row in cyp
 #pragma unroll
-    for (int m = 0; m < kA; ++m) {
-      const int d = pick(sa[m], pa, c0);
-      const bool ok = pa >= 0 && pa < nx && in_grid(sa[m], d);
-      const uint32_t c = ok ? (uint32_t)pa * sxy + sa[m].off + (uint32_t)d : 0u;
-      const uint32_t f = g.flags[c];
-      const float r = ldf<TR>(rhs, c);
-      fA[m] = ok ? f : (uint32_t)kNotFree;
-      rA[m] = ok ? r : 0.0f;
+  for (int m = 0; m < kSwSlotIters; ++m) {
+    const int t = trot + m * kSwThreads;
+    const bool ok = t < kSwSlots;
+    const int r = 1 + (ok ? t / kSwSlotPairs : 0), q = ok ? t % kSwSlotPairs : 0;
+    const int j = j0 + r;
+    const bool jin = ok && j >= 0 && j < ny;
+    cya[m] = (uint32_t)r * (uint32_t)sizeof(float2);
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const int col = 2 * q + ((c0 + par + j) & 1), k = k0 + col;
+      const bool in = jin && k >= 0 && k < nz;
+      const bool ua = in && col >= 1 && col <= kSwCols - 2;
+      const bool ub = in && r >= kSwHalo && r < kSwHalo + kSwTJ && col >= kSwHalo && col < kSwHalo + kSwTK;
+      W[par][m] = (uint32_t)((r - 1) * kSwCols + col) * (uint32_t)sizeof(TE) | (ua ? kSwUpdA : 0u) |
+                  (ub ? kSwUpdB : 0u);
+      cza[par][m] = (uint32_t)col * (uint32_t)sizeof(float4);
+      gof[par][m] = in ? (uint32_t)(j * nz + k) : 0u;
     }
+  }
+
+  // flags and right-hand sides of a step's cells: [parity of i][first / second colour][slot]
+  uint32_t fl[2][2][kSwSlotIters];
+  float rv[2][2][kSwSlotIters];
+  // the cells of step i (parity Q): first colour on plane i, second colour on plane i - 1
+  auto fetch_cells = [&](auto Qc, int i) {
+    constexpr int Q = decltype(Qc)::value;
+    if (i >= 0 && i < nx) {
+      const uint8_t* f = g.flags + (size_t)i * sxy;
+      const char* rh = reinterpret_cast<const char*>(rhs + (size_t)i * sxy);
 #pragma unroll
-    for (int m = 0; m < kB; ++m) {
-      const int d = pick(sb[m], pb, 1 - c0);
-      const bool ok = pb >= 0 && pb < nx && in_grid(sb[m], d);
-      const uint32_t c = ok ? (uint32_t)pb * sxy + sb[m].off + (uint32_t)d : 0u;
-      const uint32_t f = g.flags[c];
-      const float r = ldf<TR>(rhs, c);
-      fB[m] = ok ? f : (uint32_t)kNotFree;
-      rB[m] = ok ? r : 0.0f;
+      for (int m = 0; m < kSwSlotIters; ++m) {
+        fl[Q][0][m] = f[gof[Q][m]];
+        rv[Q][0][m] = to_c_exact(*reinterpret_cast<const TR*>(rh + gof[Q][m] * (uint32_t)sizeof(TR)));
+      }
+    }
+    if (i - 1 >= 0 && i - 1 < nx) {
+      const uint8_t* f = g.flags + (size_t)(i - 1) * sxy;
+      const char* rh = reinterpret_cast<const char*>(rhs + (size_t)(i - 1) * sxy);
+#pragma unroll
+      for (int m = 0; m < kSwSlotIters; ++m) {
+        fl[Q][1][m] = f[gof[Q][m]];
+        rv[Q][1][m] = to_c_exact(*reinterpret_cast<const TR*>(rh + gof[Q][m] * (uint32_t)sizeof(TR)));
+      }
     }
   };
 
-  // 2-element loads of plane p into registers (0 outside the grid), a step before they go to LDS; the
-  // in-plane offset of each of this thread's pairs is computed once (-1: outside the grid)
-  int32_t poff[kSwLoadIters];
+  // 2-element loads of plane p into registers, a step before they go to LDS; the in-plane byte offset of
+  // each of this thread's pairs is computed once (cell 0 for pairs outside the grid)
+  uint32_t poff[kSwLoadIters];
 #pragma unroll
   for (int m = 0; m < kSwLoadIters; ++m) {
     const int t = tid + m * kSwThreads;
-    const int r = t / (kSwCols / 2), k = k0 + 2 * (t % (kSwCols / 2)), j = j0 + r;
-    poff[m] = (t < kSwPairs && j >= 0 && j < ny && k >= 0 && k < nz) ? j * nz + k : -1;
+    const int r = t / kSwSlotPairs, k = k0 + 2 * (t % kSwSlotPairs), j = j0 + r;
+    poff[m] = (t < kSwPairs && j >= 0 && j < ny && k >= 0 && k < nz) ? (uint32_t)(j * nz + k) * (uint32_t)sizeof(TE)
+                                                                       : 0u;
   }
   typename P::raw pre[kSwLoadIters];
-  auto fetch = [&](int p, typename P::raw* dst) {
+  auto fetch = [&](int p) {
+    if (FROMZERO || p < 0 || p >= nx) {
 #pragma unroll
-    for (int m = 0; m < kSwLoadIters; ++m) {
-      dst[m] = P::zero();
-      if (FROMZERO || poff[m] < 0 || p < 0 || p >= nx) continue;
-      dst[m] = P::load(ein + (size_t)p * sxy + (size_t)poff[m]);
+      for (int m = 0; m < kSwLoadIters; ++m) pre[m] = P::zero();
+      return;
     }
+    const char* src = reinterpret_cast<const char*>(ein + (size_t)p * sxy);
+#pragma unroll
+    for (int m = 0; m < kSwLoadIters; ++m) pre[m] = P::load(reinterpret_cast<const TE*>(src + poff[m]));
   };
-  auto slot = [](int p) { return ((p % 5) + 5) % 5; };
+  auto slot = [](int p) { return p & (kSwRing - 1); };
   auto put = [&](int p) {
     TE* pl = E[slot(p)];
 #pragma unroll
@@ -462,34 +508,41 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
       if (t < kSwPairs) P::put(pl + 2 * t, pre[m]);
     }
   };
-  // one Gauss-Seidel update of the cell (row r, column kk) from LDS planes (xm, x, xp); all six
-  // neighbours are read (LDS addresses inside the tile), closed faces are skipped by rb_value
-  auto update = [&](int r, int kk, bool kzero, uint32_t f, float rv, float cxl, float cxh, const TE* xm,
-                    const TE* x, const TE* xp) -> float {
-    const int li = r * kSwCols + kk;
-    return rb_value(f, cxl, cxh, cy[0][r], cy[1][r], cz[0][kk], cz[1][kk], to_c_exact(xm[li]), to_c_exact(xp[li]),
-                    to_c_exact(x[li - kSwCols]), to_c_exact(x[li + kSwCols]), to_c_exact(x[li - 1]),
-                    to_c_exact(x[li + 1]), (g.outlet && kzero) ? g.outlet_add : 0.0f, rs, rv);
-  };
-  auto run_colour = [&](const SweepSlot* sl, int ns, const uint32_t* fl, const float* rv, int p, int color) {
-    TE* x = E[slot(p)];
-    const TE* xm = E[slot(p - 1)];
-    const TE* xp = E[slot(p + 1)];
+  // one colour on plane p: colour Bb (0 first, 1 second) with the slot words of parity Wp and the cells
+  // fetched into set Sp (both the parity of the step)
+  auto colour = [&](auto Wc, auto Sc, auto Bc, int p) {
+    constexpr int Wp = decltype(Wc)::value, Sp = decltype(Sc)::value, Bb = decltype(Bc)::value;
+    constexpr uint32_t upd = Bb ? kSwUpdB : kSwUpdA;
+    char* xb = reinterpret_cast<char*>(E[slot(p)]);
+    const char* mb = reinterpret_cast<const char*>(E[slot(p - 1)]);
+    const char* pb = reinterpret_cast<const char*>(E[slot(p + 1)]);
     const float cxl = g.cl[0][p], cxh = g.ch[0][p];
-    for (int m = 0; m < ns; ++m) {
-      const int d = pick(sl[m], p, color);
-      const int r = (int)(sl[m].bits & 31u), kk = (int)((sl[m].bits >> 5) & 127u) + d;
-      const bool kzero = (sl[m].bits >> (15 + d)) & 1u;
-      const TE v = (TE)update(r < 1 ? 1 : r, kk < 1 ? 1 : kk, kzero, fl[m], rv[m], cxl, cxh, xm, x, xp);
-      if (in_grid(sl[m], d)) x[r * kSwCols + kk] = v;
+#pragma unroll
+    for (int m = 0; m < kSwSlotIters; ++m) {
+      const uint32_t w = W[Wp][m];
+      if (!(w & upd)) continue;
+      const uint32_t o = w & kSwLdsMask;
+      TE* x = reinterpret_cast<TE*>(xb + o) + kSwCols;
+      const TE xm = reinterpret_cast<const TE*>(mb + o)[kSwCols], xp = reinterpret_cast<const TE*>(pb + o)[kSwCols];
+      const float2 cy = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(cyp) + cya[m]);
+      const float4 cz = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(czp) + cza[Wp][m]);
+      const uint32_t f = fl[Sp][Bb][m];
+      const float r = rv[Sp][Bb][m];
+      float v;
+      if (wave_all_open(f))
+        v = rb_value_open<TE>(cxl, cxh, cy.x, cy.y, cz.x, cz.y, xm, xp, x[-kSwCols], x[kSwCols], x[-1], x[1], cz.z,
+                              rs, r);
+      else
+        v = rb_value<TE>(f, cxl, cxh, cy.x, cy.y, cz.x, cz.y, xm, xp, x[-kSwCols], x[kSwCols], x[-1], x[1], cz.z,
+                         rs, r);
+      x[0] = (TE)v;
     }
   };
   // the tile's pairs this thread stores: in-plane grid offset (-1: outside the grid) and LDS index
-  constexpr int kSt = kSwTJ * (kSwTK / 2) / kSwThreads;
-  int32_t soff[kSt];
-  int sli[kSt];
+  int32_t soff[kSwStoreIters];
+  int sli[kSwStoreIters];
 #pragma unroll
-  for (int m = 0; m < kSt; ++m) {
+  for (int m = 0; m < kSwStoreIters; ++m) {
     const int t = tid + m * kSwThreads;
     const int r = t / (kSwTK / 2), kk = kSwHalo + 2 * (t % (kSwTK / 2));
     const int j = j0 + kSwHalo + r, k = k0 + kk;
@@ -499,31 +552,41 @@ __global__ void __launch_bounds__(kSwThreads) k_mix_sweep(OpGeo<float> g, const 
   auto store_plane = [&](int p) {
     const TE* pl = E[slot(p)];
 #pragma unroll
-    for (int m = 0; m < kSt; ++m)
+    for (int m = 0; m < kSwStoreIters; ++m)
       if (soff[m] >= 0) P::store(eout + (size_t)p * sxy + (size_t)soff[m], pl + sli[m]);
+  };
+  // step i (parity Q = i & 1): planes i - 2 .. i + 1 are in LDS after its first barrier
+  auto step = [&](auto Qc, int i) {
+    constexpr int Q = decltype(Qc)::value;
+    put(i + 1);  // into the ring slot of plane i - 7
+    if (i + 2 <= iend + 1) fetch(i + 2);
+    if (i + 1 <= iend) fetch_cells(SwC<1 - Q>{}, i + 1);
+    lds_barrier();
+    if (i - 2 >= ia) store_plane(i - 2);  // final since the second colour on it, the step before
+    if (i >= 0 && i < nx) colour(Qc, Qc, SwC<0>{}, i);  // first colour on plane i, tile + 1
+    lds_barrier();
+    if (i - 1 >= ia && i - 1 < iend) colour(Qc, Qc, SwC<1>{}, i - 1);  // second colour, plane i - 1
   };
 
   // prologue: planes ia - 2 and ia - 1 into LDS, plane ia into registers, the cells of the first step
-  fetch(ia - 2, pre);
+  fetch(ia - 2);
   put(ia - 2);
-  fetch(ia - 1, pre);
+  fetch(ia - 1);
   put(ia - 1);
-  fetch(ia, pre);
-  fetch_cells(ia - 1, ia - 2, fa, ra, fb, rb);
-  for (int i = ia - 1; i <= iend; ++i) {
-    put(i + 1);
-    if (i + 2 <= iend + 1) fetch(i + 2, pre);
-    if (i + 1 <= iend) fetch_cells(i + 1, i, fa_n, ra_n, fb_n, rb_n);
-    lds_barrier();
-    if (i - 2 >= ia) store_plane(i - 2);
-    if (i >= 0 && i < nx) run_colour(sa, kA, fa, ra, i, c0);  // first colour on plane i, tile + 1
-    lds_barrier();
-    if (i - 1 >= ia && i - 1 < iend) run_colour(sb, kB, fb, rb, i - 1, 1 - c0);  // second colour, plane i - 1
-#pragma unroll
-    for (int m = 0; m < kA; ++m) { fa[m] = fa_n[m]; ra[m] = ra_n[m]; }
-#pragma unroll
-    for (int m = 0; m < kB; ++m) { fb[m] = fb_n[m]; rb[m] = rb_n[m]; }
+  fetch(ia);
+  int i = ia - 1;
+  if (i & 1) {
+    fetch_cells(SwC<1>{}, i);
+    step(SwC<1>{}, i);
+    ++i;
+  } else {
+    fetch_cells(SwC<0>{}, i);
   }
+  for (; i + 1 <= iend; i += 2) {
+    step(SwC<0>{}, i);
+    step(SwC<1>{}, i + 1);
+  }
+  if (i <= iend) step(SwC<0>{}, i);
   lds_barrier();
   if (iend - 1 >= ia) store_plane(iend - 1);
 }

@@ -1085,7 +1085,8 @@ struct hfsi_fluid {
   int mix_tail_first = 0;
   double mix_kappa = 1.0;
   uint32_t mix_half_min_cells = 1u << 20;  // levels with at least this many cells store fp16
-  uint32_t mix_fused_min_cells = 1u << 23;  // fused two-colour sweeps (k_mix_sweep) on levels this large
+  uint32_t mix_fused_min_cells = 1u << 21;  // fused two-colour sweeps (k_mix_sweep) on levels this large
+  uint32_t mix_sweep_wgs = 1024;            // target workgroup count of a fused sweep (sets planes per chunk)
   hfsi::MixLevel ml[hfsi::kMixMaxLevels];
   void* mix_tail_args = nullptr;  // MixTailArgs (device)
   int mix_tail_outlet = -1;       // outlet switch the tail arguments were written for
@@ -1590,13 +1591,14 @@ static inline bool mix_fused(const hfsi_fluid* h, const MixLevel& m) {
   return m.e2 != nullptr && m.n >= h->mix_fused_min_cells;
 }
 
-static SweepGeo mix_sweep_geo(const MixLevel& m) {
+static SweepGeo mix_sweep_geo(const hfsi_fluid* h, const MixLevel& m) {
   SweepGeo sg;
   sg.kblocks = (m.nz + kSwTK - 1) / kSwTK;
   sg.jblocks = (m.ny + kSwTJ - 1) / kSwTJ;
-  // planes per workgroup: about 1024 workgroups, 8..32 planes (each chunk re-reads 3 planes)
-  uint32_t c = (uint32_t)(((uint64_t)m.nx * sg.kblocks * sg.jblocks + 512) / 1024);
-  c = c < 8 ? 8 : (c > 32 ? 32 : c);
+  // planes per workgroup: about mix_sweep_wgs workgroups, 4..64 planes (each chunk recomputes 3 planes)
+  const uint64_t wgs = h->mix_sweep_wgs;
+  uint32_t c = (uint32_t)(((uint64_t)m.nx * sg.kblocks * sg.jblocks + wgs / 2) / wgs);
+  c = c < 4 ? 4 : (c > 64 ? 64 : c);
   sg.ichunk = c > m.nx ? m.nx : c;
   sg.nblocks = sg.kblocks * sg.jblocks * ((m.nx + sg.ichunk - 1) / sg.ichunk);
   return sg;
@@ -1627,7 +1629,7 @@ static void mix_smooth(hfsi_fluid* h, int l, int outlet, const TR* rhs, const do
   MixLevel& m = h->ml[l];
   const int c0 = pre ? 0 : 1;
   if (mix_fused(h, m)) {
-    const SweepGeo sg = mix_sweep_geo(m);
+    const SweepGeo sg = mix_sweep_geo(h, m);
     const uint32_t grid = (sg.nblocks + HFSI_NXCD - 1) / HFSI_NXCD * HFSI_NXCD;
     OpGeo<float> g = mix_geo(h, l, outlet);
     for (int s = 0; s < sweeps; ++s) {
@@ -2816,6 +2818,15 @@ int hfsi_fluid_mixed_set_fused_min_cells(hfsi_fluid* h, int64_t min_cells) {
     return HFSI_ERR_ARG;
   }
   h->mix_fused_min_cells = (uint32_t)std::min<int64_t>(min_cells, (int64_t)UINT32_MAX);
+  return HFSI_OK;
+}
+
+int hfsi_fluid_mixed_set_sweep_workgroups(hfsi_fluid* h, int64_t workgroups) {
+  if (!h || workgroups < 1 || workgroups > (1 << 20)) {
+    set_error("hfsi_fluid_mixed_set_sweep_workgroups: workgroups must be in [1, 2^20]");
+    return HFSI_ERR_ARG;
+  }
+  h->mix_sweep_wgs = (uint32_t)workgroups;
   return HFSI_OK;
 }
 

@@ -272,9 +272,15 @@ int hfsi_fluid_mixed_level_shape(const hfsi_fluid* h, int32_t level, int32_t* sh
 int hfsi_fluid_mixed_set_half_min_cells(hfsi_fluid* h, int64_t min_cells);
 /* Fused sweeps: both colour passes of a red-black sweep in one marching pass
  * over memory, on grid-launched levels with nz even and at least min_cells
- * cells (default 2^23); other levels run one launch per colour pass.  Both
+ * cells (default 2^21); other levels run one launch per colour pass.  Both
  * forms give identical results. */
 int hfsi_fluid_mixed_set_fused_min_cells(hfsi_fluid* h, int64_t min_cells);
+/* Target number of workgroups of a fused sweep (default 1024): a workgroup
+ * marches a chunk of nx * tiles / workgroups planes (4 .. 64), and every
+ * chunk recomputes 3 planes of its neighbour, so fewer workgroups do less
+ * redundant work and keep fewer waves in flight.  Results do not depend on
+ * it. */
+int hfsi_fluid_mixed_set_sweep_workgroups(hfsi_fluid* h, int64_t workgroups);
 
 /* ---- learned pressure initial guess (no reference counterpart) -----------
  * A 3-level 3D U-Net (3x3x3 convolutions, channels 16 / 32 / 64, two per

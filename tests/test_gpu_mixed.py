@@ -139,8 +139,8 @@ def test_mixed_fused_sweeps_equal_colour_passes(case, half):
     """The fused marching sweep computes both colour passes of a sweep with
     the first colour's results rounded to the storage type in LDS, as the
     two-pass form rounds them in memory: the V-cycles agree bit for bit.
-    Cases cover partial (j, k) tiles, several i-chunks and, in wide_k,
-    several k-tiles (nz = 130)."""
+    Cases cover partial (j, k) tiles, chunks of 4 planes and whole columns
+    of planes per workgroup and, in wide_k, several k-tiles (nz = 130)."""
     _say(f"mixed fused sweeps {case} half={half}")
     if case == "sphere":
         w, obst = _sphere_case(16, 8, 1.2, 0.12, order=(1, 2, 0))
@@ -157,11 +157,15 @@ def test_mixed_fused_sweeps_equal_colour_passes(case, half):
     if half:
         _ok(s, s._lib.hfsi_fluid_mixed_set_half_min_cells(s._handle, 0))
     for outlet in (False, True):
-        _ok(s, s._lib.hfsi_fluid_mixed_set_fused_min_cells(s._handle, 0))
-        fused = _vcycle(s, r, outlet)
         _ok(s, s._lib.hfsi_fluid_mixed_set_fused_min_cells(s._handle, 1 << 40))
         passes = _vcycle(s, r, outlet)
-        np.testing.assert_array_equal(fused, passes)
+        _ok(s, s._lib.hfsi_fluid_mixed_set_fused_min_cells(s._handle, 0))
+        # chunks of 4 planes (many chunk seams) and of up to 64 planes (one chunk per tile column)
+        for wgs in (1 << 20, 1):
+            _ok(s, s._lib.hfsi_fluid_mixed_set_sweep_workgroups(s._handle, wgs))
+            fused = _vcycle(s, r, outlet)
+            np.testing.assert_array_equal(fused, passes)
+        _ok(s, s._lib.hfsi_fluid_mixed_set_sweep_workgroups(s._handle, 1024))
         want = X.vcycle(X.build_hierarchy(w, obst), r, outlet=outlet)
         np.testing.assert_allclose(fused, want, rtol=0, atol=(5e-3 if half else 1e-4) * np.abs(want).max())
 
