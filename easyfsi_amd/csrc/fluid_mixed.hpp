@@ -17,8 +17,8 @@
 //   * smoother: red-black Gauss-Seidel (omega = 1), red then black before the
 //     coarse correction, black then red after it (a symmetric V-cycle).
 //   * restriction: volume-weighted mean of the free children's residuals;
-//     a coarse cell with any obstacle child is an obstacle (the reference's
-//     coarse-mask rule).
+//     a coarse cell is solid only when none of its children is free (the
+//     reference's any-child rule thickens the solid on every level).
 //   * prolongation: per-axis linear interpolation between the parent and the
 //     nearer neighbouring coarse cell (graded weights from the cell centres),
 //     renormalised over free coarse cells.
@@ -171,6 +171,39 @@ __global__ void __launch_bounds__(kBlock) k_mix_rb(OpGeo<float> g, TE* e, const 
   }
 }
 
+// Residual rs * r - (den e - sum_open c e_nb) of one cell (0 on cells that are
+// not free), with explicit fused multiply-adds so that every kernel computing
+// it gives the same bits.
+__device__ inline float res_value(uint32_t f, float cxl, float cxh, float cyl, float cyh, float czl, float czh,
+                                  float ec, float exm, float exp_, float eym, float eyp, float ezm, float ezp,
+                                  float den_extra, float rs, float r) {
+  float nb = 0.0f, den = 0.0f;
+  if (f & kXm) { nb = fmaf(cxl, exm, nb); den += cxl; }
+  if (f & kXp) { nb = fmaf(cxh, exp_, nb); den += cxh; }
+  if (f & kYm) { nb = fmaf(cyl, eym, nb); den += cyl; }
+  if (f & kYp) { nb = fmaf(cyh, eyp, nb); den += cyh; }
+  if (f & kZm) { nb = fmaf(czl, ezm, nb); den += czl; }
+  if (f & kZp) { nb = fmaf(czh, ezp, nb); den += czh; }
+  den += den_extra;
+  return (f & kNotFree) ? 0.0f : fmaf(rs, r, -fmaf(den, ec, -nb));
+}
+
+// res_value of a free cell with all six faces open (the same operations in
+// the same order).
+__device__ inline float res_value_open(float cxl, float cxh, float cyl, float cyh, float czl, float czh, float ec,
+                                       float exm, float exp_, float eym, float eyp, float ezm, float ezp,
+                                       float den_extra, float rs, float r) {
+  float nb = 0.0f, den = 0.0f;
+  nb = fmaf(cxl, exm, nb); den += cxl;
+  nb = fmaf(cxh, exp_, nb); den += cxh;
+  nb = fmaf(cyl, eym, nb); den += cyl;
+  nb = fmaf(cyh, eyp, nb); den += cyh;
+  nb = fmaf(czl, ezm, nb); den += czl;
+  nb = fmaf(czh, ezp, nb); den += czh;
+  den += den_extra;
+  return fmaf(rs, r, -fmaf(den, ec, -nb));
+}
+
 // Residual rs * rhs - L e of every cell of a level (0 on cells that are not
 // free), first half of the two-pass restriction.
 template <typename TE, typename TR, typename TRes>
@@ -178,8 +211,7 @@ __global__ void __launch_bounds__(kBlock) k_mix_residual(OpGeo<float> g, const T
                                                           const double* rs_dev, TRes* res) {
   const float rs = rhs_scale(rs_dev);
   HFSI_CELL_LOOP(g.n) {
-    // every load is issued before the flag test (neighbours guarded by the grid bounds only), so a
-    // cell costs one memory round trip
+    // every load is issued before the flag test, so a cell costs one memory round trip
     const uint32_t k = c % g.nz;
     const uint32_t t = c / g.nz;
     const uint32_t j = t % g.ny, i = t / g.ny;
@@ -190,96 +222,155 @@ __global__ void __launch_bounds__(kBlock) k_mix_residual(OpGeo<float> g, const T
     const float exm = ldf<TE>(e, i > 0 ? c - g.sx : c), exp_ = ldf<TE>(e, i + 1 < g.nx ? c + g.sx : c);
     const float eym = ldf<TE>(e, j > 0 ? c - g.sy : c), eyp = ldf<TE>(e, j + 1 < g.ny ? c + g.sy : c);
     const float ezm = ldf<TE>(e, k > 0 ? c - 1 : c), ezp = ldf<TE>(e, k + 1 < g.nz ? c + 1 : c);
-    float out = 0.0f;
-    if (!(f & kNotFree)) {
-      float nb = 0.0f, den = 0.0f;
-      if (f & kXm) { const float q = g.cl[0][i]; nb += q * exm; den += q; }
-      if (f & kXp) { const float q = g.ch[0][i]; nb += q * exp_; den += q; }
-      if (f & kYm) { const float q = g.cl[1][j]; nb += q * eym; den += q; }
-      if (f & kYp) { const float q = g.ch[1][j]; nb += q * eyp; den += q; }
-      if (f & kZm) { const float q = g.cl[2][k]; nb += q * ezm; den += q; }
-      if (f & kZp) { const float q = g.ch[2][k]; nb += q * ezp; den += q; }
-      if (g.outlet && k == 0) den += g.outlet_add;
-      out = rs * r - (den * ec - nb);
-    }
-    stf<TRes>(res, c, out);
+    stf<TRes>(res, c, res_value(f, g.cl[0][i], g.ch[0][i], g.cl[1][j], g.ch[1][j], g.cl[2][k], g.ch[2][k], ec, exm,
+                                exp_, eym, eyp, ezm, ezp, (g.outlet && k == 0) ? g.outlet_add : 0.0f, rs, r));
   }
+}
+
+// Two storage values at once (a k-pair, 2-element aligned).
+template <typename T>
+struct Two;
+template <>
+struct Two<__half> {
+  __half2 v;
+  __device__ static inline Two load(const __half* p) { return Two{*reinterpret_cast<const __half2*>(p)}; }
+  __device__ inline float lo() const { return __low2float(v); }
+  __device__ inline float hi() const { return __high2float(v); }
+  __device__ static inline void store(__half* p, float a, float b) {
+    *reinterpret_cast<__half2*>(p) = __halves2half2(__float2half_rn(a), __float2half_rn(b));
+  }
+};
+template <>
+struct Two<float> {
+  float2 v;
+  __device__ static inline Two load(const float* p) { return Two{*reinterpret_cast<const float2*>(p)}; }
+  __device__ inline float lo() const { return v.x; }
+  __device__ inline float hi() const { return v.y; }
+  __device__ static inline void store(float* p, float a, float b) { *reinterpret_cast<float2*>(p) = make_float2(a, b); }
+};
+
+// The same residual on a level with nz even, a k-pair (2q, 2q + 1) of one row
+// per thread: 2-element loads of the pair and of its four +-i / +-j
+// neighbour pairs, the two k neighbours, the pair's right-hand sides and
+// flags.  fq divides by nz / 2 (pairs per row), fy by ny.
+template <typename TE, typename TR, typename TRes>
+__global__ void __launch_bounds__(kBlock) k_mix_residual_pairs(OpGeo<float> g, FastDiv fq, FastDiv fy, const TE* e,
+                                                                const TR* rhs, const double* rs_dev, TRes* res) {
+  const float rs = rhs_scale(rs_dev);
+  const uint32_t np = g.n >> 1;
+  HFSI_CELL_LOOP(np) {
+    const uint32_t row = fq.div(c), k = 2 * (c - row * fq.d);
+    const uint32_t i = fy.div(row), j = row - i * g.ny;
+    const uint32_t c0 = 2 * c;  // first cell of the pair
+    using E2 = Two<TE>;
+    using R2 = Two<TR>;
+    const E2 ec = E2::load(e + c0);
+    const E2 exm = E2::load(e + (i > 0 ? c0 - g.sx : c0)), exp_ = E2::load(e + (i + 1 < g.nx ? c0 + g.sx : c0));
+    const E2 eym = E2::load(e + (j > 0 ? c0 - g.sy : c0)), eyp = E2::load(e + (j + 1 < g.ny ? c0 + g.sy : c0));
+    const float ezm = ldf<TE>(e, k > 0 ? c0 - 1 : c0), ezp = ldf<TE>(e, k + 2 < g.nz ? c0 + 2 : c0);
+    const R2 r = R2::load(rhs + c0);
+    const uint32_t f2 = *reinterpret_cast<const uint16_t*>(g.flags + c0);
+    const float cxl = g.cl[0][i], cxh = g.ch[0][i], cyl = g.cl[1][j], cyh = g.ch[1][j];
+    const float2 czl = *reinterpret_cast<const float2*>(g.cl[2] + k), czh = *reinterpret_cast<const float2*>(g.ch[2] + k);
+    const float extra = (g.outlet && k == 0) ? g.outlet_add : 0.0f;
+    float a, b;
+    if (__builtin_amdgcn_ballot_w64(!all_open(f2 & 0xffu) || !all_open(f2 >> 8)) == 0) {
+      // interior wavefront: no face tests
+      a = res_value_open(cxl, cxh, cyl, cyh, czl.x, czh.x, ec.lo(), exm.lo(), exp_.lo(), eym.lo(), eyp.lo(), ezm,
+                         ec.hi(), extra, rs, r.lo());
+      b = res_value_open(cxl, cxh, cyl, cyh, czl.y, czh.y, ec.hi(), exm.hi(), exp_.hi(), eym.hi(), eyp.hi(), ec.lo(),
+                         ezp, 0.0f, rs, r.hi());
+    } else {
+      a = res_value(f2 & 0xffu, cxl, cxh, cyl, cyh, czl.x, czh.x, ec.lo(), exm.lo(), exp_.lo(), eym.lo(), eyp.lo(),
+                    ezm, ec.hi(), extra, rs, r.lo());
+      b = res_value(f2 >> 8, cxl, cxh, cyl, cyh, czl.y, czh.y, ec.hi(), exm.hi(), exp_.hi(), eym.hi(), eyp.hi(),
+                    ec.lo(), ezp, 0.0f, rs, r.hi());
+    }
+    Two<TRes>::store(res + c0, a, b);
+  }
+}
+
+// Load of element idx with a 32-bit byte offset from a uniform base (one
+// vector address register, no 64-bit address arithmetic).
+template <typename T>
+__device__ inline T ld_at(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (uint32_t)sizeof(T));
 }
 
 // Second half: coarse rhs = volume-weighted mean of the stored residuals of
 // the free children (cells that are not free hold 0 and weigh nothing).
+// All 8 (child-count-clamped) children are loaded together; a missing second
+// child along an axis has width 0 and a solid child weight 0, so the sums run
+// over the same terms in the same order as a loop over the real children
+// (zero terms leave them unchanged).  fz / fy divide by the coarse nz / ny.
 template <typename TRes, typename TC>
-__global__ void __launch_bounds__(kBlock) k_mix_restrict_res(OpGeo<float> g, const TRes* res, MixXfer x, uint32_t cnx,
-                                                              uint32_t cny, uint32_t cnz, const uint8_t* cflags,
+__global__ void __launch_bounds__(kBlock) k_mix_restrict_res(OpGeo<float> g, FastDiv fz, FastDiv fy, const TRes* res,
+                                                              MixXfer x, uint32_t cn, const uint8_t* cflags,
                                                               TC* crhs) {
-  const uint32_t cn = cnx * cny * cnz;
   HFSI_CELL_LOOP(cn) {
-    const uint32_t K = c % cnz;
-    const uint32_t t = c / cnz;
-    const uint32_t J = t % cny, I = t / cny;
-    const uint8_t cf = cflags[c];
-    const int32_t i0 = x.first[0][I], j0 = x.first[1][J], k0 = x.first[2][K];
-    const int32_t ni = x.cnt[0][I], nj = x.cnt[1][J], nk = x.cnt[2][K];
-    // the (up to) 8 children's residuals, flags and widths, loaded together (a missing second child
-    // along an axis repeats the first and is skipped below)
-    float rv[2][2][2], wx[2], wy[2], wz[2];
-    uint8_t fv[2][2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      wx[a] = g.wf[0][(uint32_t)(i0 + (a < ni ? a : 0))];
-      wy[a] = g.wf[1][(uint32_t)(j0 + (a < nj ? a : 0))];
-      wz[a] = g.wf[2][(uint32_t)(k0 + (a < nk ? a : 0))];
-    }
+    const uint32_t t = fz.div(c), K = c - t * fz.d;
+    const uint32_t I = fy.div(t), J = t - I * fy.d;
+    const uint32_t cf = cflags[c];
+    const uint32_t i0 = (uint32_t)ld_at(x.first[0], I), j0 = (uint32_t)ld_at(x.first[1], J);
+    const uint32_t k0 = (uint32_t)ld_at(x.first[2], K);
+    const int32_t ni = ld_at(x.cnt[0], I), nj = ld_at(x.cnt[1], J), nk = ld_at(x.cnt[2], K);
+    const uint32_t i1 = i0 + (ni > 1 ? 1u : 0u), j1 = j0 + (nj > 1 ? 1u : 0u), k1 = k0 + (nk > 1 ? 1u : 0u);
+    // (unconditional loads, then selects: a conditional load would cost a round trip of its own)
+    const float wx1 = ld_at(g.wf[0], i1), wy1 = ld_at(g.wf[1], j1), wz1 = ld_at(g.wf[2], k1);
+    const float wx[2] = {ld_at(g.wf[0], i0), ni > 1 ? wx1 : 0.0f};
+    const float wy[2] = {ld_at(g.wf[1], j0), nj > 1 ? wy1 : 0.0f};
+    const float wz[2] = {ld_at(g.wf[2], k0), nk > 1 ? wz1 : 0.0f};
+    const uint32_t ii[2] = {i0 * g.sx, i1 * g.sx}, jj[2] = {j0 * g.sy, j1 * g.sy}, kk[2] = {k0, k1};
+    float rv[2][2][2];
+    uint32_t fv[2][2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-          const uint32_t cc = (uint32_t)(i0 + (a < ni ? a : 0)) * g.sx + (uint32_t)(j0 + (b < nj ? b : 0)) * g.sy +
-                              (uint32_t)(k0 + (d < nk ? d : 0));
-          fv[a][b][d] = g.flags[cc];
-          rv[a][b][d] = ldf<TRes>(res, cc);
+          const uint32_t cc = ii[a] + jj[b] + kk[d];
+          fv[a][b][d] = ld_at(g.flags, cc);
+          rv[a][b][d] = to_c_exact(ld_at(res, cc));
         }
-    float out = 0.0f;
-    if (!(cf & kNotFree)) {
-      float s = 0.0f, vs = 0.0f;
-      for (int32_t oi = 0; oi < ni; ++oi) {
-        const float wi = wx[oi];
-        for (int32_t oj = 0; oj < nj; ++oj) {
-          const float wij = wi * wy[oj];
-          for (int32_t ok = 0; ok < nk; ++ok) {
-            if (fv[oi][oj][ok] & kNotFree) continue;
-            const float V = wij * wz[ok];
-            s += V * rv[oi][oj][ok];
-            vs += V;
-          }
+    float s = 0.0f, vs = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const float wij = wx[a] * wy[b];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const float V = (fv[a][b][d] & kNotFree) ? 0.0f : wij * wz[d];
+          s = fmaf(V, rv[a][b][d], s);
+          vs += V;
         }
       }
-      out = vs > 0.0f ? s / vs : 0.0f;
-    }
-    stf<TC>(crhs, c, out);
+    stf<TC>(crhs, c, (!(cf & kNotFree) && vs > 0.0f) ? s / vs : 0.0f);
   }
 }
 
 // e_f += P e_c on free fine cells (per-axis linear interpolation, renormalised
-// over free coarse cells).
+// over free coarse cells).  Weights of 0 (the partner at a wall, where it is
+// the parent again) and solid coarse cells add exact zeros, so the sums equal
+// those over the contributing cells alone.  fz / fy divide by the fine nz /
+// ny.
 template <typename TF, typename TCo>
-__global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, MixXfer x, uint32_t cny, uint32_t cnz,
-                                                         const uint8_t* cflags, const TCo* ec) {
+__global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, FastDiv fz, FastDiv fy, TF* ef, MixXfer x,
+                                                         uint32_t cny, uint32_t cnz, const uint8_t* cflags,
+                                                         const TCo* ec) {
   const uint32_t csx = cny * cnz, csy = cnz;
   HFSI_CELL_LOOP(g.n) {
-    const uint32_t k = c % g.nz;
-    const uint32_t t = c / g.nz;
-    const uint32_t j = t % g.ny, i = t / g.ny;
-    const uint8_t f = g.flags[c];
+    const uint32_t t = fz.div(c), k = c - t * fz.d;
+    const uint32_t i = fy.div(t), j = t - i * fy.d;
+    const uint32_t f = g.flags[c];
     const float efc = ldf<TF>(ef, c);
-    const int32_t I[2] = {x.par[0][i], x.pnb[0][i]};
-    const int32_t J[2] = {x.par[1][j], x.pnb[1][j]};
-    const int32_t K[2] = {x.par[2][k], x.pnb[2][k]};
-    const float wI = x.pw[0][i], wJ = x.pw[1][j], wK = x.pw[2][k];
+    const uint32_t I[2] = {(uint32_t)ld_at(x.par[0], i) * csx, (uint32_t)ld_at(x.pnb[0], i) * csx};
+    const uint32_t J[2] = {(uint32_t)ld_at(x.par[1], j) * csy, (uint32_t)ld_at(x.pnb[1], j) * csy};
+    const uint32_t K[2] = {(uint32_t)ld_at(x.par[2], k), (uint32_t)ld_at(x.pnb[2], k)};
+    const float wI = ld_at(x.pw[0], i), wJ = ld_at(x.pw[1], j), wK = ld_at(x.pw[2], k);
     // the 8 coarse flags and values, loaded together
-    uint8_t cf[2][2][2];
+    uint32_t cf[2][2][2];
     float cv[2][2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -287,26 +378,35 @@ __global__ void __launch_bounds__(kBlock) k_mix_prolong(OpGeo<float> g, TF* ef, 
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-          const uint32_t cc = (uint32_t)I[a] * csx + (uint32_t)J[b] * csy + (uint32_t)K[d];
-          cf[a][b][d] = cflags[cc];
-          cv[a][b][d] = ldf<TCo>(ec, cc);
+          const uint32_t cc = I[a] + J[b] + K[d];
+          cf[a][b][d] = ld_at(cflags, cc);
+          cv[a][b][d] = to_c_exact(ld_at(ec, cc));
         }
-    if (f & kNotFree) continue;
-    const float wi[2] = {wI, 1.0f - wI}, wj[2] = {wJ, 1.0f - wJ}, wk[2] = {wK, 1.0f - wK};
-    float s = 0.0f, ws = 0.0f;
+    // pin the loads here (issued together, one round trip): otherwise the compiler defers the value loads
+    // into the branch of the store, a round trip of their own
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
+        for (int d = 0; d < 2; ++d) asm volatile("" : "+v"(cv[a][b][d]));
+    // every value is used (weight 0 for solid coarse cells, whose stored values are 0), so no load is
+    // deferred behind a branch: one memory round trip for the tables, one for the coarse cells
+    const float wi[2] = {wI, 1.0f - wI}, wj[2] = {wJ, 1.0f - wJ}, wk[2] = {wK, 1.0f - wK};
+    float s = 0.0f, ws = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const float wab = wi[a] * wj[b];
+#pragma unroll
         for (int d = 0; d < 2; ++d) {
-          const float w = wi[a] * wj[b] * wk[d];
-          if (w == 0.0f) continue;
-          if (cf[a][b][d] & kNotFree) continue;
-          s += w * cv[a][b][d];
+          const float w = (cf[a][b][d] & kNotFree) ? 0.0f : wab * wk[d];
+          s = fmaf(w, cv[a][b][d], s);
           ws += w;
         }
-    if (ws > 0.0f) stf<TF>(ef, c, efc + s / ws);
+      }
+    if (!(f & kNotFree) && ws > 0.0f) stf<TF>(ef, c, efc + s / ws);
   }
 }
 

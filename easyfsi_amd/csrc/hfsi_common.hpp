@@ -82,6 +82,23 @@ __device__ inline ChunkRange xcd_chunks(uint32_t nchunks) {
     for (uint32_t c = _ch * blockDim.x + threadIdx.x, _once = 1; _once; _once = 0) \
       if (c < (n))
 
+// Division by a divisor d (1 <= d < 2^31) fixed at launch, for numerators
+// n < 2^31, as a multiply-high and a shift (T. Granlund and P. Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, sec. 4):
+// with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1 < 2^32,
+// floor(n / d) = (mulhi(m, n) + n) >> l; the sum stays below 2^32 because
+// mulhi(m, n) < n < 2^31.
+struct FastDiv {
+  uint32_t d, m, l;
+  static inline FastDiv make(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d) ++l;
+    const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+    return FastDiv{d, (uint32_t)m, l};
+  }
+  __device__ inline uint32_t div(uint32_t n) const { return (__umulhi(m, n) + n) >> l; }
+};
+
 // Number of workgroups for a full-grid launch over n cells.
 inline uint32_t grid_blocks(uint64_t n, uint32_t max_blocks) {
   uint64_t chunks = (n + kBlock - 1) / kBlock;

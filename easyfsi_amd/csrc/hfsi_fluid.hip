@@ -1662,10 +1662,15 @@ static void mix_restrict(hfsi_fluid* h, int l, int outlet, const TR* rhs, const 
   const MixLevel& c = h->ml[l + 1];
   TE* e = (TE*)f.cur;
   OpGeo<float> g = mix_geo(h, l, outlet);
-  hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, (const TE*)e, rhs,
-                     rs, (TE*)f.res);
-  hipLaunchKernelGGL((k_mix_restrict_res<TE, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, g, (const TE*)f.res,
-                     mix_xfer(h, l), c.nx, c.ny, c.nz, (const uint8_t*)c.flags, (TC*)c.rhs);
+  if (f.nz % 2 == 0)
+    hipLaunchKernelGGL((k_mix_residual_pairs<TE, TR, TE>), dim3(blocks_for(f.n / 2)), dim3(kBlock), 0, st, g,
+                       FastDiv::make(f.nz / 2), FastDiv::make(f.ny), (const TE*)e, rhs, rs, (TE*)f.res);
+  else
+    hipLaunchKernelGGL((k_mix_residual<TE, TR, TE>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, g, (const TE*)e,
+                       rhs, rs, (TE*)f.res);
+  hipLaunchKernelGGL((k_mix_restrict_res<TE, TC>), dim3(blocks_for(c.n)), dim3(kBlock), 0, st, g,
+                     FastDiv::make(c.nz), FastDiv::make(c.ny), (const TE*)f.res, mix_xfer(h, l), c.n,
+                     (const uint8_t*)c.flags, (TC*)c.rhs);
 }
 
 template <typename TF, typename TCo>
@@ -1673,8 +1678,9 @@ static void mix_prolong(hfsi_fluid* h, int l, int outlet, hipStream_t st) {
   const MixLevel& c = h->ml[l + 1];
   const MixLevel& f = h->ml[l];
   TF* ef = (TF*)f.cur;
-  hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet), ef,
-                     mix_xfer(h, l), c.ny, c.nz, (const uint8_t*)c.flags, (const TCo*)c.cur);
+  hipLaunchKernelGGL((k_mix_prolong<TF, TCo>), dim3(blocks_for(f.n)), dim3(kBlock), 0, st, mix_geo(h, l, outlet),
+                     FastDiv::make(f.nz), FastDiv::make(f.ny), ef, mix_xfer(h, l), c.ny, c.nz,
+                     (const uint8_t*)c.flags, (const TCo*)c.cur);
 }
 
 // Down-leg of level l (smoothing + restriction), with the storage types of l
