@@ -720,6 +720,9 @@ struct MixTailArgs {
   const double* dense_inv;  // row-major [dense_m][dense_m]
 };
 
+// One colour pass on a tail level.  All loads of a cell (flag, right-hand
+// side, coefficients, the six neighbours from indices clamped into the grid)
+// are issued together: a closed face's value is never used.
 __device__ inline void mix_tail_rb(const OpGeo<float>& g, float* e, const float* rhs, int color, bool from_zero) {
   const uint32_t kp = (g.nz + 1) >> 1;
   const uint32_t npairs = g.nx * g.ny * kp;
@@ -736,20 +739,49 @@ __device__ inline void mix_tail_rb(const OpGeo<float>& g, float* e, const float*
     if (k >= g.nz) continue;
     const uint32_t cc = row + k;
     const uint8_t f = g.flags[cc];
-    float v = 0.0f;
-    if (!(f & kNotFree)) {
-      float nb = 0.0f, den = 0.0f;
-      if (f & kXm) { const float q = g.cl[0][i]; if (!from_zero) nb += q * e[cc - g.sx]; den += q; }
-      if (f & kXp) { const float q = g.ch[0][i]; if (!from_zero) nb += q * e[cc + g.sx]; den += q; }
-      if (f & kYm) { const float q = g.cl[1][j]; if (!from_zero) nb += q * e[cc - g.sy]; den += q; }
-      if (f & kYp) { const float q = g.ch[1][j]; if (!from_zero) nb += q * e[cc + g.sy]; den += q; }
-      if (f & kZm) { const float q = g.cl[2][k]; if (!from_zero) nb += q * e[cc - 1]; den += q; }
-      if (f & kZp) { const float q = g.ch[2][k]; if (!from_zero) nb += q * e[cc + 1]; den += q; }
-      if (g.outlet && k == 0) den += g.outlet_add;
-      if (den > 0.0f) v = (nb + rhs[cc]) / den;
+    const float r = rhs[cc];
+    const float cxl = g.cl[0][i], cxh = g.ch[0][i], cyl = g.cl[1][j], cyh = g.ch[1][j];
+    const float czl = g.cl[2][k], czh = g.ch[2][k];
+    float exm = 0.0f, exp_ = 0.0f, eym = 0.0f, eyp = 0.0f, ezm = 0.0f, ezp = 0.0f;
+    if (!from_zero) {
+      exm = e[i > 0 ? cc - g.sx : cc];
+      exp_ = e[i + 1 < g.nx ? cc + g.sx : cc];
+      eym = e[j > 0 ? cc - g.sy : cc];
+      eyp = e[j + 1 < g.ny ? cc + g.sy : cc];
+      ezm = e[k > 0 ? cc - 1 : cc];
+      ezp = e[k + 1 < g.nz ? cc + 1 : cc];
     }
-    e[cc] = v;
+    float nb = 0.0f, den = 0.0f;
+    if (f & kXm) { nb += cxl * exm; den += cxl; }
+    if (f & kXp) { nb += cxh * exp_; den += cxh; }
+    if (f & kYm) { nb += cyl * eym; den += cyl; }
+    if (f & kYp) { nb += cyh * eyp; den += cyh; }
+    if (f & kZm) { nb += czl * ezm; den += czl; }
+    if (f & kZp) { nb += czh * ezp; den += czh; }
+    if (g.outlet && k == 0) den += g.outlet_add;
+    e[cc] = (!(f & kNotFree) && den > 0.0f) ? (nb + r) / den : 0.0f;
   }
+}
+
+// Residual of one cell of a tail level, (rhs - (den e - nb)), with all loads
+// issued together (neighbours from indices clamped into the grid).
+__device__ inline float mix_tail_res(const OpGeo<float>& g, const float* e, const float* rhs, uint32_t i, uint32_t j,
+                                     uint32_t k, uint32_t cc, uint32_t f) {
+  const float ec = e[cc], r = rhs[cc];
+  const float cxl = g.cl[0][i], cxh = g.ch[0][i], cyl = g.cl[1][j], cyh = g.ch[1][j];
+  const float czl = g.cl[2][k], czh = g.ch[2][k];
+  const float exm = e[i > 0 ? cc - g.sx : cc], exp_ = e[i + 1 < g.nx ? cc + g.sx : cc];
+  const float eym = e[j > 0 ? cc - g.sy : cc], eyp = e[j + 1 < g.ny ? cc + g.sy : cc];
+  const float ezm = e[k > 0 ? cc - 1 : cc], ezp = e[k + 1 < g.nz ? cc + 1 : cc];
+  float nb = 0.0f, den = 0.0f;
+  if (f & kXm) { nb += cxl * exm; den += cxl; }
+  if (f & kXp) { nb += cxh * exp_; den += cxh; }
+  if (f & kYm) { nb += cyl * eym; den += cyl; }
+  if (f & kYp) { nb += cyh * eyp; den += cyh; }
+  if (f & kZm) { nb += czl * ezm; den += czl; }
+  if (f & kZp) { nb += czh * ezp; den += czh; }
+  if (g.outlet && k == 0) den += g.outlet_add;
+  return r - (den * ec - nb);
 }
 
 __global__ void __launch_bounds__(kMixTailThreads) k_mix_tail(const MixTailArgs* A, int pre, int coarse, int post) {
@@ -762,39 +794,32 @@ __global__ void __launch_bounds__(kMixTailThreads) k_mix_tail(const MixTailArgs*
       mix_tail_rb(L.g, L.e, L.rhs, 1, false);
       __syncthreads();
     }
-    // residual + restriction into level l + 1
+    // residual + restriction into level l + 1: the (up to) 8 children of a coarse cell, a missing second
+    // child along an axis repeating the first with weight 0
     const MixTailLevel& Cl = A->lv[l + 1];
     const OpGeo<float>& g = L.g;
     for (uint32_t c = threadIdx.x; c < Cl.g.n; c += blockDim.x) {
       const uint32_t K = c % Cl.g.nz;
       const uint32_t t = c / Cl.g.nz;
       const uint32_t J = t % Cl.g.ny, I = t / Cl.g.ny;
-      float out = 0.0f;
-      if (!(Cl.g.flags[c] & kNotFree)) {
-        float s = 0.0f, vs = 0.0f;
-        for (int32_t oi = 0; oi < L.x.cnt[0][I]; ++oi)
-          for (int32_t oj = 0; oj < L.x.cnt[1][J]; ++oj)
-            for (int32_t ok = 0; ok < L.x.cnt[2][K]; ++ok) {
-              const uint32_t i = (uint32_t)(L.x.first[0][I] + oi), j = (uint32_t)(L.x.first[1][J] + oj),
-                             k = (uint32_t)(L.x.first[2][K] + ok);
-              const uint32_t cc = i * g.sx + j * g.sy + k;
-              const uint8_t f = g.flags[cc];
-              if (f & kNotFree) continue;
-              float nb = 0.0f, den = 0.0f;
-              if (f & kXm) { const float q = g.cl[0][i]; nb += q * L.e[cc - g.sx]; den += q; }
-              if (f & kXp) { const float q = g.ch[0][i]; nb += q * L.e[cc + g.sx]; den += q; }
-              if (f & kYm) { const float q = g.cl[1][j]; nb += q * L.e[cc - g.sy]; den += q; }
-              if (f & kYp) { const float q = g.ch[1][j]; nb += q * L.e[cc + g.sy]; den += q; }
-              if (f & kZm) { const float q = g.cl[2][k]; nb += q * L.e[cc - 1]; den += q; }
-              if (f & kZp) { const float q = g.ch[2][k]; nb += q * L.e[cc + 1]; den += q; }
-              if (g.outlet && k == 0) den += g.outlet_add;
-              const float V = g.wf[0][i] * g.wf[1][j] * g.wf[2][k];
-              s += V * (L.rhs[cc] - (den * L.e[cc] - nb));
-              vs += V;
-            }
-        out = vs > 0.0f ? s / vs : 0.0f;
-      }
-      Cl.rhs[c] = out;
+      const uint32_t i0 = (uint32_t)L.x.first[0][I], j0 = (uint32_t)L.x.first[1][J], k0 = (uint32_t)L.x.first[2][K];
+      const int32_t ni = L.x.cnt[0][I], nj = L.x.cnt[1][J], nk = L.x.cnt[2][K];
+      const uint32_t ci[2] = {i0, i0 + (ni > 1 ? 1u : 0u)}, cj[2] = {j0, j0 + (nj > 1 ? 1u : 0u)};
+      const uint32_t ck[2] = {k0, k0 + (nk > 1 ? 1u : 0u)};
+      float s = 0.0f, vs = 0.0f;
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          for (int d = 0; d < 2; ++d) {
+            const uint32_t i = ci[a], j = cj[b], k = ck[d];
+            const uint32_t cc = i * g.sx + j * g.sy + k;
+            const uint32_t f = g.flags[cc];
+            const float res = mix_tail_res(g, L.e, L.rhs, i, j, k, cc, f);
+            const bool real = a < ni && b < nj && d < nk && !(f & kNotFree);
+            const float V = real ? g.wf[0][i] * g.wf[1][j] * g.wf[2][k] : 0.0f;
+            s += V * res;
+            vs += V;
+          }
+      Cl.rhs[c] = (!(Cl.g.flags[c] & kNotFree) && vs > 0.0f) ? s / vs : 0.0f;
     }
     __syncthreads();
   }
@@ -830,10 +855,12 @@ __global__ void __launch_bounds__(kMixTailThreads) k_mix_tail(const MixTailArgs*
     const OpGeo<float>& g = L.g;
     const uint32_t csx = Cl.g.ny * Cl.g.nz, csy = Cl.g.nz;
     for (uint32_t c = threadIdx.x; c < g.n; c += blockDim.x) {
-      if (g.flags[c] & kNotFree) continue;
+      // all loads first (weight 0 for solid coarse cells and zero weights), the flag test last
       const uint32_t k = c % g.nz;
       const uint32_t t = c / g.nz;
       const uint32_t j = t % g.ny, i = t / g.ny;
+      const uint32_t f = g.flags[c];
+      const float ecur = L.e[c];
       const int32_t I[2] = {L.x.par[0][i], L.x.pnb[0][i]};
       const int32_t J[2] = {L.x.par[1][j], L.x.pnb[1][j]};
       const int32_t K[2] = {L.x.par[2][k], L.x.pnb[2][k]};
@@ -844,14 +871,13 @@ __global__ void __launch_bounds__(kMixTailThreads) k_mix_tail(const MixTailArgs*
       for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
           for (int d = 0; d < 2; ++d) {
-            const float w = wi[a] * wj[b] * wk[d];
-            if (w == 0.0f) continue;
             const uint32_t cc = (uint32_t)I[a] * csx + (uint32_t)J[b] * csy + (uint32_t)K[d];
-            if (Cl.g.flags[cc] & kNotFree) continue;
+            const float w0 = wi[a] * wj[b] * wk[d];
+            const float w = (Cl.g.flags[cc] & kNotFree) ? 0.0f : w0;
             s += w * Cl.e[cc];
             ws += w;
           }
-      if (ws > 0.0f) L.e[c] += s / ws;
+      if (!(f & kNotFree) && ws > 0.0f) L.e[c] = ecur + s / ws;
     }
     __syncthreads();
     for (int s = 0; s < post; ++s) {
