@@ -366,6 +366,114 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_rr(OpGeo<double> g, double
   block_partials<1>(acc, partials, lds);
 }
 
+// ---------------------------------------------------------------------------
+// The same CG vector updates four cells per thread (a quad of one row, nz a
+// multiple of 4): 16-byte loads and stores, four independent cells per
+// memory round trip.  fq divides by nz / 4 (quads per row), fy by ny.  Each
+// cell is computed exactly as in the one-cell kernels.
+// ---------------------------------------------------------------------------
+
+struct QuadPos {
+  uint32_t c0, i, j, k0;
+};
+__device__ inline QuadPos quad_pos(uint32_t q, const FastDiv& fq, const FastDiv& fy) {
+  const uint32_t row = fq.div(q);
+  const uint32_t i = fy.div(row);
+  return QuadPos{4 * q, i, row - i * fy.d, 4 * (q - row * fq.d)};
+}
+
+template <typename TE>
+struct Quad;
+template <>
+struct Quad<__half> {
+  static __device__ inline void load(const __half* p, float v[4]) {
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    const __half2 a = *reinterpret_cast<const __half2*>(&u.x), b = *reinterpret_cast<const __half2*>(&u.y);
+    v[0] = __low2float(a); v[1] = __high2float(a); v[2] = __low2float(b); v[3] = __high2float(b);
+  }
+};
+template <>
+struct Quad<float> {
+  static __device__ inline void load(const float* p, float v[4]) {
+    const float4 u = *reinterpret_cast<const float4*>(p);
+    v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+  }
+};
+
+template <int NS, typename TE>
+__global__ void __launch_bounds__(kBlock) k_mix_finish4(OpGeo<double> g, FastDiv fq, FastDiv fy, const TE* e,
+                                                         const float* r, const float* q, float* z, const Scalars* s,
+                                                         double* partials) {
+  __shared__ double lds[NS * (kBlock / 64)];
+  double acc[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) acc[t] = 0.0;
+  const float vout = (float)s->vout;
+  HFSI_CELL_LOOP(g.n >> 2) {
+    const QuadPos P = quad_pos(c, fq, fy);
+    const uint32_t f4 = *reinterpret_cast<const uint32_t*>(g.flags + P.c0);
+    float ev[4];
+    Quad<TE>::load(e + P.c0, ev);
+    const float4 r4 = *reinterpret_cast<const float4*>(r + P.c0);
+    float4 q4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (NS == 5) q4 = *reinterpret_cast<const float4*>(q + P.c0);
+    const double wxy = g.wd[0][P.i] * g.wd[1][P.j];
+    const double2 wz01 = *reinterpret_cast<const double2*>(g.wd[2] + P.k0);
+    const double2 wz23 = *reinterpret_cast<const double2*>(g.wd[2] + P.k0 + 2);
+    const float rv[4] = {r4.x, r4.y, r4.z, r4.w}, qv[4] = {q4.x, q4.y, q4.z, q4.w};
+    const double wz[4] = {wz01.x, wz01.y, wz23.x, wz23.y};
+    float zo[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint32_t f = (f4 >> (8 * m)) & 0xffu;
+      if (f & kNotFree) {
+        zo[m] = 0.0f;
+        continue;
+      }
+      const float zf = ev[m] * vout;
+      zo[m] = zf;
+      const double V = wxy * wz[m];
+      const double zd = (double)zf, rd = (double)rv[m];
+      acc[0] += rd * zd * V;
+      if (NS == 5) {
+        const double qd = (double)qv[m];
+        acc[1] += zd * qd * V;
+        acc[2] += zd * V;
+        acc[3] += rd * V;
+        acc[4] += qd * V;
+      } else {
+        acc[1] += zd * V;
+        acc[2] += rd * V;
+      }
+    }
+    *reinterpret_cast<float4*>(z + P.c0) = make_float4(zo[0], zo[1], zo[2], zo[3]);
+  }
+  block_partials<NS>(acc, partials, lds);
+}
+
+// d = z (- mean) + beta d on four cells (OpCgDir on float vectors).
+__global__ void __launch_bounds__(kBlock) k_cgdir4(uint32_t n, const uint8_t* flags, const float* z, float* d,
+                                                    const Scalars* s, int remove_mean, int first) {
+  const double mz = s->mz, beta = s->beta;
+  HFSI_CELL_LOOP(n >> 2) {
+    const uint32_t c0 = 4 * c;
+    const uint32_t f4 = *reinterpret_cast<const uint32_t*>(flags + c0);
+    const float4 z4 = *reinterpret_cast<const float4*>(z + c0);
+    float4 d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (!first) d4 = *reinterpret_cast<const float4*>(d + c0);
+    const float zv4[4] = {z4.x, z4.y, z4.z, z4.w};
+    float* dv = reinterpret_cast<float*>(&d4);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint32_t f = (f4 >> (8 * m)) & 0xffu;
+      double zv = (double)zv4[m];
+      if (remove_mean) zv = !(f & kNotFree) ? zv - mz : 0.0;
+      dv[m] = (f & kObst) ? 0.0f : (first ? (float)zv : (float)(zv + beta * (double)dv[m]));
+    }
+    *reinterpret_cast<float4*>(d + c0) = d4;
+  }
+}
+
 // z = e * vout on free cells (0 elsewhere) and the PCG sums of FinDots5
 // (NS = 5: <r,z>, <z,q>, sum Vz, sum Vr, sum Vq) or FinDots3 (NS = 3: <r,z>,
 // sum Vz, sum Vr).
@@ -1180,6 +1288,19 @@ static int launch_pointwise(hfsi_fluid* h, int l, Op op, hipStream_t st) {
   return HFSI_OK;
 }
 
+// d = z (- mean) + beta d, four cells per thread on float vectors.
+template <typename TV>
+static int launch_cgdir(hfsi_fluid* h, const TV* z, TV* d, const Scalars* sc, int rm, int first, hipStream_t st) {
+  const LevelMem& m = h->lv[0];
+  if (std::is_same<TV, float>::value && m.n % 4 == 0) {
+    hipLaunchKernelGGL(k_cgdir4, dim3(blocks_for(m.n / 4)), dim3(kBlock), 0, st, m.n, m.flags,
+                       reinterpret_cast<const float*>(z), reinterpret_cast<float*>(d), sc, rm, first);
+    HFSI_LAUNCH_CHECK();
+    return HFSI_OK;
+  }
+  return launch_pointwise(h, 0, OpCgDir<TV, TV>{z, d, sc, rm, first}, st);
+}
+
 // Combine the partials of a reducing launch of `nblocks` workgroups and run `fin`.
 template <int NS, typename Fin>
 static void finish_sums(hfsi_fluid* h, uint32_t nblocks, Fin fin, hipStream_t st, uint64_t maxmask = 0) {
@@ -1752,13 +1873,23 @@ static int mix_precondition(hfsi_fluid* h, const float* r, const float* q, float
                             int outlet, Fin fin, hipStream_t st) {
   HFSI_TRY(run_mix_vcycle(h, r, pre, coarse, post, outlet, st));
   OpGeo<double> g64 = geo<double>(h, 0, outlet);
-  const uint32_t nb = blocks_for(g64.n);
-  if (mix_half(h, 0))
+  uint32_t nb = blocks_for(g64.n);
+  if (g64.nz % 4 == 0) {
+    nb = blocks_for(g64.n / 4);
+    const FastDiv fq = FastDiv::make(g64.nz / 4), fy = FastDiv::make(g64.ny);
+    if (mix_half(h, 0))
+      hipLaunchKernelGGL((k_mix_finish4<NS, __half>), dim3(nb), dim3(kBlock), 0, st, g64, fq, fy,
+                         (const __half*)h->ml[0].cur, r, q, z, h->scal, h->partials);
+    else
+      hipLaunchKernelGGL((k_mix_finish4<NS, float>), dim3(nb), dim3(kBlock), 0, st, g64, fq, fy,
+                         (const float*)h->ml[0].cur, r, q, z, h->scal, h->partials);
+  } else if (mix_half(h, 0)) {
     hipLaunchKernelGGL((k_mix_finish<NS, __half>), dim3(nb), dim3(kBlock), 0, st, g64, (const __half*)h->ml[0].cur,
                        r, q, z, h->scal, h->partials);
-  else
+  } else {
     hipLaunchKernelGGL((k_mix_finish<NS, float>), dim3(nb), dim3(kBlock), 0, st, g64, (const float*)h->ml[0].cur, r,
                        q, z, h->scal, h->partials);
+  }
   finish_sums<NS>(h, nb, fin, st);
   HFSI_LAUNCH_CHECK();
   return HFSI_OK;
@@ -1913,7 +2044,7 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
     S.breakdown_kind = HFSI_BREAKDOWN_INITIAL_DOT;
     return HFSI_OK;
   }
-  HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 1}, st));
+  HFSI_TRY(launch_cgdir<TV>(h, v.z, v.d, sc, rm, 1, st));
 
   // Convergence monitor (mixed solve, monitor_lag = L > 0): after each update of
   // x the float64 true residual norm of x (mean-free, not stored) goes to a pinned
@@ -1953,10 +2084,12 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
     monitor_pending = false;
     if (use_mix) {
       timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
-      hipLaunchKernelGGL(k_cg_update_rr, dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, (const float*)v.d,
-                         (float*)v.r, (const float*)v.q, sc, h->partials);
+      // (one cell per thread: a four-cell form with two 16-byte stores of x per thread measured slower)
+      const uint32_t nbu = blocks_for(g64.n);
+      hipLaunchKernelGGL(k_cg_update_rr, dim3(nbu), dim3(kBlock), 0, st, g64, x, (const float*)v.d, (float*)v.r,
+                         (const float*)v.q, sc, h->partials);
       timer_mark(h, HFSI_TIMER_CG_UPDATE, st);
-      finish_sums<1>(h, blocks_for(g64.n), FinVScale{sc, kappa}, st);
+      finish_sums<1>(h, nbu, FinVScale{sc, kappa}, st);
       if (monitor) {
         const FinTrueMonitor fin{sc, rm, h->mon_ring + (it % kMonRing)};
         if (march_ok(h, 0)) {
@@ -2024,7 +2157,7 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
         HFSI_TRY(sync_scalars(h, st));
         rz = h->host_scal->rz;
         if (std::isfinite(rz) && rz > 0.0) {
-          HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 1}, st));
+          HFSI_TRY(launch_cgdir<TV>(h, v.z, v.d, sc, rm, 1, st));
           Scalars fix = *h->host_scal;
           fix.code = 0;
           fix.code_dAd = 0.0;
@@ -2049,7 +2182,7 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
     if (rm) S.mean_projection_count += 1;
     if (!fd.done)
       HFSI_TRY(launch_sums<5>(h, g64, OpDots5<TV, TV, TV>{v.r, v.z, v.q}, FinDots5{sc, rm, use_mg ? 1 : 0}, st));
-    HFSI_TRY(launch_pointwise(h, 0, OpCgDir<TV, TV>{v.z, v.d, sc, rm, 0}, st));
+    HFSI_TRY(launch_cgdir<TV>(h, v.z, v.d, sc, rm, 0, st));
     if (monitor && it >= lag) {
       const int m = it - lag + 1;
       HFSI_CHECK(hipEventSynchronize(h->mon_events[m % kMonRing]));
