@@ -52,6 +52,20 @@ if ROOT not in sys.path:
 
 METRIC = "projection solves/s at 256³ to resid 1e-8: learned+mixed vs plain MGPCG"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the roofline kernel, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
+# scaled by the counts of a device-to-device copy of known size; the summary holds the command and factors
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_mix_sweep_256_graded_hbm_traffic.md")
+
+
+def measured_traffic(path: str = TRAFFIC_PROFILE):
+    """Bytes per launch recorded in the traffic summary (line 'traffic_bytes_per_launch: <float>'), or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def graded_axis(n_core: int, n_side: int, growth: float) -> np.ndarray:
@@ -542,7 +556,7 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic() if head_name in ("mixed", "learned_mixed") and args.size == 256 else None,
             "bytes_per_cell": bytes_per_cell,
             "avg_launch_us": round(avg_s * 1e6, 2),
             "launches": head["timer_launches"],
