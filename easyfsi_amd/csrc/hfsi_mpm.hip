@@ -450,6 +450,12 @@ constexpr int kBinEdge = 8;                 // nodes per block edge
 constexpr int kTileEdge = kBinEdge + 2;     // stencil reach of a block
 constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 
+// Components of a block's P2G tile: mass, momentum (3), external force (3).
+constexpr int kTileComps = 7;
+constexpr size_t kTileStride = (size_t)kTileComps * kTileNodes;  // floats per block
+// Blocks up to this many get a tile buffer (28 KB each); larger grids flush tiles with global atomics.
+constexpr size_t kMaxTileBlocks = 65536;
+
 struct BinArgs {
   int32_t nbx, nby, nbz;   // block grid
   int32_t* key;            // per particle: block, or -1 when out of bounds
@@ -462,6 +468,14 @@ struct BinArgs {
   int32_t* oob;            // out-of-bounds particles (unordered)
   int32_t* noob;
   int32_t* scan_sums;      // per 1024-block chunk
+  // tile path: every active block stores its whole tile (plain stores) and the grid update gathers each
+  // node from the (up to 8) tiles that cover it, in a fixed order
+  float* tiles;            // [block][kTileComps][kTileNodes], or nullptr (atomic flush)
+  int32_t* need_stamp;     // per block: stamp of the step that last listed it
+  int32_t* need_list;      // blocks with nodes a tile covers: the active blocks and their +1 neighbours
+  int32_t* nneed;
+  int32_t stamp;           // this step's stamp (never 0: need_stamp starts at 0)
+  int32_t has_force;       // particles carry an external force (force components stored)
 };
 
 __global__ void k_bin_clear(BinArgs b) {
@@ -470,6 +484,23 @@ __global__ void k_bin_clear(BinArgs b) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *b.nactive = 0;
     *b.noob = 0;
+    if (b.nneed) *b.nneed = 0;
+  }
+}
+
+// List every block a tile of an active block reaches (the block and its seven +1 neighbours) once: the
+// stamp exchange admits the first marker of a block in this step.
+__global__ void __launch_bounds__(kBlock) k_need_mark(BinArgs b) {
+  const int32_t nact = *b.nactive;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nact; i += gridDim.x * blockDim.x) {
+    const int32_t k = b.active[i];
+    const int32_t bx = k / (b.nby * b.nbz), by = (k / b.nbz) % b.nby, bz = k % b.nbz;
+    for (int e = 0; e < 8; ++e) {
+      const int32_t x = bx + (e >> 2), y = by + ((e >> 1) & 1), z = bz + (e & 1);
+      if (x >= b.nbx || y >= b.nby || z >= b.nbz) continue;
+      const int32_t k2 = (x * b.nby + y) * b.nbz + z;
+      if (atomicExch(&b.need_stamp[k2], b.stamp) != b.stamp) b.need_list[atomicAdd(b.nneed, 1)] = k2;
+    }
   }
 }
 
@@ -583,16 +614,30 @@ __global__ void __launch_bounds__(kBlock) k_mpm_p2g_binned(MpmArgs a, BinArgs b)
       });
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kTileNodes; i += blockDim.x) {
-      const int tz = i % kTileEdge, ty = (i / kTileEdge) % kTileEdge, tx = i / (kTileEdge * kTileEdge);
-      const int gx = ox0 + tx, gy = oy0 + ty, gz = oz0 + tz;
-      if (gx >= a.nx || gy >= a.ny || gz >= a.nz) continue;
-      const uint32_t node = (uint32_t)gx * sx + (uint32_t)gy * sy + (uint32_t)gz;
-      if (tm[i] != 0.0f) atomicAdd(&a.gm[node], tm[i]);
+    if (b.tiles) {
+      // the whole tile, plain stores (the grid update gathers it)
+      float* T = b.tiles + (size_t)k * kTileStride;
+      for (int i = threadIdx.x; i < kTileNodes; i += blockDim.x) {
+        T[i] = tm[i];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        if (tp[d][i] != 0.0f) atomicAdd(&a.gv[3 * (size_t)node + d], tp[d][i]);
-        if (tf[d][i] != 0.0f) atomicAdd(&a.gf[3 * (size_t)node + d], tf[d][i]);
+        for (int d = 0; d < 3; ++d) T[(1 + d) * kTileNodes + i] = tp[d][i];
+        if (b.has_force) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) T[(4 + d) * kTileNodes + i] = tf[d][i];
+        }
+      }
+    } else {
+      for (int i = threadIdx.x; i < kTileNodes; i += blockDim.x) {
+        const int tz = i % kTileEdge, ty = (i / kTileEdge) % kTileEdge, tx = i / (kTileEdge * kTileEdge);
+        const int gx = ox0 + tx, gy = oy0 + ty, gz = oz0 + tz;
+        if (gx >= a.nx || gy >= a.ny || gz >= a.nz) continue;
+        const uint32_t node = (uint32_t)gx * sx + (uint32_t)gy * sy + (uint32_t)gz;
+        if (tm[i] != 0.0f) atomicAdd(&a.gm[node], tm[i]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (tp[d][i] != 0.0f) atomicAdd(&a.gv[3 * (size_t)node + d], tp[d][i]);
+          if (tf[d][i] != 0.0f) atomicAdd(&a.gf[3 * (size_t)node + d], tf[d][i]);
+        }
       }
     }
     __syncthreads();  // the tile is reused by the next block
@@ -638,6 +683,70 @@ __global__ void __launch_bounds__(kBlock) k_mpm_grid(MpmArgs a) {
     }
     acc[6] += 1.0;
     acc[7] = fmax(acc[7], (double)sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+  }
+  block_partials<NS>(acc, a.partials, lds, kGridMax);
+}
+
+// Grid update of the tile path: every node of a listed block sums the tiles
+// that cover it (its own block's and, within 2 nodes of a low face, the lower
+// neighbours'), in a fixed order, then updates as k_mpm_grid and writes mass,
+// velocity and force.  Nodes of listed blocks without mass get the raw sums
+// (zero), as the cleared grid has.  Totals 12..19.
+__global__ void __launch_bounds__(kBlock) k_mpm_grid_gather(MpmArgs a, BinArgs b) {
+  constexpr int NS = 8;
+  __shared__ double lds[NS * (kBlock / 64)];
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  const int32_t nneed = *b.nneed;
+  const uint32_t sx = (uint32_t)a.ny * a.nz, sy = a.nz;
+  constexpr int kBlockNodes = kBinEdge * kBinEdge * kBinEdge;
+  for (int32_t w = blockIdx.x; w < nneed; w += gridDim.x) {
+    const int32_t B = b.need_list[w];
+    const int32_t bx = B / (b.nby * b.nbz), by = (B / b.nbz) % b.nby, bz = B % b.nbz;
+    for (int t = threadIdx.x; t < kBlockNodes; t += blockDim.x) {
+      const int lx = t / (kBinEdge * kBinEdge), ly = (t / kBinEdge) % kBinEdge, lz = t % kBinEdge;
+      const int gx = bx * kBinEdge + lx, gy = by * kBinEdge + ly, gz = bz * kBinEdge + lz;
+      if (gx >= a.nx || gy >= a.ny || gz >= a.nz) continue;
+      float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f}, f[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
+        if ((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2)) continue;
+        if (bx - ex < 0 || by - ey < 0 || bz - ez < 0) continue;
+        const int32_t k = ((bx - ex) * b.nby + (by - ey)) * b.nbz + (bz - ez);
+        if (b.count[k] <= 0) continue;
+        const float* T = b.tiles + (size_t)k * kTileStride;
+        const int ti = ((lx + ex * kBinEdge) * kTileEdge + (ly + ey * kBinEdge)) * kTileEdge + (lz + ez * kBinEdge);
+        m += T[ti];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) p[d] += T[(1 + d) * kTileNodes + ti];
+        if (b.has_force) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) f[d] += T[(4 + d) * kTileNodes + ti];
+        }
+      }
+      const uint32_t node = (uint32_t)gx * sx + (uint32_t)gy * sy + (uint32_t)gz;
+      a.gm[node] = m;
+      float v[3] = {p[0], p[1], p[2]};
+      if (m > 1.0e-20f) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          v[d] = p[d] / m;
+          v[d] += a.dt * f[d] / m;
+          acc[d] += (double)(m * v[d]);
+          v[d] *= a.damping;
+          acc[3 + d] += (double)(m * v[d]);
+        }
+        acc[6] += 1.0;
+        acc[7] = fmax(acc[7], (double)sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a.gv[3 * (size_t)node + d] = v[d];
+        a.gf[3 * (size_t)node + d] = f[d];
+      }
+    }
   }
   block_partials<NS>(acc, a.partials, lds, kGridMax);
 }
@@ -935,6 +1044,19 @@ int hfsi_mpm_create(hfsi_mpm** out, int32_t capacity, const double* bounds_min3,
       hfsi_mpm_destroy(h);
       return HFSI_ERR_HIP;
     }
+    if (nb <= kMaxTileBlocks) {
+      void* q = nullptr;
+      if (!(al(&b.need_stamp, nb) && al(&b.need_list, nb) && al(&b.nneed, 1) &&
+            hipMalloc(&q, nb * kTileStride * sizeof(float)) == hipSuccess &&
+            hipMemset(b.need_stamp, 0, nb * sizeof(int32_t)) == hipSuccess)) {
+        set_error("hfsi_mpm_create: device allocation failed");
+        if (q) (void)hipFree(q);
+        hfsi_mpm_destroy(h);
+        return HFSI_ERR_HIP;
+      }
+      h->allocs.push_back(q);
+      b.tiles = (float*)q;
+    }
   }
   bool ok = hipMalloc((void**)&h->partials, sizeof(double) * 2048 * 32) == hipSuccess &&
             hipMalloc((void**)&h->counter, sizeof(unsigned int)) == hipSuccess &&
@@ -1028,7 +1150,9 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* P, int32_t count, const
   hipLaunchKernelGGL(k_mpm_clear, dim3(mpm_blocks(nodes)), dim3(kBlock), 0, st, nodes, a.gm, a.gv, a.gf);
   const uint32_t pb = mpm_blocks(count), nb = mpm_blocks(nodes);
   if (h->p2g_mode == HFSI_MPM_P2G_BINNED) {
-    const BinArgs& b = h->bins;
+    BinArgs& b = h->bins;
+    b.stamp = b.stamp == INT32_MAX ? 1 : b.stamp + 1;
+    b.has_force = a.f_ext != nullptr;
     const int32_t nb = b.nbx * b.nby * b.nbz;
     const int32_t nchunks = (nb + kScanChunk - 1) / kScanChunk;
     const uint32_t G = (uint32_t)std::min<int32_t>(nb, 2048), G2 = 64;
@@ -1039,6 +1163,7 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* P, int32_t count, const
     hipLaunchKernelGGL(k_scan_add, dim3(nchunks), dim3(kScanChunk), 0, st, b.offset, nb, (const int32_t*)b.scan_sums,
                        nchunks);
     hipLaunchKernelGGL(k_bin_scatter, dim3(pb), dim3(kBlock), 0, st, count, b);
+    if (b.tiles) hipLaunchKernelGGL(k_need_mark, dim3(std::min<int32_t>((nb + 255) / 256, 256)), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_mpm_p2g_binned, dim3(G), dim3(kBlock), 0, st, a, b);
     hipLaunchKernelGGL(k_mpm_p2g_oob, dim3(G2), dim3(kBlock), 0, st, a, b, a.partials + (size_t)G * 12);
     hipLaunchKernelGGL((k_finish_sums<12, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials,
@@ -1048,9 +1173,16 @@ int hfsi_mpm_step(hfsi_mpm* h, const hfsi_mpm_particles* P, int32_t count, const
     hipLaunchKernelGGL((k_finish_sums<12, FinNone>), dim3(1), dim3(kFinishThreads), 0, st,
                        (const double*)a.partials, pb, a.totals + 0, FinNone{}, kP2gMax);
   }
-  hipLaunchKernelGGL(k_mpm_grid, dim3(nb), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL((k_finish_sums<8, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials, nb,
-                     a.totals + T_TRANS, FinNone{}, kGridMax);
+  if (h->p2g_mode == HFSI_MPM_P2G_BINNED && h->bins.tiles) {
+    const uint32_t GG = 2048;
+    hipLaunchKernelGGL(k_mpm_grid_gather, dim3(GG), dim3(kBlock), 0, st, a, h->bins);
+    hipLaunchKernelGGL((k_finish_sums<8, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials,
+                       GG, a.totals + T_TRANS, FinNone{}, kGridMax);
+  } else {
+    hipLaunchKernelGGL(k_mpm_grid, dim3(nb), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_finish_sums<8, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials,
+                       nb, a.totals + T_TRANS, FinNone{}, kGridMax);
+  }
   hipLaunchKernelGGL(k_mpm_g2p, dim3(pb), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_finish_sums<21, FinNone>), dim3(1), dim3(kFinishThreads), 0, st, (const double*)a.partials,
                      pb, a.totals + T_PDISP, FinNone{}, (uint64_t)0);

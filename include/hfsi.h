@@ -237,10 +237,12 @@ int hfsi_mpm_create(hfsi_mpm** out, int32_t particle_capacity, const double* bou
 int hfsi_mpm_destroy(hfsi_mpm* h);
 
 /* P2G scatter: HFSI_MPM_P2G_BINNED (default) buckets particles by 8^3-node
- * block and accumulates each block in an LDS tile (one global atomic per
- * touched node value); HFSI_MPM_P2G_ATOMIC scatters every particle's 27 node
- * contributions with global float atomics.  Same arithmetic per particle;
- * float summation order differs. */
+ * block and accumulates each block in an LDS tile; the tiles are stored
+ * whole and the grid update sums, for every node, the (up to 8) tiles that
+ * cover it in a fixed order (grids of more than 65536 blocks add each tile
+ * to the grid with global atomics instead).  HFSI_MPM_P2G_ATOMIC scatters
+ * every particle's 27 node contributions with global float atomics.  Same
+ * arithmetic per particle; float summation order differs. */
 #define HFSI_MPM_P2G_BINNED 0
 #define HFSI_MPM_P2G_ATOMIC 1
 int hfsi_mpm_set_p2g_mode(hfsi_mpm* h, int32_t mode);
