@@ -23,6 +23,8 @@ HFSI_PREC_REF64 = 1
 HFSI_PREC_FP32 = 2
 HFSI_PREC_MIXED = 4
 
+ADVECTION_SCHEMES = {"euler": 0, "rk2": 1}
+
 PRECONDITIONERS = {"auto": 0, "jacobi": 1, "fv_multigrid": 2, "fv_multigrid_light": 3}
 
 BREAKDOWN_MESSAGES = {
@@ -88,6 +90,13 @@ class MpmGrid(ctypes.Structure):
     _fields_ = [("mass", ctypes.c_void_p), ("velocity", ctypes.c_void_p), ("force", ctypes.c_void_p)]
 
 
+class AxisMetrics(ctypes.Structure):
+    """hfsi_axis_metrics: device pointers of the float32 per-axis metric arrays."""
+
+    _fields_ = [("faces", ctypes.c_void_p * 3), ("centers", ctypes.c_void_p * 3), ("widths", ctypes.c_void_p * 3),
+                ("center_distances", ctypes.c_void_p * 3)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 _D = ctypes.c_double
@@ -116,6 +125,7 @@ SIGNATURES = {
     "hfsi_fluid_subtract_gradient": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F, _I, _P]),
     "hfsi_fluid_wall_clamp": (ctypes.c_int, [_P, _P, _I, _P]),
     "hfsi_fluid_divergence_report": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P]),
+    "hfsi_fluid_predict": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(AxisMetrics), _F, _F, _I, _P]),
     "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
     "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),
     "hfsi_fluid_mixed_levels": (_I, [_P, ctypes.POINTER(_I)]),
@@ -142,7 +152,7 @@ SIGNATURES = {
 }
 
 TIMER_KERNELS = {"l0_sweep": 1, "spmv_dot": 2, "l0_restrict": 3, "l0_prolong_sweep": 4, "mix_rb": 5,
-                 "cg_update": 6}
+                 "cg_update": 6, "predict": 7}
 
 _lock = threading.Lock()
 _lib = None

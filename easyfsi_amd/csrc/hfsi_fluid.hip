@@ -25,6 +25,7 @@
 #include "../../include/hfsi.h"
 #include "fluid_march.hpp"
 #include "fluid_mixed.hpp"
+#include "fluid_predict.hpp"
 
 namespace hfsi {
 
@@ -3200,8 +3201,76 @@ int hfsi_fluid_wall_clamp(hfsi_fluid* h, float* velocity, int32_t outlet_zmin, v
   return HFSI_OK;
 }
 
+int hfsi_fluid_predict(hfsi_fluid* h, const float* velocity_prev, const int32_t* obstacle, float* velocity,
+                       const hfsi_axis_metrics* metrics, float dt_s, float nu_m2_s, int32_t scheme, void* stream) {
+  if (!h || !velocity_prev || !obstacle || !velocity || !metrics) {
+    set_error("hfsi_fluid_predict: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  const LevelMem& m = h->lv[0];
+  const size_t bytes = (size_t)m.n * 3 * sizeof(float);
+  const char* vp0 = (const char*)velocity_prev;
+  const char* v0 = (const char*)velocity;
+  if (vp0 < v0 + bytes && v0 < vp0 + bytes) {
+    set_error("hfsi_fluid_predict: velocity and velocity_prev overlap");
+    return HFSI_ERR_ARG;
+  }
+  if (!(dt_s > 0.0f) || !std::isfinite(dt_s) || !std::isfinite(nu_m2_s)) {
+    set_error("hfsi_fluid_predict: dt_s must be positive and nu_m2_s finite");
+    return HFSI_ERR_ARG;
+  }
+  if (scheme != HFSI_ADVECT_EULER && scheme != HFSI_ADVECT_RK2) {
+    set_error("hfsi_fluid_predict: unsupported advection scheme");
+    return HFSI_ERR_ARG;
+  }
+  if (m.nx < 2 || m.ny < 2 || m.nz < 2) {
+    set_error("hfsi_fluid_predict: the predictor needs at least 2 cells along every axis");
+    return HFSI_ERR_ARG;
+  }
+  PredictArgs a;
+  a.nx = m.nx;
+  a.ny = m.ny;
+  a.nz = m.nz;
+  a.n = m.n;
+  for (int d = 0; d < 3; ++d) {
+    if (!metrics->faces[d] || !metrics->centers[d] || !metrics->widths[d] || !metrics->center_distances[d]) {
+      set_error("hfsi_fluid_predict: missing metric array");
+      return HFSI_ERR_ARG;
+    }
+    a.face[d] = metrics->faces[d];
+    a.centre[d] = metrics->centers[d];
+    a.width[d] = metrics->widths[d];
+    a.cdist[d] = metrics->center_distances[d];
+  }
+  a.vp = velocity_prev;
+  a.obst = obstacle;
+  a.v = velocity;
+  a.dt = dt_s;
+  a.nu = nu_m2_s;
+  const uint32_t ncen = m.nx + m.ny + m.nz;
+  const bool lds = ncen <= kPredictLdsMaxCentres;
+  const size_t shmem = lds ? ncen * sizeof(float) : 0;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid(blocks_for(m.n)), block(kBlock);
+  timer_mark(h, HFSI_TIMER_PREDICT, st);
+  if (scheme == HFSI_ADVECT_RK2) {
+    if (lds)
+      hipLaunchKernelGGL((k_predict<true, true>), grid, block, shmem, st, a);
+    else
+      hipLaunchKernelGGL((k_predict<true, false>), grid, block, shmem, st, a);
+  } else {
+    if (lds)
+      hipLaunchKernelGGL((k_predict<false, true>), grid, block, shmem, st, a);
+    else
+      hipLaunchKernelGGL((k_predict<false, false>), grid, block, shmem, st, a);
+  }
+  HFSI_LAUNCH_CHECK();
+  timer_mark(h, HFSI_TIMER_PREDICT, st);
+  return HFSI_OK;
+}
+
 int hfsi_fluid_kernel_timer(hfsi_fluid* h, int32_t kernel_id) {
-  if (!h || kernel_id < 0 || kernel_id > HFSI_TIMER_CG_UPDATE) {
+  if (!h || kernel_id < 0 || kernel_id > HFSI_TIMER_PREDICT) {
     set_error("hfsi_fluid_kernel_timer: invalid arguments");
     return HFSI_ERR_ARG;
   }

@@ -100,6 +100,7 @@ class CartesianFluidSolver:
 
         # state fields (fluid.py:756-1024, plain-path subset)
         self.velocity = vector_field(3, shape, f32)
+        self.velocity_prev = vector_field(3, shape, f32)
         self.saved_velocity = vector_field(3, shape, f32)
         self.pressure = scalar_field(shape, f64)
         self.saved_pressure = scalar_field(shape, f64)
@@ -136,6 +137,15 @@ class CartesianFluidSolver:
         self.center_distance_x_m = self._axis_field(g.center_distances_x_m)
         self.center_distance_y_m = self._axis_field(g.center_distances_y_m)
         self.center_distance_z_m = self._axis_field(g.center_distances_z_m)
+        self._axis_metrics = _lib.AxisMetrics()
+        for a, (fc, cc, wc, dc) in enumerate((
+                (self.cell_face_x_m, self.cell_center_x_m, self.cell_width_x_m, self.center_distance_x_m),
+                (self.cell_face_y_m, self.cell_center_y_m, self.cell_width_y_m, self.center_distance_y_m),
+                (self.cell_face_z_m, self.cell_center_z_m, self.cell_width_z_m, self.center_distance_z_m))):
+            self._axis_metrics.faces[a] = fc.ptr
+            self._axis_metrics.centers[a] = cc.ptr
+            self._axis_metrics.widths[a] = wc.ptr
+            self._axis_metrics.center_distances[a] = dc.ptr
 
         self._mg_shapes = multigrid_shapes(shape)
         self._mg_axis_widths_m = multigrid_axis_widths(self.grid, self._mg_shapes)
@@ -233,7 +243,7 @@ class CartesianFluidSolver:
     # ------------------------------------------------------------------
     def clear(self) -> None:
         """Zero every field (fluid.py:1201-1325)."""
-        for f in (self.velocity, self.saved_velocity, self.pressure, self.saved_pressure, self.pressure_tmp,
+        for f in (self.velocity, self.velocity_prev, self.saved_velocity, self.pressure, self.saved_pressure, self.pressure_tmp,
                   self.pressure_accum, self.divergence, self.pressure_interface_projection_divergence_s,
                   self.volume_source_s, self.pressure_interface_matrix_diagonal, self.pressure_interface_matrix_rhs,
                   self.obstacle, self.velocity_dirichlet_boundary_active, self.velocity_dirichlet_boundary_value_mps,
@@ -256,6 +266,7 @@ class CartesianFluidSolver:
     def restore_state(self) -> None:
         """fluid.py:2283-2296."""
         self.velocity.copy_from(self.saved_velocity)
+        self.velocity_prev.copy_from(self.saved_velocity)
         self.pressure.copy_from(self.saved_pressure)
 
     def set_uniform_velocity(self, velocity_mps) -> None:
@@ -814,4 +825,26 @@ class CartesianFluidSolver:
         }
 
     def predict(self, dt_s: float | None = None, *, advection_scheme: str = "euler") -> None:
-        raise NotImplementedError("the semi-Lagrangian predictor (fluid.py:2447-2527) is not in this build yet")
+        """Semi-Lagrangian advection plus explicit FV diffusion (fluid.py:2515-2527).
+
+        The reference copies ``velocity`` into ``velocity_prev`` and then
+        recomputes ``velocity`` from it (``_predict_kernel``, fluid.py:2447-2513).
+        Here the two fields exchange their device storage instead of copying:
+        the end state is the same (``velocity_prev`` holds the velocity before
+        the step, ``velocity`` the predicted one) without the copy's traffic.
+        Hold the ``Field`` objects, not their ``.tensor``, across a predict.
+        """
+        step_dt_s = self.dt if dt_s is None else float(dt_s)
+        if step_dt_s <= 0.0:
+            raise ValueError("dt_s must be positive")
+        scheme = str(advection_scheme).lower()
+        if scheme not in _lib.ADVECTION_SCHEMES:
+            raise ValueError(f"unsupported advection_scheme: {advection_scheme!r}")
+        if min(self.nx, self.ny, self.nz) < 2:
+            raise ValueError("the predictor needs at least 2 cells along every axis")
+        self.velocity.tensor, self.velocity_prev.tensor = self.velocity_prev.tensor, self.velocity.tensor
+        _lib.check(self._lib.hfsi_fluid_predict(self._handle, self.velocity_prev.ptr, self.obstacle.ptr,
+                                                self.velocity.ptr, ctypes.byref(self._axis_metrics),
+                                                float(step_dt_s), float(self.mu / self.rho),
+                                                _lib.ADVECTION_SCHEMES[scheme], self._stream),
+                   "hfsi_fluid_predict")

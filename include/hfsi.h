@@ -315,6 +315,36 @@ int hfsi_unet_loss(hfsi_unet* h, const float* out, const float* target, const ui
 int hfsi_unet_adam_step(hfsi_unet* h, float lr, float beta1, float beta2, float eps, float weight_decay,
                         void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Velocity predictor
+ * ------------------------------------------------------------------------- */
+
+/* Float32 metric arrays of the grid, as the solver's 1D fields hold them
+ * (simulation_core/fluid.py:1013-1024): per axis a, faces[a] has n_a + 1
+ * entries, centers[a] and widths[a] n_a, center_distances[a] n_a (entry d > 0
+ * is centre d minus centre d - 1).  Device pointers. */
+typedef struct hfsi_axis_metrics {
+  const float* faces[3];
+  const float* centers[3];
+  const float* widths[3];
+  const float* center_distances[3];
+} hfsi_axis_metrics;
+
+#define HFSI_ADVECT_EULER 0
+#define HFSI_ADVECT_RK2 1
+
+/* velocity = predictor(velocity_prev): semi-Lagrangian advection (Euler or RK2
+ * backtrace, obstacle-aware trilinear sampling) plus explicit FV diffusion
+ * dt * nu * lap(u), obstacle == 1 cells zeroed.  Replaces
+ * CartesianFluidSolver._predict_kernel (simulation_core/fluid.py:2447-2513)
+ * as launched by predict() (fluid.py:2515-2527), whose velocity -> velocity_prev
+ * copy is the caller's (the Python host swaps the two buffers instead).
+ * velocity_prev and velocity are [n][3] float32 and must not overlap;
+ * obstacle is [n] int32.  Float32 arithmetic in the reference's operation
+ * order without contraction.  Needs at least 2 cells per axis, dt_s > 0. */
+int hfsi_fluid_predict(hfsi_fluid* h, const float* velocity_prev, const int32_t* obstacle, float* velocity,
+                       const hfsi_axis_metrics* metrics, float dt_s, float nu_m2_s, int32_t scheme, void* stream);
+
 /* Live kernel timing (no reference counterpart; measurement support):
  * while enabled, HIP events are recorded on the launch stream around every
  * launch of the selected level-0 kernel and their elapsed times summed.
@@ -326,6 +356,7 @@ int hfsi_unet_adam_step(hfsi_unet* h, float lr, float beta1, float beta2, float 
 #define HFSI_TIMER_L0_PROLONG_SWEEP 4 /* level-0 prolongation fused into the first post-smoothing sweep          */
 #define HFSI_TIMER_MIX_RB 5           /* mixed V-cycle: level-0 red-black colour pass (not the first from zero)  */
 #define HFSI_TIMER_CG_UPDATE 6        /* mixed solve: x += alpha d, r -= alpha q with <r,r>_V                    */
+#define HFSI_TIMER_PREDICT 7          /* velocity predictor (hfsi_fluid_predict)                                 */
 int hfsi_fluid_kernel_timer(hfsi_fluid* h, int32_t kernel_id);
 /* Synchronises on the recorded events; totals since the last enable. */
 int hfsi_fluid_kernel_timer_read(hfsi_fluid* h, int64_t* launches, double* total_ms);
