@@ -19,16 +19,23 @@
 // evaluation of the reference's expressions, operation by operation.
 //
 // Layout: velocity [n][3] float32 (component fastest), obstacle [n] int32, cell
-// c = (i * ny + j) * nz + k.  One thread per cell under the XCD-aware chunk loop; the
-// cell centres of the three axes are staged in LDS for the binary searches when they
-// fit (kLds), otherwise read from global memory.
+// c = (i * ny + j) * nz + k.  One thread per cell under the XCD-aware chunk loop.
+// Every divisor but the sample weight is a metric value, so each workgroup stages the
+// centres and the correctly rounded reciprocals of the metric divisors in LDS and forms
+// each quotient from its reciprocal with two residual corrections (div_ry), which
+// yields the correctly rounded quotient in five operations instead of the dozen of a
+// general division.
 #pragma once
 
 #include "hfsi_common.hpp"
 
 namespace hfsi {
 
-constexpr uint32_t kPredictLdsMaxCentres = 12288;  // 48 KiB of LDS
+// LDS tables per workgroup (kLds): for each axis the centres, widths, centre distances
+// and the correctly rounded reciprocals of the widths, centre distances and centre gaps,
+// 6 floats per cell.
+constexpr uint32_t kPredictLdsTables = 6;
+constexpr uint32_t kPredictLdsMaxCells = 2048;  // nx + ny + nz; 48 KiB of LDS
 
 struct PredictArgs {
   uint32_t nx, ny, nz, n;
@@ -46,42 +53,104 @@ struct Vec3 {
   float x, y, z;
 };
 
-__device__ inline Vec3 ld3(const float* p, uint32_t c) {
-  const float* q = p + 3 * (size_t)c;
+// Loads and stores at a byte offset from a uniform base.  With Off = uint32_t the
+// offset stays a 32-bit register operand of the memory instruction.
+template <typename Off>
+__device__ inline Vec3 ld3(const float* p, Off byte_off) {
+  const float* q = (const float*)((const char*)p + byte_off);
   return Vec3{q[0], q[1], q[2]};
 }
 
-__device__ inline void st3(float* p, uint32_t c, Vec3 v) {
-  float* q = p + 3 * (size_t)c;
+template <typename Off>
+__device__ inline void st3(float* p, Off byte_off, Vec3 v) {
+  float* q = (float*)((char*)p + byte_off);
   q[0] = v.x;
   q[1] = v.y;
   q[2] = v.z;
 }
 
-// Fractional coordinate of position `pos` on an axis with `n` centres `cen` (fluid.py:2406-2435).
-__device__ inline float predict_coord(float pos, const float* cen, const float* face, uint32_t n) {
+template <typename Off>
+__device__ inline int32_t ld_obst(const int32_t* p, Off byte_off) {
+  return *(const int32_t*)((const char*)p + byte_off);
+}
+
+// RN(n / d) for d > 0 from y = RN(1 / d).  q = RN(n y) is within 2 ulps of n / d; one
+// residual correction brings it within 1 ulp, and a second one, with the residual
+// n - d q exact under the fused multiply-add, rounds to RN(n / d) (P. Markstein, "Computation
+// of elementary functions on the IBM RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990,
+// theorem for y within half an ulp of 1/d and q within one ulp of n/d).  Valid for finite
+// normal operands and quotients away from the overflow and underflow thresholds; a zero
+// numerator yields +0.
+__device__ inline float div_ry(float n, float d, float y) {
+  float q = n * y;
+  float r = __builtin_fmaf(-d, q, n);
+  q = __builtin_fmaf(r, y, q);
+  r = __builtin_fmaf(-d, q, n);
+  return __builtin_fmaf(r, y, q);
+}
+
+// Per-axis data of the coordinate lookup and the diffusion term.
+struct PredictAxis {
+  const float* cen;   // centres
+  const float* rw;    // RN(1 / width)            (LDS tables) or nullptr
+  const float* rcd;   // RN(1 / centre distance)  (LDS tables) or nullptr
+  const float* rgap;  // RN(1 / max(c[l+1] - c[l], 1e-18))
+  const float* w;     // widths
+  const float* cd;    // centre distances
+  float c0, cl, hwlo, hwhi, rhwlo, rhwhi;
+  uint32_t n;
+};
+
+// Fractional coordinate of `pos` (fluid.py:2406-2435).  The bracketing pair of centres is
+// looked for next to the cell's own index `own` first and by the reference's binary search
+// over the remaining range otherwise; for non-decreasing centres the bracket
+// c[lo] <= pos < c[lo + 1] is unique, so both find the same lo.
+template <bool kTables>
+__device__ inline float predict_coord(float pos, uint32_t own, const PredictAxis& A) {
 #pragma clang fp contract(off)
-  const float c0 = cen[0];
-  const float cl = cen[n - 1];
-  if (pos <= c0) {
-    const float hw = fmaxf(c0 - face[0], 1.0e-18f);
-    return (-0.5f * (c0 - pos)) / hw;
+  if (pos <= A.c0) {
+    const float num = -0.5f * (A.c0 - pos);
+    return kTables ? div_ry(num, A.hwlo, A.rhwlo) : num / A.hwlo;
   }
-  if (pos >= cl) {
-    const float hw = fmaxf(face[n] - cl, 1.0e-18f);
-    return (float)(n - 1) + (0.5f * (pos - cl)) / hw;
+  if (pos >= A.cl) {
+    const float num = 0.5f * (pos - A.cl);
+    return (float)(A.n - 1) + (kTables ? div_ry(num, A.hwhi, A.rhwhi) : num / A.hwhi);
   }
-  uint32_t lo = 0, hi = n - 1;
-  while (hi - lo > 1) {
+  const float* cen = A.cen;
+  uint32_t lo, hi;
+  const uint32_t l = min(own, A.n - 2);
+  if (pos >= cen[l]) {
+    // l + 1 == n - 1 implies pos < c[n - 1] = c[l + 1]; otherwise l + 2 <= n - 1
+    if (pos < cen[l + 1]) {
+      lo = l;
+      hi = l;
+    } else if (pos < cen[l + 2]) {
+      lo = l + 1;
+      hi = l + 1;
+    } else {
+      lo = l + 2;  // c[l + 2] <= pos < c[n - 1]
+      hi = A.n - 1;
+    }
+  } else {
+    // pos > c[0] here, so l >= 1
+    if (pos >= cen[l - 1]) {
+      lo = l - 1;
+      hi = l - 1;
+    } else {
+      lo = 0;
+      hi = l - 1;
+    }
+  }
+  while (hi > lo + 1) {
     const uint32_t m = (lo + hi) >> 1;
     if (pos >= cen[m])
       lo = m;
     else
       hi = m;
   }
-  hi = min(lo + 1, n - 1);
-  const float dist = fmaxf(cen[hi] - cen[lo], 1.0e-18f);
-  return (float)lo + (pos - cen[lo]) / dist;
+  const float gap = fmaxf(cen[lo + 1] - cen[lo], 1.0e-18f);
+  const float num = pos - cen[lo];
+  return (float)lo + (kTables ? div_ry(num, gap, A.rgap[lo]) : num / gap);
 }
 
 // Lower corner index clamp(floor(g), 0, n - 2) and its fraction clamp(g - i0, 0, 1)
@@ -96,21 +165,22 @@ __device__ inline uint32_t predict_corner(float g, uint32_t n, float& t) {
 }
 
 // Obstacle-aware trilinear sample of a.vp at grid coordinates (gx, gy, gz) (fluid.py:2382-2404).
+template <bool kTables, typename Off>
 __device__ inline Vec3 predict_sample(const PredictArgs& a, float gx, float gy, float gz, Vec3 fallback) {
 #pragma clang fp contract(off)
   float tx, ty, tz;
   const uint32_t i0 = predict_corner(gx, a.nx, tx);
   const uint32_t j0 = predict_corner(gy, a.ny, ty);
   const uint32_t k0 = predict_corner(gz, a.nz, tz);
-  const uint32_t sx = a.ny * a.nz, sy = a.nz;
-  const uint32_t base = (i0 * a.ny + j0) * a.nz + k0;
+  const Off sx = (Off)a.ny * a.nz, sy = a.nz;
+  const Off base = ((Off)i0 * a.ny + j0) * a.nz + k0;
   int32_t ob[8];
   Vec3 u[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const uint32_t c = base + ((q >> 2) & 1) * sx + ((q >> 1) & 1) * sy + (q & 1);
-    ob[q] = a.obst[c];
-    u[q] = ld3(a.vp, c);
+    const Off c = base + ((q >> 2) & 1) * sx + ((q >> 1) & 1) * sy + (q & 1);
+    ob[q] = ld_obst(a.obst, c * 4);
+    u[q] = ld3(a.vp, c * 12);
   }
   Vec3 val{0.0f, 0.0f, 0.0f};
   float fw = 0.0f;
@@ -127,79 +197,151 @@ __device__ inline Vec3 predict_sample(const PredictArgs& a, float gx, float gy, 
       fw = fw + w;
     }
   }
-  if (fw > 1.0e-12f) return Vec3{val.x / fw, val.y / fw, val.z / fw};
+  if (fw > 1.0e-12f) {
+    if (kTables) {
+      const float y = 1.0f / fw;
+      return Vec3{div_ry(val.x, fw, y), div_ry(val.y, fw, y), div_ry(val.z, fw, y)};
+    }
+    return Vec3{val.x / fw, val.y / fw, val.z / fw};
+  }
   return fallback;
 }
 
-// Diffusive flux across one face: (hi - lo) / d when `open`, else 0.
-__device__ inline Vec3 predict_flux(bool open, Vec3 lo, Vec3 hi, float d) {
+// (hi - lo) / d when `open`, else 0.
+template <bool kTables>
+__device__ inline Vec3 predict_flux(bool open, Vec3 lo, Vec3 hi, float d, float rd) {
 #pragma clang fp contract(off)
   if (!open) return Vec3{0.0f, 0.0f, 0.0f};
+  if (kTables) return Vec3{div_ry(hi.x - lo.x, d, rd), div_ry(hi.y - lo.y, d, rd), div_ry(hi.z - lo.z, d, rd)};
   return Vec3{(hi.x - lo.x) / d, (hi.y - lo.y) / d, (hi.z - lo.z) / d};
 }
 
-template <bool kRk2, bool kLds>
+template <bool kTables>
+__device__ inline float predict_lap(float f, float b, float w, float rw) {
+#pragma clang fp contract(off)
+  return kTables ? div_ry(f - b, w, rw) : (f - b) / w;
+}
+
+// kLds: centres and reciprocal tables in LDS (nx + ny + nz <= kPredictLdsMaxCells), all
+// quotients through div_ry; otherwise centres from global memory and IEEE division.
+// Off: byte-offset type (uint32_t when 12 n < 2^32).
+template <bool kRk2, bool kLds, typename Off>
 __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
 #pragma clang fp contract(off)
-  extern __shared__ float s_cen[];
-  const float* cen[3] = {a.centre[0], a.centre[1], a.centre[2]};
-  if (kLds) {
-    const uint32_t tot = a.nx + a.ny + a.nz;
-    for (uint32_t t = threadIdx.x; t < tot; t += blockDim.x)
-      s_cen[t] = t < a.nx ? a.centre[0][t] : (t < a.nx + a.ny ? a.centre[1][t - a.nx] : a.centre[2][t - a.nx - a.ny]);
-    __syncthreads();
-    cen[0] = s_cen;
-    cen[1] = s_cen + a.nx;
-    cen[2] = s_cen + a.nx + a.ny;
+  extern __shared__ float s_tab[];
+  const uint32_t tot = a.nx + a.ny + a.nz;
+  const uint32_t nn[3] = {a.nx, a.ny, a.nz};
+  PredictAxis ax[3];
+  uint32_t off = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    PredictAxis& A = ax[d];
+    const uint32_t n = nn[d];
+    A.n = n;
+    A.w = a.width[d];
+    A.cd = a.cdist[d];
+    A.c0 = a.centre[d][0];
+    A.cl = a.centre[d][n - 1];
+    A.hwlo = fmaxf(A.c0 - a.face[d][0], 1.0e-18f);
+    A.hwhi = fmaxf(a.face[d][n] - A.cl, 1.0e-18f);
+    A.rhwlo = 1.0f / A.hwlo;
+    A.rhwhi = 1.0f / A.hwhi;
+    if (kLds) {
+      A.cen = s_tab + off;
+      A.rw = s_tab + tot + off;
+      A.rcd = s_tab + 2 * tot + off;
+      A.rgap = s_tab + 3 * tot + off;
+      A.w = s_tab + 4 * tot + off;
+      A.cd = s_tab + 5 * tot + off;
+      for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+        const float c = a.centre[d][t];
+        const float w = a.width[d][t];
+        const float cd = a.cdist[d][t];
+        s_tab[off + t] = c;
+        s_tab[tot + off + t] = 1.0f / w;
+        s_tab[2 * tot + off + t] = 1.0f / cd;
+        s_tab[3 * tot + off + t] = t + 1 < n ? 1.0f / fmaxf(a.centre[d][t + 1] - c, 1.0e-18f) : 0.0f;
+        s_tab[4 * tot + off + t] = w;
+        s_tab[5 * tot + off + t] = cd;
+      }
+    } else {
+      A.cen = a.centre[d];
+      A.rw = A.rcd = A.rgap = nullptr;
+    }
+    off += n;
   }
+  if (kLds) __syncthreads();
+
   const float dt = a.dt;
   const float dtnu = a.dt * a.nu;
-  const uint32_t sx = a.ny * a.nz, sy = a.nz;
+  const Off sx = (Off)a.ny * a.nz, sy = a.nz;
   HFSI_CELL_LOOP(a.n) {
     const uint32_t k = c % a.nz;
     const uint32_t t = c / a.nz;
     const uint32_t j = t % a.ny;
     const uint32_t i = t / a.ny;
     // every load below is at a clamped, in-range index; masks select what counts
-    const int32_t oc = a.obst[c];
-    const Vec3 ctr = ld3(a.vp, c);
-    const uint32_t cxm = i > 0 ? c - sx : c, cxp = i + 1 < a.nx ? c + sx : c;
-    const uint32_t cym = j > 0 ? c - sy : c, cyp = j + 1 < a.ny ? c + sy : c;
-    const uint32_t czm = k > 0 ? c - 1 : c, czp = k + 1 < a.nz ? c + 1 : c;
-    const int32_t oxm = a.obst[cxm], oxp = a.obst[cxp], oym = a.obst[cym], oyp = a.obst[cyp];
-    const int32_t ozm = a.obst[czm], ozp = a.obst[czp];
-    const Vec3 uxm = ld3(a.vp, cxm), uxp = ld3(a.vp, cxp), uym = ld3(a.vp, cym), uyp = ld3(a.vp, cyp);
-    const Vec3 uzm = ld3(a.vp, czm), uzp = ld3(a.vp, czp);
+    const Off cc = c;
+    const int32_t oc = ld_obst(a.obst, cc * 4);
+    const Vec3 ctr = ld3(a.vp, cc * 12);
+    const Off cxm = i > 0 ? cc - sx : cc, cxp = i + 1 < a.nx ? cc + sx : cc;
+    const Off cym = j > 0 ? cc - sy : cc, cyp = j + 1 < a.ny ? cc + sy : cc;
+    const Off czm = k > 0 ? cc - 1 : cc, czp = k + 1 < a.nz ? cc + 1 : cc;
+    const int32_t oxm = ld_obst(a.obst, cxm * 4), oxp = ld_obst(a.obst, cxp * 4);
+    const int32_t oym = ld_obst(a.obst, cym * 4), oyp = ld_obst(a.obst, cyp * 4);
+    const int32_t ozm = ld_obst(a.obst, czm * 4), ozp = ld_obst(a.obst, czp * 4);
+    const Vec3 uxm = ld3(a.vp, cxm * 12), uxp = ld3(a.vp, cxp * 12);
+    const Vec3 uym = ld3(a.vp, cym * 12), uyp = ld3(a.vp, cyp * 12);
+    const Vec3 uzm = ld3(a.vp, czm * 12), uzp = ld3(a.vp, czp * 12);
     if (oc == 1) {
-      st3(a.v, c, Vec3{0.0f, 0.0f, 0.0f});
+      st3(a.v, cc * 12, Vec3{0.0f, 0.0f, 0.0f});
       continue;
     }
-    const float x = cen[0][i], y = cen[1][j], z = cen[2][k];
+    const float x = ax[0].cen[i], y = ax[1].cen[j], z = ax[2].cen[k];
     Vec3 tr = ctr;
     if (kRk2) {
       const float hd = 0.5f * dt;
-      const float mgx = predict_coord(x - hd * ctr.x, cen[0], a.face[0], a.nx);
-      const float mgy = predict_coord(y - hd * ctr.y, cen[1], a.face[1], a.ny);
-      const float mgz = predict_coord(z - hd * ctr.z, cen[2], a.face[2], a.nz);
-      tr = predict_sample(a, mgx, mgy, mgz, ctr);
+      const float mgx = predict_coord<kLds>(x - hd * ctr.x, i, ax[0]);
+      const float mgy = predict_coord<kLds>(y - hd * ctr.y, j, ax[1]);
+      const float mgz = predict_coord<kLds>(z - hd * ctr.z, k, ax[2]);
+      tr = predict_sample<kLds, Off>(a, mgx, mgy, mgz, ctr);
     }
-    const float gx = predict_coord(x - dt * tr.x, cen[0], a.face[0], a.nx);
-    const float gy = predict_coord(y - dt * tr.y, cen[1], a.face[1], a.ny);
-    const float gz = predict_coord(z - dt * tr.z, cen[2], a.face[2], a.nz);
-    const Vec3 adv = predict_sample(a, gx, gy, gz, ctr);
+    const float gx = predict_coord<kLds>(x - dt * tr.x, i, ax[0]);
+    const float gy = predict_coord<kLds>(y - dt * tr.y, j, ax[1]);
+    const float gz = predict_coord<kLds>(z - dt * tr.z, k, ax[2]);
+    const Vec3 adv = predict_sample<kLds, Off>(a, gx, gy, gz, ctr);
 
-    const Vec3 fxb = predict_flux(i > 0 && oxm == 0, uxm, ctr, a.cdist[0][i]);
-    const Vec3 fxf = predict_flux(i + 1 < a.nx && oxp == 0, ctr, uxp, a.cdist[0][i + 1 < a.nx ? i + 1 : i]);
-    const Vec3 fyb = predict_flux(j > 0 && oym == 0, uym, ctr, a.cdist[1][j]);
-    const Vec3 fyf = predict_flux(j + 1 < a.ny && oyp == 0, ctr, uyp, a.cdist[1][j + 1 < a.ny ? j + 1 : j]);
-    const Vec3 fzb = predict_flux(k > 0 && ozm == 0, uzm, ctr, a.cdist[2][k]);
-    const Vec3 fzf = predict_flux(k + 1 < a.nz && ozp == 0, ctr, uzp, a.cdist[2][k + 1 < a.nz ? k + 1 : k]);
-    const float wx = a.width[0][i], wy = a.width[1][j], wz = a.width[2][k];
+    const uint32_t ip = i + 1 < a.nx ? i + 1 : i, jp = j + 1 < a.ny ? j + 1 : j, kp = k + 1 < a.nz ? k + 1 : k;
+    const float cdxm = ax[0].cd[i], cdxp = ax[0].cd[ip], cdym = ax[1].cd[j], cdyp = ax[1].cd[jp];
+    const float cdzm = ax[2].cd[k], cdzp = ax[2].cd[kp];
+    const float wx = ax[0].w[i], wy = ax[1].w[j], wz = ax[2].w[k];
+    float rcdxm = 0.f, rcdxp = 0.f, rcdym = 0.f, rcdyp = 0.f, rcdzm = 0.f, rcdzp = 0.f, rwx = 0.f, rwy = 0.f,
+          rwz = 0.f;
+    if (kLds) {
+      rcdxm = ax[0].rcd[i];
+      rcdxp = ax[0].rcd[ip];
+      rcdym = ax[1].rcd[j];
+      rcdyp = ax[1].rcd[jp];
+      rcdzm = ax[2].rcd[k];
+      rcdzp = ax[2].rcd[kp];
+      rwx = ax[0].rw[i];
+      rwy = ax[1].rw[j];
+      rwz = ax[2].rw[k];
+    }
+    const Vec3 fxb = predict_flux<kLds>(i > 0 && oxm == 0, uxm, ctr, cdxm, rcdxm);
+    const Vec3 fxf = predict_flux<kLds>(i + 1 < a.nx && oxp == 0, ctr, uxp, cdxp, rcdxp);
+    const Vec3 fyb = predict_flux<kLds>(j > 0 && oym == 0, uym, ctr, cdym, rcdym);
+    const Vec3 fyf = predict_flux<kLds>(j + 1 < a.ny && oyp == 0, ctr, uyp, cdyp, rcdyp);
+    const Vec3 fzb = predict_flux<kLds>(k > 0 && ozm == 0, uzm, ctr, cdzm, rcdzm);
+    const Vec3 fzf = predict_flux<kLds>(k + 1 < a.nz && ozp == 0, ctr, uzp, cdzp, rcdzp);
     Vec3 lap;
-    lap.x = ((fxf.x - fxb.x) / wx + (fyf.x - fyb.x) / wy) + (fzf.x - fzb.x) / wz;
-    lap.y = ((fxf.y - fxb.y) / wx + (fyf.y - fyb.y) / wy) + (fzf.y - fzb.y) / wz;
-    lap.z = ((fxf.z - fxb.z) / wx + (fyf.z - fyb.z) / wy) + (fzf.z - fzb.z) / wz;
-    st3(a.v, c, Vec3{adv.x + dtnu * lap.x, adv.y + dtnu * lap.y, adv.z + dtnu * lap.z});
+    lap.x = (predict_lap<kLds>(fxf.x, fxb.x, wx, rwx) + predict_lap<kLds>(fyf.x, fyb.x, wy, rwy)) +
+            predict_lap<kLds>(fzf.x, fzb.x, wz, rwz);
+    lap.y = (predict_lap<kLds>(fxf.y, fxb.y, wx, rwx) + predict_lap<kLds>(fyf.y, fyb.y, wy, rwy)) +
+            predict_lap<kLds>(fzf.y, fzb.y, wz, rwz);
+    lap.z = (predict_lap<kLds>(fxf.z, fxb.z, wx, rwx) + predict_lap<kLds>(fyf.z, fyb.z, wy, rwy)) +
+            predict_lap<kLds>(fzf.z, fzb.z, wz, rwz);
+    st3(a.v, cc * 12, Vec3{adv.x + dtnu * lap.x, adv.y + dtnu * lap.y, adv.z + dtnu * lap.z});
   }
 }
 

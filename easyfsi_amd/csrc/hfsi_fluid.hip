@@ -3247,22 +3247,34 @@ int hfsi_fluid_predict(hfsi_fluid* h, const float* velocity_prev, const int32_t*
   a.v = velocity;
   a.dt = dt_s;
   a.nu = nu_m2_s;
-  const uint32_t ncen = m.nx + m.ny + m.nz;
-  const bool lds = ncen <= kPredictLdsMaxCentres;
-  const size_t shmem = lds ? ncen * sizeof(float) : 0;
+  const uint32_t ncells = m.nx + m.ny + m.nz;
+  const bool lds = ncells <= kPredictLdsMaxCells;
+  const size_t shmem = lds ? kPredictLdsTables * (size_t)ncells * sizeof(float) : 0;
+  const bool off32 = (uint64_t)m.n * 12 < (1ull << 32);
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(blocks_for(m.n)), block(kBlock);
   timer_mark(h, HFSI_TIMER_PREDICT, st);
-  if (scheme == HFSI_ADVECT_RK2) {
-    if (lds)
-      hipLaunchKernelGGL((k_predict<true, true>), grid, block, shmem, st, a);
+  const bool rk2 = scheme == HFSI_ADVECT_RK2;
+  if (lds && off32) {
+    if (rk2)
+      hipLaunchKernelGGL((k_predict<true, true, uint32_t>), grid, block, shmem, st, a);
     else
-      hipLaunchKernelGGL((k_predict<true, false>), grid, block, shmem, st, a);
+      hipLaunchKernelGGL((k_predict<false, true, uint32_t>), grid, block, shmem, st, a);
+  } else if (lds) {
+    if (rk2)
+      hipLaunchKernelGGL((k_predict<true, true, uint64_t>), grid, block, shmem, st, a);
+    else
+      hipLaunchKernelGGL((k_predict<false, true, uint64_t>), grid, block, shmem, st, a);
+  } else if (off32) {
+    if (rk2)
+      hipLaunchKernelGGL((k_predict<true, false, uint32_t>), grid, block, shmem, st, a);
+    else
+      hipLaunchKernelGGL((k_predict<false, false, uint32_t>), grid, block, shmem, st, a);
   } else {
-    if (lds)
-      hipLaunchKernelGGL((k_predict<false, true>), grid, block, shmem, st, a);
+    if (rk2)
+      hipLaunchKernelGGL((k_predict<true, false, uint64_t>), grid, block, shmem, st, a);
     else
-      hipLaunchKernelGGL((k_predict<false, false>), grid, block, shmem, st, a);
+      hipLaunchKernelGGL((k_predict<false, false, uint64_t>), grid, block, shmem, st, a);
   }
   HFSI_LAUNCH_CHECK();
   timer_mark(h, HFSI_TIMER_PREDICT, st);
