@@ -28,7 +28,8 @@ The headline value is the faster of learned_mixed and mixed; the speed-up
 over plain_fp32 is reported beside it.  Also reported: a time-coherent
 sequence (moving sphere, turning velocity) solved from p = 0, from the
 previous step's pressure and from the learned guess; the neo-Hookean MPM step
-(1M particles); the CPU baseline (C + OpenMP restatement of the plain
+(1M particles); the velocity predictor (Euler and RK2 semi-Lagrangian advection
+plus FV diffusion) on the same grid; the CPU baseline (C + OpenMP restatement of the plain
 algorithm on the host cores).
 
 Prints one JSON line.  --gpus above 1 is refused: this build measures one GPU.
@@ -430,6 +431,83 @@ def run_mpm(steps: int, warmup: int, lattice: int = 100, nodes: int = 256):
     }
 
 
+# Algorithmic HBM bytes per cell of one predictor launch: read the previous velocity (12 B) and the
+# obstacle (4 B), write the new velocity (12 B); neighbour and departure-point re-reads are cache traffic.
+PREDICT_BYTES_PER_CELL = 28
+
+
+def run_predict(solver, steps: int, warmup: int, cpu: bool, cpu_budget: float = 10.0):
+    """Velocity predictor (semi-Lagrangian advection + FV diffusion) on the solver's grid and obstacle:
+    velocity = seeded Fourier field (|k| <= 16, unit RMS per component), dt at CFL 1 on the finest cell,
+    nu = mu / rho.  Each scheme: `warmup` untimed then `steps` timed predicts (the product path, buffer
+    swap included), kernel time from HIP events on the launch stream."""
+    import ctypes
+
+    import torch
+
+    from easyfsi_amd import _lib
+
+    shape = (solver.nx, solver.ny, solver.nz)
+    n = solver.nx * solver.ny * solver.nz
+    v0 = fourier_velocity(shape, 16, 4242)
+    dx_min = float(min(solver.grid.cell_widths_x_m))
+    dt = dx_min / float(v0.abs().max())
+    L = solver._lib
+    out = {"cells": n, "dt_s": dt, "cfl_finest_cell": 1.0, "nu_m2_s": solver.mu / solver.rho}
+    for scheme in ("euler", "rk2"):
+        solver.velocity.tensor.copy_(v0)
+        for _ in range(warmup):
+            solver.predict(dt, advection_scheme=scheme)
+        torch.cuda.synchronize()
+        _lib.check(L.hfsi_fluid_kernel_timer(solver._handle, _lib.TIMER_KERNELS["predict"]), "timer")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solver.predict(dt, advection_scheme=scheme)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        cnt, ms = ctypes.c_int64(), ctypes.c_double()
+        _lib.check(L.hfsi_fluid_kernel_timer_read(solver._handle, ctypes.byref(cnt), ctypes.byref(ms)), "timer")
+        _lib.check(L.hfsi_fluid_kernel_timer(solver._handle, 0), "timer")
+        avg_s = ms.value / max(cnt.value, 1) / 1e3
+        achieved = PREDICT_BYTES_PER_CELL * n / avg_s / 1e9
+        out[scheme] = {
+            "predicts_per_s": round(steps / wall, 3), "ms_per_predict": round(1e3 * wall / steps, 4),
+            "kernel_avg_us": round(avg_s * 1e6, 2), "launches": cnt.value,
+            "roofline": {"bound": "hbm", "kernel": f"k_predict ({scheme})", "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": None, "bytes_per_cell": PREDICT_BYTES_PER_CELL},
+        }
+    if cpu:
+        from oracle import predict as P
+
+        g = solver.grid
+        axes = [P.AxisMetrics.from_arrays(g.cell_faces_x_m, g.cell_centers_x_m, g.cell_widths_x_m,
+                                          g.center_distances_x_m),
+                P.AxisMetrics.from_arrays(g.cell_faces_y_m, g.cell_centers_y_m, g.cell_widths_y_m,
+                                          g.center_distances_y_m),
+                P.AxisMetrics.from_arrays(g.cell_faces_z_m, g.cell_centers_z_m, g.cell_widths_z_m,
+                                          g.center_distances_z_m)]
+        vh = v0.cpu().numpy()
+        obst = solver.obstacle.to_numpy()
+        rng = np.random.default_rng(1)
+        cpu_out = {}
+        for scheme in ("euler", "rk2"):
+            sample = 1 << 18
+            done, t_used = 0, 0.0
+            while t_used < cpu_budget / 2:
+                cells = rng.choice(n, sample, replace=False)
+                t0 = time.perf_counter()
+                P.predict(vh, obst, axes, dt, solver.mu / solver.rho, scheme, cells=cells)
+                t_used += time.perf_counter() - t0
+                done += sample
+            cpu_out[scheme] = {"value": round(done / t_used / n, 5), "unit": "predicts/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"NumPy float32 oracle (oracle/predict.py) on {done} random cells of the "
+                                         f"same 256^3 workload in {t_used:.1f} s, scaled to whole-grid predicts"}
+        out["cpu_baseline"] = cpu_out
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -449,6 +527,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mpm-steps", type=int, default=20, help="neo-Hookean MPM steps timed beside (0: skip)")
+    ap.add_argument("--predict-steps", type=int, default=20,
+                    help="velocity predictor launches timed beside, per advection scheme (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -524,6 +604,12 @@ def main(argv=None) -> int:
         mpm = {k: (float(f"{v:.6g}") if isinstance(v, float) else v)
                for k, v in run_mpm(args.mpm_steps, 2).items()}
         print(f"[bench] MPM: {mpm['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    predict = None
+    if args.predict_steps > 0:
+        print(f"[bench] velocity predictor: {args.predict_steps} predicts per scheme", file=sys.stderr, flush=True)
+        predict = run_predict(solver, args.predict_steps, 3, not args.no_cpu)
+        print(f"[bench] predictor: euler {predict['euler']['kernel_avg_us']:.1f} us, "
+              f"rk2 {predict['rk2']['kernel_avg_us']:.1f} us per launch", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (C oracle)", file=sys.stderr, flush=True)
@@ -567,6 +653,8 @@ def main(argv=None) -> int:
     }
     if mpm is not None:
         line["mpm"] = mpm
+    if predict is not None:
+        line["predictor"] = predict
     if sequence is not None:
         line["warm_start_sequence"] = sequence
     if project is not None:

@@ -38,7 +38,8 @@ constexpr uint32_t kPredictLdsTables = 6;
 constexpr uint32_t kPredictLdsMaxCells = 2048;  // nx + ny + nz; 48 KiB of LDS
 
 struct PredictArgs {
-  uint32_t nx, ny, nz, n;
+  uint32_t nx, ny, nz, n;  // n < 2^31
+  FastDiv fz, fy;          // division by nz and by ny
   const float* face[3];    // n_a + 1 face coordinates
   const float* centre[3];  // n_a centres
   const float* width[3];   // n_a widths
@@ -184,17 +185,33 @@ __device__ inline Vec3 predict_sample(const PredictArgs& a, float gx, float gy, 
   }
   Vec3 val{0.0f, 0.0f, 0.0f};
   float fw = 0.0f;
+  const int32_t any_solid = ob[0] | ob[1] | ob[2] | ob[3] | ob[4] | ob[5] | ob[6] | ob[7];
+  if (__builtin_amdgcn_ballot_w64(any_solid != 0) == 0) {
+    // every corner of every lane of the wave is fluid: the same sums without the masks
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const float wx = (q & 4) ? tx : 1.0f - tx;
-    const float wy = (q & 2) ? ty : 1.0f - ty;
-    const float wz = (q & 1) ? tz : 1.0f - tz;
-    const float w = wx * wy * wz;
-    if (ob[q] == 0) {
+    for (int q = 0; q < 8; ++q) {
+      const float wx = (q & 4) ? tx : 1.0f - tx;
+      const float wy = (q & 2) ? ty : 1.0f - ty;
+      const float wz = (q & 1) ? tz : 1.0f - tz;
+      const float w = wx * wy * wz;
       val.x = val.x + w * u[q].x;
       val.y = val.y + w * u[q].y;
       val.z = val.z + w * u[q].z;
       fw = fw + w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float wx = (q & 4) ? tx : 1.0f - tx;
+      const float wy = (q & 2) ? ty : 1.0f - ty;
+      const float wz = (q & 1) ? tz : 1.0f - tz;
+      const float w = wx * wy * wz;
+      if (ob[q] == 0) {
+        val.x = val.x + w * u[q].x;
+        val.y = val.y + w * u[q].y;
+        val.z = val.z + w * u[q].z;
+        fw = fw + w;
+      }
     }
   }
   if (fw > 1.0e-12f) {
@@ -220,6 +237,34 @@ template <bool kTables>
 __device__ inline float predict_lap(float f, float b, float w, float rw) {
 #pragma clang fp contract(off)
   return kTables ? div_ry(f - b, w, rw) : (f - b) / w;
+}
+
+// Neighbour data of the FV diffusion term of one cell.
+struct PredictNb {
+  Vec3 ctr, uxm, uxp, uym, uyp, uzm, uzp;
+  float cd[6], rcd[6];  // centre distances (and reciprocals) of the -x, +x, -y, +y, -z, +z faces
+  float w[3], rw[3];    // widths (and reciprocals) along x, y, z
+};
+
+// lap(u) = ((fxf - fxb) / wx + (fyf - fyb) / wy) + (fzf - fzb) / wz with face fluxes
+// (hi - lo) / d; kMasked zeroes the fluxes of closed faces (fluid.py:2484-2512).
+template <bool kTables, bool kMasked>
+__device__ inline Vec3 predict_laplacian(const PredictNb& b, const bool (&open)[6]) {
+#pragma clang fp contract(off)
+  const Vec3 fxb = predict_flux<kTables>(!kMasked || open[0], b.uxm, b.ctr, b.cd[0], b.rcd[0]);
+  const Vec3 fxf = predict_flux<kTables>(!kMasked || open[1], b.ctr, b.uxp, b.cd[1], b.rcd[1]);
+  const Vec3 fyb = predict_flux<kTables>(!kMasked || open[2], b.uym, b.ctr, b.cd[2], b.rcd[2]);
+  const Vec3 fyf = predict_flux<kTables>(!kMasked || open[3], b.ctr, b.uyp, b.cd[3], b.rcd[3]);
+  const Vec3 fzb = predict_flux<kTables>(!kMasked || open[4], b.uzm, b.ctr, b.cd[4], b.rcd[4]);
+  const Vec3 fzf = predict_flux<kTables>(!kMasked || open[5], b.ctr, b.uzp, b.cd[5], b.rcd[5]);
+  Vec3 lap;
+  lap.x = (predict_lap<kTables>(fxf.x, fxb.x, b.w[0], b.rw[0]) + predict_lap<kTables>(fyf.x, fyb.x, b.w[1], b.rw[1])) +
+          predict_lap<kTables>(fzf.x, fzb.x, b.w[2], b.rw[2]);
+  lap.y = (predict_lap<kTables>(fxf.y, fxb.y, b.w[0], b.rw[0]) + predict_lap<kTables>(fyf.y, fyb.y, b.w[1], b.rw[1])) +
+          predict_lap<kTables>(fzf.y, fzb.y, b.w[2], b.rw[2]);
+  lap.z = (predict_lap<kTables>(fxf.z, fxb.z, b.w[0], b.rw[0]) + predict_lap<kTables>(fyf.z, fyb.z, b.w[1], b.rw[1])) +
+          predict_lap<kTables>(fzf.z, fzb.z, b.w[2], b.rw[2]);
+  return lap;
 }
 
 // kLds: centres and reciprocal tables in LDS (nx + ny + nz <= kPredictLdsMaxCells), all
@@ -276,10 +321,10 @@ __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
   const float dtnu = a.dt * a.nu;
   const Off sx = (Off)a.ny * a.nz, sy = a.nz;
   HFSI_CELL_LOOP(a.n) {
-    const uint32_t k = c % a.nz;
-    const uint32_t t = c / a.nz;
-    const uint32_t j = t % a.ny;
-    const uint32_t i = t / a.ny;
+    const uint32_t t = a.fz.div(c);
+    const uint32_t k = c - t * a.nz;
+    const uint32_t i = a.fy.div(t);
+    const uint32_t j = t - i * a.ny;
     // every load below is at a clamped, in-range index; masks select what counts
     const Off cc = c;
     const int32_t oc = ld_obst(a.obst, cc * 4);
@@ -312,35 +357,43 @@ __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
     const Vec3 adv = predict_sample<kLds, Off>(a, gx, gy, gz, ctr);
 
     const uint32_t ip = i + 1 < a.nx ? i + 1 : i, jp = j + 1 < a.ny ? j + 1 : j, kp = k + 1 < a.nz ? k + 1 : k;
-    const float cdxm = ax[0].cd[i], cdxp = ax[0].cd[ip], cdym = ax[1].cd[j], cdyp = ax[1].cd[jp];
-    const float cdzm = ax[2].cd[k], cdzp = ax[2].cd[kp];
-    const float wx = ax[0].w[i], wy = ax[1].w[j], wz = ax[2].w[k];
-    float rcdxm = 0.f, rcdxp = 0.f, rcdym = 0.f, rcdyp = 0.f, rcdzm = 0.f, rcdzp = 0.f, rwx = 0.f, rwy = 0.f,
-          rwz = 0.f;
+    PredictNb b;
+    b.ctr = ctr;
+    b.uxm = uxm;
+    b.uxp = uxp;
+    b.uym = uym;
+    b.uyp = uyp;
+    b.uzm = uzm;
+    b.uzp = uzp;
+    b.cd[0] = ax[0].cd[i];
+    b.cd[1] = ax[0].cd[ip];
+    b.cd[2] = ax[1].cd[j];
+    b.cd[3] = ax[1].cd[jp];
+    b.cd[4] = ax[2].cd[k];
+    b.cd[5] = ax[2].cd[kp];
+    b.w[0] = ax[0].w[i];
+    b.w[1] = ax[1].w[j];
+    b.w[2] = ax[2].w[k];
     if (kLds) {
-      rcdxm = ax[0].rcd[i];
-      rcdxp = ax[0].rcd[ip];
-      rcdym = ax[1].rcd[j];
-      rcdyp = ax[1].rcd[jp];
-      rcdzm = ax[2].rcd[k];
-      rcdzp = ax[2].rcd[kp];
-      rwx = ax[0].rw[i];
-      rwy = ax[1].rw[j];
-      rwz = ax[2].rw[k];
+      b.rcd[0] = ax[0].rcd[i];
+      b.rcd[1] = ax[0].rcd[ip];
+      b.rcd[2] = ax[1].rcd[j];
+      b.rcd[3] = ax[1].rcd[jp];
+      b.rcd[4] = ax[2].rcd[k];
+      b.rcd[5] = ax[2].rcd[kp];
+      b.rw[0] = ax[0].rw[i];
+      b.rw[1] = ax[1].rw[j];
+      b.rw[2] = ax[2].rw[k];
+    } else {
+      for (int q = 0; q < 6; ++q) b.rcd[q] = 0.0f;
+      for (int q = 0; q < 3; ++q) b.rw[q] = 0.0f;
     }
-    const Vec3 fxb = predict_flux<kLds>(i > 0 && oxm == 0, uxm, ctr, cdxm, rcdxm);
-    const Vec3 fxf = predict_flux<kLds>(i + 1 < a.nx && oxp == 0, ctr, uxp, cdxp, rcdxp);
-    const Vec3 fyb = predict_flux<kLds>(j > 0 && oym == 0, uym, ctr, cdym, rcdym);
-    const Vec3 fyf = predict_flux<kLds>(j + 1 < a.ny && oyp == 0, ctr, uyp, cdyp, rcdyp);
-    const Vec3 fzb = predict_flux<kLds>(k > 0 && ozm == 0, uzm, ctr, cdzm, rcdzm);
-    const Vec3 fzf = predict_flux<kLds>(k + 1 < a.nz && ozp == 0, ctr, uzp, cdzp, rcdzp);
-    Vec3 lap;
-    lap.x = (predict_lap<kLds>(fxf.x, fxb.x, wx, rwx) + predict_lap<kLds>(fyf.x, fyb.x, wy, rwy)) +
-            predict_lap<kLds>(fzf.x, fzb.x, wz, rwz);
-    lap.y = (predict_lap<kLds>(fxf.y, fxb.y, wx, rwx) + predict_lap<kLds>(fyf.y, fyb.y, wy, rwy)) +
-            predict_lap<kLds>(fzf.y, fzb.y, wz, rwz);
-    lap.z = (predict_lap<kLds>(fxf.z, fxb.z, wx, rwx) + predict_lap<kLds>(fyf.z, fyb.z, wy, rwy)) +
-            predict_lap<kLds>(fzf.z, fzb.z, wz, rwz);
+    const bool open[6] = {i > 0 && oxm == 0, i + 1 < a.nx && oxp == 0, j > 0 && oym == 0,
+                          j + 1 < a.ny && oyp == 0, k > 0 && ozm == 0, k + 1 < a.nz && ozp == 0};
+    const bool shut = !(open[0] && open[1] && open[2] && open[3] && open[4] && open[5]);
+    // every face of every lane of the wave open: the same fluxes without the masks
+    const Vec3 lap = __builtin_amdgcn_ballot_w64(shut) == 0 ? predict_laplacian<kLds, false>(b, open)
+                                                            : predict_laplacian<kLds, true>(b, open);
     st3(a.v, cc * 12, Vec3{adv.x + dtnu * lap.x, adv.y + dtnu * lap.y, adv.z + dtnu * lap.z});
   }
 }

@@ -3227,11 +3227,17 @@ int hfsi_fluid_predict(hfsi_fluid* h, const float* velocity_prev, const int32_t*
     set_error("hfsi_fluid_predict: the predictor needs at least 2 cells along every axis");
     return HFSI_ERR_ARG;
   }
+  if (m.n >= (1u << 31)) {
+    set_error("hfsi_fluid_predict: grids of 2^31 cells or more are not supported");
+    return HFSI_ERR_ARG;
+  }
   PredictArgs a;
   a.nx = m.nx;
   a.ny = m.ny;
   a.nz = m.nz;
   a.n = m.n;
+  a.fz = FastDiv::make(m.nz);
+  a.fy = FastDiv::make(m.ny);
   for (int d = 0; d < 3; ++d) {
     if (!metrics->faces[d] || !metrics->centers[d] || !metrics->widths[d] || !metrics->center_distances[d]) {
       set_error("hfsi_fluid_predict: missing metric array");
