@@ -34,11 +34,12 @@ namespace hfsi {
 // LDS tables per workgroup (kLds): for each axis the centres, widths, centre distances
 // and the correctly rounded reciprocals of the widths, centre distances and centre gaps,
 // 6 floats per cell.
+// With 32-bit offsets the x axis also keeps its plane offsets i * ny * nz in LDS.
 constexpr uint32_t kPredictLdsTables = 6;
-constexpr uint32_t kPredictLdsMaxCells = 2048;  // nx + ny + nz; 48 KiB of LDS
+constexpr uint32_t kPredictLdsMaxCells = 2048;  // nx + ny + nz; at most 56 KiB of LDS
 
 struct PredictArgs {
-  uint32_t nx, ny, nz, n;  // n < 2^31
+  uint32_t nx, ny, nz, n;  // n < 2^31, every axis < 2^24
   FastDiv fz, fy;          // division by nz and by ny
   const float* face[3];    // n_a + 1 face coordinates
   const float* centre[3];  // n_a centres
@@ -74,6 +75,14 @@ template <typename Off>
 __device__ inline int32_t ld_obst(const int32_t* p, Off byte_off) {
   return *(const int32_t*)((const char*)p + byte_off);
 }
+
+// 3 x with one full-rate shift-add (a 32-bit multiply would be a quarter-rate instruction).
+__device__ inline uint32_t times3(uint32_t x) {
+  uint32_t r;
+  asm("v_lshl_add_u32 %0, %1, 1, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+__device__ inline uint64_t times3(uint64_t x) { return 3 * x; }
 
 // RN(n / d) for d > 0 from y = RN(1 / d).  q = RN(n y) is within 2 ulps of n / d; one
 // residual correction brings it within 1 ulp, and a second one, with the residual
@@ -166,22 +175,29 @@ __device__ inline uint32_t predict_corner(float g, uint32_t n, float& t) {
 }
 
 // Obstacle-aware trilinear sample of a.vp at grid coordinates (gx, gy, gz) (fluid.py:2382-2404).
+// `plane` (LDS; read with kTables and 32-bit offsets only): i * ny * nz for each plane i.
 template <bool kTables, typename Off>
-__device__ inline Vec3 predict_sample(const PredictArgs& a, float gx, float gy, float gz, Vec3 fallback) {
+__device__ inline Vec3 predict_sample(const PredictArgs& a, const uint32_t* plane, float gx, float gy, float gz,
+                                      Vec3 fallback) {
 #pragma clang fp contract(off)
   float tx, ty, tz;
   const uint32_t i0 = predict_corner(gx, a.nx, tx);
   const uint32_t j0 = predict_corner(gy, a.ny, ty);
   const uint32_t k0 = predict_corner(gz, a.nz, tz);
   const Off sx = (Off)a.ny * a.nz, sy = a.nz;
-  const Off base = ((Off)i0 * a.ny + j0) * a.nz + k0;
+  Off base;
+  if constexpr (sizeof(Off) == 4 && kTables) {
+    base = plane[i0] + __umul24(j0, a.nz) + k0;
+  } else {
+    base = ((Off)i0 * a.ny + j0) * a.nz + k0;
+  }
   int32_t ob[8];
   Vec3 u[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const Off c = base + ((q >> 2) & 1) * sx + ((q >> 1) & 1) * sy + (q & 1);
-    ob[q] = ld_obst(a.obst, c * 4);
-    u[q] = ld3(a.vp, c * 12);
+    const Off c4 = (base + ((q >> 2) & 1) * sx + ((q >> 1) & 1) * sy + (q & 1)) * 4;
+    ob[q] = ld_obst(a.obst, c4);
+    u[q] = ld3(a.vp, times3(c4));
   }
   Vec3 val{0.0f, 0.0f, 0.0f};
   float fw = 0.0f;
@@ -298,6 +314,10 @@ __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
       A.rgap = s_tab + 3 * tot + off;
       A.w = s_tab + 4 * tot + off;
       A.cd = s_tab + 5 * tot + off;
+      if (sizeof(Off) == 4 && d == 0) {
+        uint32_t* plane = (uint32_t*)(s_tab + kPredictLdsTables * tot);
+        for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) plane[t] = t * (a.ny * a.nz);
+      }
       for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
         const float c = a.centre[d][t];
         const float w = a.width[d][t];
@@ -316,6 +336,7 @@ __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
     off += n;
   }
   if (kLds) __syncthreads();
+  const uint32_t* plane = (kLds && sizeof(Off) == 4) ? (const uint32_t*)(s_tab + kPredictLdsTables * tot) : nullptr;
 
   const float dt = a.dt;
   const float dtnu = a.dt * a.nu;
@@ -326,20 +347,21 @@ __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
     const uint32_t i = a.fy.div(t);
     const uint32_t j = t - i * a.ny;
     // every load below is at a clamped, in-range index; masks select what counts
-    const Off cc = c;
-    const int32_t oc = ld_obst(a.obst, cc * 4);
-    const Vec3 ctr = ld3(a.vp, cc * 12);
-    const Off cxm = i > 0 ? cc - sx : cc, cxp = i + 1 < a.nx ? cc + sx : cc;
-    const Off cym = j > 0 ? cc - sy : cc, cyp = j + 1 < a.ny ? cc + sy : cc;
-    const Off czm = k > 0 ? cc - 1 : cc, czp = k + 1 < a.nz ? cc + 1 : cc;
-    const int32_t oxm = ld_obst(a.obst, cxm * 4), oxp = ld_obst(a.obst, cxp * 4);
-    const int32_t oym = ld_obst(a.obst, cym * 4), oyp = ld_obst(a.obst, cyp * 4);
-    const int32_t ozm = ld_obst(a.obst, czm * 4), ozp = ld_obst(a.obst, czp * 4);
-    const Vec3 uxm = ld3(a.vp, cxm * 12), uxp = ld3(a.vp, cxp * 12);
-    const Vec3 uym = ld3(a.vp, cym * 12), uyp = ld3(a.vp, cyp * 12);
-    const Vec3 uzm = ld3(a.vp, czm * 12), uzp = ld3(a.vp, czp * 12);
+    const Off c4 = (Off)c * 4, sx4 = sx * 4, sy4 = sy * 4;
+    const Off c12 = times3(c4);
+    const Off xm4 = i > 0 ? c4 - sx4 : c4, xp4 = i + 1 < a.nx ? c4 + sx4 : c4;
+    const Off ym4 = j > 0 ? c4 - sy4 : c4, yp4 = j + 1 < a.ny ? c4 + sy4 : c4;
+    const Off zm4 = k > 0 ? c4 - 4 : c4, zp4 = k + 1 < a.nz ? c4 + 4 : c4;
+    const int32_t oc = ld_obst(a.obst, c4);
+    const Vec3 ctr = ld3(a.vp, c12);
+    const int32_t oxm = ld_obst(a.obst, xm4), oxp = ld_obst(a.obst, xp4);
+    const int32_t oym = ld_obst(a.obst, ym4), oyp = ld_obst(a.obst, yp4);
+    const int32_t ozm = ld_obst(a.obst, zm4), ozp = ld_obst(a.obst, zp4);
+    const Vec3 uxm = ld3(a.vp, times3(xm4)), uxp = ld3(a.vp, times3(xp4));
+    const Vec3 uym = ld3(a.vp, times3(ym4)), uyp = ld3(a.vp, times3(yp4));
+    const Vec3 uzm = ld3(a.vp, times3(zm4)), uzp = ld3(a.vp, times3(zp4));
     if (oc == 1) {
-      st3(a.v, cc * 12, Vec3{0.0f, 0.0f, 0.0f});
+      st3(a.v, c12, Vec3{0.0f, 0.0f, 0.0f});
       continue;
     }
     const float x = ax[0].cen[i], y = ax[1].cen[j], z = ax[2].cen[k];
@@ -349,12 +371,12 @@ __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
       const float mgx = predict_coord<kLds>(x - hd * ctr.x, i, ax[0]);
       const float mgy = predict_coord<kLds>(y - hd * ctr.y, j, ax[1]);
       const float mgz = predict_coord<kLds>(z - hd * ctr.z, k, ax[2]);
-      tr = predict_sample<kLds, Off>(a, mgx, mgy, mgz, ctr);
+      tr = predict_sample<kLds, Off>(a, plane, mgx, mgy, mgz, ctr);
     }
     const float gx = predict_coord<kLds>(x - dt * tr.x, i, ax[0]);
     const float gy = predict_coord<kLds>(y - dt * tr.y, j, ax[1]);
     const float gz = predict_coord<kLds>(z - dt * tr.z, k, ax[2]);
-    const Vec3 adv = predict_sample<kLds, Off>(a, gx, gy, gz, ctr);
+    const Vec3 adv = predict_sample<kLds, Off>(a, plane, gx, gy, gz, ctr);
 
     const uint32_t ip = i + 1 < a.nx ? i + 1 : i, jp = j + 1 < a.ny ? j + 1 : j, kp = k + 1 < a.nz ? k + 1 : k;
     PredictNb b;
@@ -394,7 +416,7 @@ __global__ void __launch_bounds__(kBlock) k_predict(PredictArgs a) {
     // every face of every lane of the wave open: the same fluxes without the masks
     const Vec3 lap = __builtin_amdgcn_ballot_w64(shut) == 0 ? predict_laplacian<kLds, false>(b, open)
                                                             : predict_laplacian<kLds, true>(b, open);
-    st3(a.v, cc * 12, Vec3{adv.x + dtnu * lap.x, adv.y + dtnu * lap.y, adv.z + dtnu * lap.z});
+    st3(a.v, c12, Vec3{adv.x + dtnu * lap.x, adv.y + dtnu * lap.y, adv.z + dtnu * lap.z});
   }
 }
 

@@ -3227,8 +3227,8 @@ int hfsi_fluid_predict(hfsi_fluid* h, const float* velocity_prev, const int32_t*
     set_error("hfsi_fluid_predict: the predictor needs at least 2 cells along every axis");
     return HFSI_ERR_ARG;
   }
-  if (m.n >= (1u << 31)) {
-    set_error("hfsi_fluid_predict: grids of 2^31 cells or more are not supported");
+  if (m.n >= (1u << 31) || m.nx >= (1u << 24) || m.ny >= (1u << 24) || m.nz >= (1u << 24)) {
+    set_error("hfsi_fluid_predict: grids of 2^31 cells or more, or 2^24 cells along an axis, are not supported");
     return HFSI_ERR_ARG;
   }
   PredictArgs a;
@@ -3255,8 +3255,8 @@ int hfsi_fluid_predict(hfsi_fluid* h, const float* velocity_prev, const int32_t*
   a.nu = nu_m2_s;
   const uint32_t ncells = m.nx + m.ny + m.nz;
   const bool lds = ncells <= kPredictLdsMaxCells;
-  const size_t shmem = lds ? kPredictLdsTables * (size_t)ncells * sizeof(float) : 0;
   const bool off32 = (uint64_t)m.n * 12 < (1ull << 32);
+  const size_t shmem = lds ? (kPredictLdsTables * (size_t)ncells + (off32 ? m.nx : 0)) * sizeof(float) : 0;
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(blocks_for(m.n)), block(kBlock);
   timer_mark(h, HFSI_TIMER_PREDICT, st);
