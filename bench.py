@@ -434,6 +434,20 @@ def run_mpm(steps: int, warmup: int, lattice: int = 100, nodes: int = 256):
 # Algorithmic HBM bytes per cell of one predictor launch: read the previous velocity (12 B) and the
 # obstacle (4 B), write the new velocity (12 B); neighbour and departure-point re-reads are cache traffic.
 PREDICT_BYTES_PER_CELL = 28
+PREDICT_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_predict_256_graded_hbm_traffic.md")
+
+
+def measured_predict_traffic(scheme: str, path: str = PREDICT_TRAFFIC_PROFILE):
+    """HBM bytes per predictor launch from the committed FETCH_SIZE/WRITE_SIZE summary, or None."""
+    key = f"traffic_bytes_per_launch_{scheme}:"
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith(key):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def run_predict(solver, steps: int, warmup: int, cpu: bool, cpu_budget: float = 10.0):
@@ -475,7 +489,8 @@ def run_predict(solver, steps: int, warmup: int, cpu: bool, cpu_budget: float = 
             "kernel_avg_us": round(avg_s * 1e6, 2), "launches": cnt.value,
             "roofline": {"bound": "hbm", "kernel": f"k_predict ({scheme})", "achieved": round(achieved, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "traffic": None, "bytes_per_cell": PREDICT_BYTES_PER_CELL},
+                         "traffic": measured_predict_traffic(scheme) if n == 256 ** 3 else None,
+                         "bytes_per_cell": PREDICT_BYTES_PER_CELL},
         }
     if cpu:
         from oracle import predict as P
