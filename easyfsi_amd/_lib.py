@@ -24,6 +24,7 @@ HFSI_PREC_FP32 = 2
 HFSI_PREC_MIXED = 4
 
 ADVECTION_SCHEMES = {"euler": 0, "rk2": 1}
+INTERFACE_SLOTS = 16  # HFSI_INTERFACE_SLOTS: primary + 15 extra coupling slots per owner cell
 
 PRECONDITIONERS = {"auto": 0, "jacobi": 1, "fv_multigrid": 2, "fv_multigrid_light": 3}
 
@@ -97,6 +98,16 @@ class AxisMetrics(ctypes.Structure):
                 ("center_distances", ctypes.c_void_p * 3)]
 
 
+class InterfaceFields(ctypes.Structure):
+    """hfsi_interface_fields: device pointers of the pressure-interface coupling fields."""
+
+    _fields_ = [("slot_active", ctypes.c_void_p), ("slot_neighbor", ctypes.c_void_p),
+                ("slot_coefficient", ctypes.c_void_p), ("extra_neighbor", ctypes.c_void_p),
+                ("extra_coefficient", ctypes.c_void_p), ("row_owner", ctypes.c_void_p),
+                ("row_neighbor", ctypes.c_void_p), ("row_transmissibility", ctypes.c_void_p),
+                ("row_count", ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 _D = ctypes.c_double
@@ -125,6 +136,8 @@ SIGNATURES = {
     "hfsi_fluid_subtract_gradient": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F, _I, _P]),
     "hfsi_fluid_wall_clamp": (ctypes.c_int, [_P, _P, _I, _P]),
     "hfsi_fluid_divergence_report": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P]),
+    "hfsi_fluid_set_interface_couplings": (ctypes.c_int, [_P, _P, ctypes.POINTER(InterfaceFields), _P, _P, _P]),
+    "hfsi_fluid_interface_apply": (ctypes.c_int, [_P, _P, _P, _D, _P]),
     "hfsi_fluid_predict": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(AxisMetrics), _F, _F, _I, _P]),
     "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
     "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),

@@ -26,6 +26,7 @@
 #include "fluid_march.hpp"
 #include "fluid_mixed.hpp"
 #include "fluid_predict.hpp"
+#include "fluid_interface.hpp"
 
 namespace hfsi {
 
@@ -175,6 +176,17 @@ struct OpDot {
   const TB* b;
   __device__ void operator()(uint32_t c, double V, double (&acc)[1]) const {
     acc[0] += (double)a[c] * (double)b[c] * V;
+  }
+};
+
+// {<r,r>_V, sum V r} of a stored residual (the sums of k_true_residual).
+template <typename TR>
+struct OpResidualSums {
+  const TR* r;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[2]) const {
+    const double v = (double)r[c];
+    acc[0] += v * v * V;
+    acc[1] += v * V;
   }
 };
 
@@ -1217,6 +1229,10 @@ struct hfsi_fluid {
   int timer_used = 0;
   double timer_ms = 0.0;
   long timer_count = 0;
+  // compiled pressure-interface couplings (hfsi_fluid_set_interface_couplings)
+  hfsi::CoupleList cpl{};
+  uint32_t cpl_n = 0, cpl_cap = 0;
+  hfsi::CoupleStats* cpl_stats = nullptr;
 };
 
 namespace hfsi {
@@ -1314,6 +1330,16 @@ static int launch_sums(hfsi_fluid* h, const OpGeo<C>& g, Op op, Fin fin, hipStre
   const uint32_t nb = blocks_for(g.n);
   hipLaunchKernelGGL((k_sums<NS, C, Op>), dim3(nb), dim3(kBlock), 0, st, g, op, h->partials);
   finish_sums<NS>(h, nb, fin, st);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+// out += scale * (C p) over the compiled interface couplings.
+template <typename TP, typename TO>
+static int launch_couple(hfsi_fluid* h, const TP* p, TO* out, double scale, hipStream_t st) {
+  if (!h->cpl_n) return HFSI_OK;
+  const uint32_t blocks = std::min<uint32_t>((h->cpl_n + kBlock - 1) / kBlock, kMaxGrid);
+  hipLaunchKernelGGL((k_couple_apply<TP, TO>), dim3(blocks), dim3(kBlock), 0, st, h->cpl, h->cpl_n, p, out, scale);
   HFSI_LAUNCH_CHECK();
   return HFSI_OK;
 }
@@ -1931,6 +1957,9 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
                     const hfsi_cg_options& o, hfsi_cg_stats& S, hipStream_t st, bool mixed = false) {
   Vecs<C, TV> v = vecs<C, TV>(h);
   const int outlet = o.outlet_zmin ? 1 : 0;
+  // interface couplings: the operator is the stencil plus the coupling list, so q = A d and
+  // the true residual take an unfused stencil launch followed by the coupling pass
+  const bool coupled = h->cpl_n > 0;
   const int rm = o.remove_mean ? 1 : 0;
   OpGeo<C> g = geo<C>(h, 0, outlet);
   OpGeo<double> g64 = geo<double>(h, 0, outlet);
@@ -1975,7 +2004,13 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   auto true_residual = [&](bool count_projection) -> int {
     hipLaunchKernelGGL((k_true_residual<TV>), dim3(blocks_for(g64.n)), dim3(kBlock), 0, st, g64, x, h->b, v.r,
                        h->partials);
-    finish_sums<2>(h, blocks_for(g64.n), FinMeanR{sc, rm}, st);
+    if (coupled) {
+      HFSI_LAUNCH_CHECK();
+      HFSI_TRY((launch_couple<double, TV>(h, x, v.r, -1.0, st)));
+      HFSI_TRY(launch_sums<2>(h, g64, OpResidualSums<TV>{v.r}, FinMeanR{sc, rm}, st));
+    } else {
+      finish_sums<2>(h, blocks_for(g64.n), FinMeanR{sc, rm}, st);
+    }
     HFSI_LAUNCH_CHECK();
     if (rm) {
       HFSI_TRY(launch_sums<1>(h, g64, OpSubMeans<TV>{nullptr, v.r, sc}, FinRR{sc}, st));
@@ -2002,7 +2037,9 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   const bool light = o.preconditioner == HFSI_PRECOND_MG_LIGHT;
   const int pre = light ? 1 : 2, coarse = light ? 12 : 24, post = light ? 1 : 2;
   // mixed V-cycle: float32 CG vectors, no velocity-Dirichlet or interface terms
-  bool use_mix = mixed && use_mg && std::is_same<TV, float>::value && h->ML >= 1 && !h->has_vd && !h->has_diag;
+  bool use_mix = mixed && use_mg && std::is_same<TV, float>::value && h->ML >= 1 && !h->has_vd && !h->has_diag &&
+                 h->cpl_n == 0;
+
   // coarsest mixed level: the float64 dense solve when it is set up (0), else
   // RB-GS sweeps (4, 8 and 24 sweeps give the same iteration counts)
   const int mix_coarse = h->mixd_m > 0 ? 0 : (light ? 4 : 8);
@@ -2059,7 +2096,12 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
   while (it < max_it) {
     ++it;
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
-    if (use_mix) {
+    if (coupled) {
+      hipLaunchKernelGGL((k_apply<C, TV, TV>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, v.d, v.q);
+      HFSI_LAUNCH_CHECK();
+      HFSI_TRY((launch_couple<TV, TV>(h, v.d, v.q, 1.0, st)));
+      HFSI_TRY(launch_sums<1>(h, g64, OpDot<TV, TV>{v.d, v.q}, FinDad{sc}, st));
+    } else if (use_mix) {
       // q = A d with float64 arithmetic on the float32 d: for smooth d the stencil
       // sum den d - sum c d_nb cancels to a small fraction of den d, and float32
       // arithmetic would leave q -- and through r -= alpha q the recursive residual --
@@ -2079,7 +2121,7 @@ static int solve_cg(hfsi_fluid* h, const float* div, const float* src, const dou
                          h->partials);
     }
     timer_mark(h, HFSI_TIMER_SPMV_DOT, st);
-    if (!march_ok(h, 0)) finish_sums<1>(h, blocks_for(use_mix ? g64.n : g.n), FinDad{sc}, st);
+    if (!coupled && !march_ok(h, 0)) finish_sums<1>(h, blocks_for(use_mix ? g64.n : g.n), FinDad{sc}, st);
     HFSI_LAUNCH_CHECK();
     const bool monitor_hit = monitor_pending;
     monitor_pending = false;
@@ -2846,6 +2888,8 @@ int hfsi_fluid_destroy(hfsi_fluid* h) {
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
   if (h->host_totals) (void)hipHostFree(h->host_totals);
+  for (void* p : {(void*)h->cpl.a, (void*)h->cpl.b, (void*)h->cpl.ca, (void*)h->cpl.cb, (void*)h->cpl_stats})
+    if (p) (void)hipFree(p);
   delete h;
   return HFSI_OK;
 }
@@ -2992,7 +3036,104 @@ int hfsi_fluid_apply(hfsi_fluid* h, const double* p, double* q, int32_t outlet_z
   hipLaunchKernelGGL((k_apply<double, double, double>), dim3(blocks_for(g.n)), dim3(kBlock), 0, (hipStream_t)stream,
                      g, p, q);
   HFSI_LAUNCH_CHECK();
+  if (h->cpl_n) {
+    const int rc = launch_couple<double, double>(h, p, q, 1.0, (hipStream_t)stream);
+    if (rc != HFSI_OK) return rc;
+  }
   return HFSI_OK;
+}
+
+int hfsi_fluid_set_interface_couplings(hfsi_fluid* h, const int32_t* obstacle, const hfsi_interface_fields* f,
+                                       double* row_diagonal_density, double* report, void* stream) {
+  if (!h || !obstacle || !f) {
+    set_error("hfsi_fluid_set_interface_couplings: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  const LevelMem& m = h->lv[0];
+  const bool rows = f->row_count > 0;
+  if (rows ? (!f->row_owner || !f->row_neighbor || !f->row_transmissibility)
+           : (f->slot_active && (!f->slot_neighbor || !f->slot_coefficient ||
+                                 !f->extra_neighbor != !f->extra_coefficient))) {
+    set_error("hfsi_fluid_set_interface_couplings: missing coupling array");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (!h->cpl_stats) HFSI_CHECK(hipMalloc((void**)&h->cpl_stats, sizeof(CoupleStats)));
+  CoupleSrc src{};
+  src.nx = m.nx;
+  src.ny = m.ny;
+  src.nz = m.nz;
+  src.n = m.n;
+  src.obst = obstacle;
+  for (int d = 0; d < 3; ++d) src.w[d] = m.wf[d];
+  src.slot_active = f->slot_active;
+  src.slot_nb = f->slot_neighbor;
+  src.slot_coef = f->slot_coefficient;
+  src.extra_nb = f->extra_neighbor;
+  src.extra_coef = f->extra_coefficient;
+  src.row_owner = f->row_owner;
+  src.row_nb = f->row_neighbor;
+  src.row_T = f->row_transmissibility;
+  src.rows = rows ? (uint32_t)std::min<int64_t>(f->row_count, m.n) : 0;
+  const bool any = rows || f->slot_active;
+  const uint32_t work = rows ? src.rows : m.n;
+  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((work + kBlock - 1) / kBlock, kMaxGrid));
+  CoupleStats zero{};
+  CoupleStats host{};
+  HFSI_CHECK(hipMemcpyAsync(h->cpl_stats, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+  if (row_diagonal_density) HFSI_CHECK(hipMemsetAsync(row_diagonal_density, 0, (size_t)m.n * sizeof(double), st));
+  if (any) {
+    hipLaunchKernelGGL((k_couple_build<false>), dim3(blocks), dim3(kBlock), 0, st, src, h->cpl, h->cpl_stats,
+                       row_diagonal_density);
+    HFSI_LAUNCH_CHECK();
+  }
+  HFSI_CHECK(hipMemcpyAsync(&host, h->cpl_stats, sizeof(host), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  const uint32_t count = host.applied;
+  if (count > h->cpl_cap) {
+    for (void* p : {(void*)h->cpl.a, (void*)h->cpl.b, (void*)h->cpl.ca, (void*)h->cpl.cb})
+      if (p) HFSI_CHECK(hipFree(p));
+    h->cpl = CoupleList{};
+    h->cpl_cap = 0;
+    HFSI_CHECK(hipMalloc((void**)&h->cpl.a, count * sizeof(uint32_t)));
+    HFSI_CHECK(hipMalloc((void**)&h->cpl.b, count * sizeof(uint32_t)));
+    HFSI_CHECK(hipMalloc((void**)&h->cpl.ca, count * sizeof(double)));
+    HFSI_CHECK(hipMalloc((void**)&h->cpl.cb, count * sizeof(double)));
+    h->cpl_cap = count;
+    h->cpl.cap = count;
+  }
+  h->cpl_n = 0;
+  if (count) {
+    HFSI_CHECK(hipMemcpyAsync(h->cpl_stats, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_couple_build<true>), dim3(blocks), dim3(kBlock), 0, st, src, h->cpl, h->cpl_stats,
+                       (double*)nullptr);
+    HFSI_LAUNCH_CHECK();
+    CoupleStats filled{};
+    HFSI_CHECK(hipMemcpyAsync(&filled, h->cpl_stats, sizeof(filled), hipMemcpyDeviceToHost, st));
+    HFSI_CHECK(hipStreamSynchronize(st));
+    if (filled.applied != count) {
+      set_error("hfsi_fluid_set_interface_couplings: coupling fields changed between the two passes");
+      return HFSI_ERR_STATE;
+    }
+    h->cpl_n = count;
+  }
+  if (report) {
+    report[0] = (double)host.active;
+    report[1] = (double)host.invalid;
+    report[2] = rows ? (double)std::max<int64_t>((int64_t)f->row_count - (int64_t)m.n, 0) : 0.0;
+    report[3] = host.diag_integral;
+    report[4] = host.active ? __builtin_bit_cast(double, host.tmax_bits) : 0.0;
+    report[5] = (double)count;
+  }
+  return HFSI_OK;
+}
+
+int hfsi_fluid_interface_apply(hfsi_fluid* h, const double* p, double* out, double scale, void* stream) {
+  if (!h || !p || !out) {
+    set_error("hfsi_fluid_interface_apply: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  return launch_couple<double, double>(h, p, out, scale, (hipStream_t)stream);
 }
 
 int hfsi_fluid_diagonal(hfsi_fluid* h, double* diag, int32_t outlet_zmin, void* stream) {

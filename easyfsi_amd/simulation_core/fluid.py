@@ -7,10 +7,12 @@ torch tensors on the GPU (``easyfsi_amd.simulation_core.fields.Field``); all
 per-cell work of the projection runs in the hand-written HIP kernels of
 libhfsi (include/hfsi.h).
 
-Out of this build (raise ``NotImplementedError``): the HIBM pressure-interface
-row list and coupling slots (fluid.py:4341-4456) and their BiCGSTAB fallback,
-``predict`` (the semi-Lagrangian predictor, fluid.py:2447-2527), velocity
-constraints and the local divergence cleanup.
+The velocity predictor (``predict``, fluid.py:2447-2527) and the
+sharp-interface couplings of the operator (row list and per-owner slots,
+fluid.py:4346-4456, with their report and projection divergence) run on
+libhfsi too.  Out of this build (raise ``NotImplementedError``): the BiCGSTAB
+fallback of row-list solves (fluid.py:6264-6533), velocity constraints and
+the local divergence cleanup.
 """
 
 from __future__ import annotations
@@ -116,6 +118,10 @@ class CartesianFluidSolver:
         self.velocity_dirichlet_boundary_value_mps = vector_field(3, shape, f32)
         self.velocity_dirichlet_boundary_projection_weight = scalar_field(shape, f32)
         self.velocity_dirichlet_boundary_marker_region_id = scalar_field(shape, i32)
+        self.pressure_interface_row_capacity = int(self.nx * self.ny * self.nz)
+        self.pressure_interface_row_count = scalar_field((), i32)
+        self._iface_compiled = False
+        self._iface_row_report = (0.0,) * 6
         self.force = vector_field(3, shape, f32)
         self.fv_diag = scalar_field(shape, f64)
         self.cg_r = scalar_field(shape, f64)
@@ -192,6 +198,36 @@ class CartesianFluidSolver:
                 pass
             self._handle = None
 
+    # Sharp-interface coupling fields (fluid.py:772-811), allocated on first access: the
+    # 15 extra slots alone take 300 B per cell, so a solver that never couples pays nothing.
+    _INTERFACE_FIELDS = {
+        "pressure_interface_coupling_active": ((), torch.int32, ()),
+        "pressure_interface_coupling_neighbor": ((), torch.int32, (3,)),
+        "pressure_interface_coupling_coefficient": ((), torch.float64, ()),
+        "pressure_interface_coupling_extra_neighbor": ((_lib.INTERFACE_SLOTS - 1,), torch.int32, (3,)),
+        "pressure_interface_coupling_extra_coefficient": ((_lib.INTERFACE_SLOTS - 1,), torch.float64, ()),
+        "pressure_interface_row_diagonal_density": ((), torch.float64, ()),
+    }
+    _INTERFACE_ROW_FIELDS = {
+        "pressure_interface_row_owner": (torch.int32, (3,)),
+        "pressure_interface_row_neighbor": (torch.int32, (3,)),
+        "pressure_interface_row_transmissibility": (torch.float64, ()),
+    }
+
+    def __getattr__(self, name):
+        spec = CartesianFluidSolver._INTERFACE_FIELDS.get(name)
+        if spec is not None:
+            extra, dtype, comps = spec
+            f = Field(tuple(self.grid.grid_nodes) + extra, dtype, components=comps)
+        else:
+            rspec = CartesianFluidSolver._INTERFACE_ROW_FIELDS.get(name)
+            if rspec is None:
+                raise AttributeError(f"{type(self).__name__!r} object has no attribute {name!r}")
+            dtype, comps = rspec
+            f = Field((self.pressure_interface_row_capacity,), dtype, components=comps)
+        self.__dict__[name] = f
+        return f
+
     @staticmethod
     def _axis_field(values) -> Field:
         f = scalar_field((len(values),), torch.float32)
@@ -251,6 +287,10 @@ class CartesianFluidSolver:
                   self.cg_Ad, self.cg_z, self.cg_d):
             f.fill(0)
         self.velocity_dirichlet_boundary_marker_region_id.fill(-1)
+        self.pressure_interface_row_count.fill(0)
+        for name in (*CartesianFluidSolver._INTERFACE_FIELDS, *CartesianFluidSolver._INTERFACE_ROW_FIELDS):
+            if name in self.__dict__:
+                self.__dict__[name].fill(0)
 
     def clear_pressure(self) -> None:
         """fluid.py:4055-4073."""
@@ -327,6 +367,63 @@ class CartesianFluidSolver:
             self.velocity_dirichlet_boundary_projection_weight.ptr if has_vd else None,
             diag.data_ptr() if has_diag else None, self._stream), "hfsi_fluid_set_masks")
         self._masks_have_vd = has_vd
+        self._set_interface_couplings()
+
+    def _set_interface_couplings(self) -> None:
+        """Compile the row list (when its count is positive) or the per-owner slots into
+        the operator's coupling list (hfsi_fluid_set_interface_couplings)."""
+        d = self.__dict__
+        row_count = int(self.pressure_interface_row_count.tensor.item())
+        has_rows = row_count > 0
+        has_slots = (not has_rows and "pressure_interface_coupling_active" in d
+                     and bool(torch.any(d["pressure_interface_coupling_active"].tensor > 0).item()))
+        if not (has_rows or has_slots or self._iface_compiled):
+            self._iface_row_report = (0.0,) * 5 + (0.0,)
+            return
+        f = _lib.InterfaceFields()
+        if has_rows:
+            f.row_owner = self.pressure_interface_row_owner.ptr
+            f.row_neighbor = self.pressure_interface_row_neighbor.ptr
+            f.row_transmissibility = self.pressure_interface_row_transmissibility.ptr
+            f.row_count = row_count
+        elif has_slots:
+            f.slot_active = self.pressure_interface_coupling_active.ptr
+            f.slot_neighbor = self.pressure_interface_coupling_neighbor.ptr
+            f.slot_coefficient = self.pressure_interface_coupling_coefficient.ptr
+            if "pressure_interface_coupling_extra_neighbor" in d or "pressure_interface_coupling_extra_coefficient" in d:
+                f.extra_neighbor = self.pressure_interface_coupling_extra_neighbor.ptr
+                f.extra_coefficient = self.pressure_interface_coupling_extra_coefficient.ptr
+        rep = (ctypes.c_double * 6)()
+        _lib.check(self._lib.hfsi_fluid_set_interface_couplings(
+            self._handle, self.obstacle.ptr, ctypes.byref(f), self.pressure_interface_row_diagonal_density.ptr, rep,
+            self._stream), "hfsi_fluid_set_interface_couplings")
+        self._iface_row_report = tuple(rep)
+        self._iface_compiled = rep[5] > 0
+
+    def _fv_laplacian_apply_kernel(self, pressure: Field, output: Field, pressure_outlet_zmin: int = 0) -> None:
+        """Operator application of fluid.py:4280-4456 (stencil, interface diagonal and couplings)."""
+        self._set_masks()
+        _lib.check(self._lib.hfsi_fluid_apply(self._handle, pressure.tensor.contiguous().data_ptr(), output.ptr,
+                                              1 if pressure_outlet_zmin else 0, self._stream),
+                   "hfsi_fluid_apply")
+
+    def _update_pressure_interface_projection_divergence(self, dt_over_rho: float, accumulate: bool = False) -> None:
+        """fluid.py:5625-5693 (accumulate=False) and 5695-5759 (True): the interface part
+        of the operator applied to the pressure, times dt/rho."""
+        k = float(np.float32(dt_over_rho))
+        p = self.pressure.tensor
+        out = self.pressure_interface_projection_divergence_s.tensor
+        diag = self.pressure_interface_matrix_diagonal.tensor
+        obst = self.obstacle.tensor
+        if accumulate:
+            out.add_(torch.where(obst == 0, k * diag * p, torch.zeros_like(p)))
+            out.masked_fill_(obst != 0, 0.0)
+        else:
+            out.copy_(torch.where(obst == 1, torch.zeros_like(p),
+                                  k * (diag * p - self.pressure_interface_matrix_rhs.tensor)))
+        if self._iface_compiled:
+            _lib.check(self._lib.hfsi_fluid_interface_apply(self._handle, p.data_ptr(), out.data_ptr(), k,
+                                                            self._stream), "hfsi_fluid_interface_apply")
 
     # ------------------------------------------------------------------
     # divergence and its statistics
@@ -503,19 +600,29 @@ class CartesianFluidSolver:
                                                momentum_delta_n_s=tuple(float(v) for v in mom.tolist()))
 
     def pressure_interface_matrix_terms_report(self) -> dict:
-        """Cell-field part of fluid.py:3080-3115 (this build keeps no row list)."""
+        """fluid.py:3080-3115: the interface cell fields and, compiled from the row list or
+        the slots, the row counts, transmissibility sums and the row-diagonal density."""
+        self._set_interface_couplings()
         d = self.pressure_interface_matrix_diagonal.tensor
         r = self.pressure_interface_matrix_rhs.tensor
-        vol = (self.cell_width_x_m.tensor.double()[:, None, None] * self.cell_width_y_m.tensor.double()[None, :, None]
-               * self.cell_width_z_m.tensor.double()[None, None, :])
+        vol = ((self.cell_width_x_m.tensor[:, None, None] * self.cell_width_y_m.tensor[None, :, None])
+               * self.cell_width_z_m.tensor[None, None, :]).double()
+        dens = (self.__dict__["pressure_interface_row_diagonal_density"].tensor
+                if "pressure_interface_row_diagonal_density" in self.__dict__ and self._iface_row_report[5] + self._iface_row_report[1] > 0
+                else torch.zeros_like(d))
+        mismatch = (d - dens).abs()
+        active, invalid, overflow, row_diag, tmax, _ = self._iface_row_report
         return {
             "diagonal_integral": float((d * vol).sum().item()),
             "rhs_integral": float((r * vol).sum().item()),
             "max_abs_diagonal": float(d.abs().max().item()),
             "active_cells": int(((d.abs() > 0) | (r.abs() > 0)).sum().item()),
-            "row_count": 0, "row_active_count": 0, "row_invalid_count": 0, "row_overflow_count": 0,
-            "row_diagonal_integral": 0.0, "row_diagonal_integral_abs_mismatch": 0.0,
-            "row_diagonal_max_abs_density_mismatch": 0.0, "row_max_transmissibility": 0.0,
+            "row_count": int(self.pressure_interface_row_count.tensor.item()),
+            "row_active_count": int(active), "row_invalid_count": int(invalid), "row_overflow_count": int(overflow),
+            "row_diagonal_integral": float(row_diag),
+            "row_diagonal_integral_abs_mismatch": float((mismatch * vol).sum().item()),
+            "row_diagonal_max_abs_density_mismatch": float(mismatch.max().item()),
+            "row_max_transmissibility": float(tmax),
         }
 
     def project(self, iterations: int = 40, pressure_outlet_zmin: bool = False, dt_s: float | None = None,
@@ -675,6 +782,7 @@ class CartesianFluidSolver:
         record_cg_stats()
         handle_failure()
         self._subtract_pressure_gradient(step_dt_s / self.rho, outlet)
+        self._update_pressure_interface_projection_divergence(step_dt_s / self.rho)
         apply_bcs()
         self._divergence_only(outlet)
         target_l2_sq = 0.0
@@ -706,6 +814,7 @@ class CartesianFluidSolver:
                 record_cg_stats()
                 handle_failure()
                 self._subtract_pressure_gradient(step_dt_s / self.rho, outlet)
+                self._update_pressure_interface_projection_divergence(step_dt_s / self.rho, accumulate=True)
                 self.pressure_accum.tensor.add_(self.pressure.tensor)
                 self.pressure.copy_from(self.pressure_accum)
                 self.pressure_tmp.copy_from(self.pressure_accum)
@@ -772,14 +881,15 @@ class CartesianFluidSolver:
             "pressure_interface_matrix_max_abs_diagonal": iface["max_abs_diagonal"],
             "pressure_interface_matrix_active_cells": iface["active_cells"],
             "pressure_interface_matrix_active": bool(iface_active),
-            "pressure_interface_matrix_row_count": 0,
-            "pressure_interface_matrix_row_active_count": 0,
-            "pressure_interface_matrix_row_invalid_count": 0,
-            "pressure_interface_matrix_row_overflow_count": 0,
-            "pressure_interface_matrix_row_diagonal_integral": 0.0,
-            "pressure_interface_matrix_row_diagonal_integral_abs_mismatch": 0.0,
-            "pressure_interface_matrix_row_diagonal_max_abs_density_mismatch": 0.0,
-            "pressure_interface_matrix_row_max_transmissibility": 0.0,
+            "pressure_interface_matrix_row_count": iface["row_count"],
+            "pressure_interface_matrix_row_active_count": iface["row_active_count"],
+            "pressure_interface_matrix_row_invalid_count": iface["row_invalid_count"],
+            "pressure_interface_matrix_row_overflow_count": iface["row_overflow_count"],
+            "pressure_interface_matrix_row_diagonal_integral": iface["row_diagonal_integral"],
+            "pressure_interface_matrix_row_diagonal_integral_abs_mismatch": iface["row_diagonal_integral_abs_mismatch"],
+            "pressure_interface_matrix_row_diagonal_max_abs_density_mismatch":
+                iface["row_diagonal_max_abs_density_mismatch"],
+            "pressure_interface_matrix_row_max_transmissibility": iface["row_max_transmissibility"],
             "pressure_nullspace_policy": ns_policy,
             "pressure_nullspace_compatibility_measured": bool(ns_measured),
             "pressure_nullspace_zero_mean_projection_applied": bool(zero_mean),

@@ -316,6 +316,47 @@ int hfsi_unet_adam_step(hfsi_unet* h, float lr, float beta1, float beta2, float 
                         void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Sharp-interface pressure couplings
+ * ------------------------------------------------------------------------- */
+
+#define HFSI_INTERFACE_SLOTS 16 /* primary + 15 extra slots per owner cell */
+
+/* The solver's pressure-interface coupling fields (simulation_core/fluid.py:772-811),
+ * device pointers: per-owner slots (count, first neighbour [n][3] and coefficient
+ * [n], extra neighbours [n][15][3] and coefficients [n][15]; slot_active may be NULL
+ * for none, the two extra arrays NULL together for all-zero extra slots) and the row list (owner and neighbour [rows][3], transmissibility
+ * [rows]), which replaces the slots when row_count > 0 (at most n rows are read). */
+typedef struct hfsi_interface_fields {
+  const int32_t* slot_active;
+  const int32_t* slot_neighbor;
+  const double* slot_coefficient;
+  const int32_t* extra_neighbor;
+  const double* extra_coefficient;
+  const int32_t* row_owner;
+  const int32_t* row_neighbor;
+  const double* row_transmissibility;
+  int32_t row_count;
+} hfsi_interface_fields;
+
+/* Compile the off-diagonal interface terms of the operator
+ * (_fv_laplacian_apply_kernel, fluid.py:4346-4456): every valid row or slot joins
+ * fluid cells a and b as (A p)[a] -= T/V_a p[b], (A p)[b] -= T/V_b p[a].  From then
+ * on hfsi_fluid_apply and the CG solves include them (the mixed V-cycle path is not
+ * used while couplings exist).  Call again after the fields or the obstacle change;
+ * a call with no rows and slot_active NULL removes them.  Also fills, when not NULL,
+ * row_diagonal_density [n] (device; sum of T/V over report-valid rows) and
+ * report[6] (host): report-valid rows, invalid rows, overflow rows, sum of 2T,
+ * max T, couplings applied -- the row part of
+ * _pressure_interface_matrix_terms_report_kernel (fluid.py:2864-3078).
+ * Synchronises the stream. */
+int hfsi_fluid_set_interface_couplings(hfsi_fluid* h, const int32_t* obstacle, const hfsi_interface_fields* f,
+                                       double* row_diagonal_density, double* report, void* stream);
+/* out += scale * (C p) with C the compiled couplings: the coupling part of
+ * _update_pressure_interface_projection_divergence_kernel (fluid.py:5625-5693) with
+ * scale = dt/rho.  p and out are [n] float64. */
+int hfsi_fluid_interface_apply(hfsi_fluid* h, const double* p, double* out, double scale, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Velocity predictor
  * ------------------------------------------------------------------------- */
 

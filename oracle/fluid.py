@@ -422,7 +422,7 @@ class PcgResult:
 
 def pcg(rhs0: np.ndarray, pressure0: np.ndarray, hier: Hierarchy, *, outlet: bool, tolerance: float,
         max_iters: int, preconditioner: str = "auto", remove_mean: bool = True,
-        interface_active: bool = False) -> PcgResult:
+        interface_active: bool = False, couplings=None) -> PcgResult:
     """Flexible-beta PCG of ``_solve_pressure_poisson_fv_cg`` (fluid.py:6535-6811).
 
     ``rhs0`` is the legacy-sign level-0 right-hand side; the CG system is
@@ -455,6 +455,10 @@ def pcg(rhs0: np.ndarray, pressure0: np.ndarray, hier: Hierarchy, *, outlet: boo
             out.mean_projection_count += 1
             return subtract_weighted_mean(v, lvl)
         return v
+
+    def fv_apply(v, lvl_, outlet_):  # level-0 operator, with the interface couplings when given
+        q = _fv_apply_plain(v, lvl_, outlet_)
+        return q + coupling_apply(v, couplings) if couplings is not None and len(couplings.a) else q
 
     x = out.pressure
     r = np.where(free, b - fv_apply(x, lvl, outlet), 0.0)
@@ -787,3 +791,144 @@ def project_plain(velocity, pressure, obstacle, widths, *, rho: float, dt: float
     info["projection"] = divergence_l2_stats(div1, src, obst)
     info["final"] = divergence_l2_stats(div2, src, obst)
     return u, p, div2, info
+
+
+_fv_apply_plain = fv_apply
+
+# ---------------------------------------------------------------------------
+# Sharp-interface pressure couplings (fluid.py:4346-4456, 2864-3078, 5584-5759)
+# ---------------------------------------------------------------------------
+
+COUPLING_SLOTS = 16  # one primary + 15 extra slots per owner cell (pressure_interface.py:10-13)
+
+
+@dataclass
+class Couplings:
+    """Off-diagonal interface terms: (A p)[a] -= ca * p[b] and (A p)[b] -= cb * p[a]."""
+
+    a: np.ndarray
+    b: np.ndarray
+    ca: np.ndarray
+    cb: np.ndarray
+
+
+def cell_volumes_f32(widths) -> np.ndarray:
+    """``_cell_volume_m3`` (fluid.py:2860-2862): float32 product (wx * wy) * wz."""
+    wx, wy, wz = (np.asarray(w, dtype=F32) for w in widths)
+    return (wx[:, None, None] * wy[None, :, None]) * wz[None, None, :]
+
+
+def _coupling_candidates(obstacle, *, slot_active=None, slot_neighbor=None, slot_coef=None, extra_neighbor=None,
+                         extra_coef=None, row_owner=None, row_neighbor=None, row_T=None, row_count=0):
+    """Candidate (owner, neighbour, T) triples in the reference's iteration order: the row
+    list when row_count > 0, else the slots of every owner with obstacle == 0
+    (fluid.py:4346-4451)."""
+    shape = obstacle.shape
+    n = int(np.prod(shape))
+    out = []
+    if int(row_count) > 0:
+        limit = min(int(row_count), n)  # row capacity = cell count (fluid.py:796)
+        for r in range(limit):
+            out.append((tuple(int(v) for v in row_owner[r]), tuple(int(v) for v in row_neighbor[r]),
+                        float(row_T[r]), True))
+        return out
+    if slot_active is None:
+        return out
+    for idx in zip(*np.nonzero((np.asarray(slot_active) > 0) & (obstacle == 0))):
+        cnt = min(int(slot_active[idx]), COUPLING_SLOTS)
+        for s in range(cnt):
+            if s == 0:
+                nb, t = slot_neighbor[idx], slot_coef[idx]
+            else:
+                nb, t = extra_neighbor[idx + (s - 1,)], extra_coef[idx + (s - 1,)]
+            out.append((tuple(int(v) for v in idx), tuple(int(v) for v in nb), float(t), False))
+    return out
+
+
+def _in_range(c, shape):
+    return all(0 <= c[d] < shape[d] for d in range(3))
+
+
+def interface_couplings(obstacle, widths, **fields) -> Couplings:
+    """The couplings the operator applies (fluid.py:4346-4456): a row counts when both
+    cells are in range and have obstacle == 0 and T > 0 (a slot: its neighbour in range
+    with obstacle == 0 and T > 0; the owner already has obstacle == 0).  ca = T / max(V_a,
+    1e-30) with V the float32 cell volume; an owner equal to its neighbour couples to itself."""
+    obst = np.asarray(obstacle)
+    shape = obst.shape
+    V = cell_volumes_f32(widths)
+    a, b, ca, cb = [], [], [], []
+    for own, nb, t, is_row in _coupling_candidates(obst, **fields):
+        if is_row and not _in_range(own, shape):
+            continue
+        if not _in_range(nb, shape) or obst[nb] != 0 or (is_row and obst[own] != 0) or not t > 0.0:
+            continue
+        va = max(V[own], F32(1.0e-30))
+        vb = max(V[nb], F32(1.0e-30))
+        a.append(np.ravel_multi_index(own, shape))
+        b.append(np.ravel_multi_index(nb, shape))
+        ca.append(t / float(va))
+        cb.append(t / float(vb))
+    return Couplings(np.asarray(a, np.int64), np.asarray(b, np.int64), np.asarray(ca, np.float64),
+                     np.asarray(cb, np.float64))
+
+
+def coupling_apply(p: np.ndarray, c: Couplings) -> np.ndarray:
+    """C p: the interface off-diagonal part of the operator."""
+    flat = np.asarray(p, np.float64).ravel()
+    out = np.zeros_like(flat)
+    np.add.at(out, c.a, -c.ca * flat[c.b])
+    np.add.at(out, c.b, -c.cb * flat[c.a])
+    return out.reshape(np.shape(p))
+
+
+def interface_terms_report(obstacle, widths, diag, rhs, **fields) -> dict:
+    """``_pressure_interface_matrix_terms_report_kernel`` (fluid.py:2864-3078).
+
+    Its row validity also requires owner != neighbour; the row diagonal density is
+    the sum of T / V over the valid rows at each of their two cells.
+    """
+    obst = np.asarray(obstacle)
+    shape = obst.shape
+    n = int(np.prod(shape))
+    V = cell_volumes_f32(widths)
+    row_count = int(fields.get("row_count", 0) or 0)
+    dens = np.zeros(shape, np.float64)
+    active = invalid = 0
+    row_diag = 0.0
+    tmax = 0.0
+    for own, nb, t, is_row in _coupling_candidates(obst, **fields):
+        ok = (_in_range(nb, shape) and obst[nb] == 0 and t > 0.0 and own != nb
+              and (not is_row or (_in_range(own, shape) and obst[own] == 0)))
+        if not ok:
+            invalid += 1
+            continue
+        dens[own] += t / float(max(V[own], F32(1.0e-30)))
+        dens[nb] += t / float(max(V[nb], F32(1.0e-30)))
+        active += 1
+        row_diag += 2.0 * t
+        tmax = max(tmax, t)
+    d = np.asarray(diag, np.float64)
+    r = np.asarray(rhs, np.float64)
+    Vd = V.astype(np.float64)
+    mismatch = d - dens
+    return {
+        "diagonal_integral": float((d * Vd).sum()), "rhs_integral": float((r * Vd).sum()),
+        "max_abs_diagonal": float(np.abs(d).max()), "active_cells": int(((np.abs(d) > 0) | (np.abs(r) > 0)).sum()),
+        "row_count": row_count, "row_active_count": active, "row_invalid_count": invalid,
+        "row_overflow_count": max(row_count - n, 0), "row_diagonal_integral": row_diag,
+        "row_diagonal_integral_abs_mismatch": float((np.abs(mismatch) * Vd).sum()),
+        "row_diagonal_max_abs_density_mismatch": float(np.abs(mismatch).max()),
+        "row_max_transmissibility": tmax,
+    }
+
+
+def interface_projection_divergence(pressure, obstacle, diag, rhs, couplings: Couplings,
+                                    dt_over_rho: float) -> np.ndarray:
+    """``_update_pressure_interface_projection_divergence_kernel`` (fluid.py:5625-5693):
+    dt/rho (D p - rhs) on cells with obstacle != 1, zero on obstacle cells, plus
+    dt/rho (C p) from the couplings."""
+    k = float(np.float32(dt_over_rho))
+    p = np.asarray(pressure, np.float64)
+    out = np.where(np.asarray(obstacle) == 1, 0.0, k * (np.asarray(diag, np.float64) * p - np.asarray(rhs, np.float64)))
+    return out + k * coupling_apply(p, couplings)
