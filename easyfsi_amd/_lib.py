@@ -33,6 +33,14 @@ BREAKDOWN_MESSAGES = {
     1: "initial residual/preconditioner dot is not positive finite",
     2: "device-side CG scalar update failed",
     3: "restarted residual/preconditioner dot is not positive finite",
+    4: "BiCGSTAB shadow residual dot is not finite",
+    5: "BiCGSTAB recurrence scalar is not finite",
+    6: "BiCGSTAB beta is not finite",
+    7: "BiCGSTAB alpha denominator is not finite",
+    8: "BiCGSTAB alpha is not finite",
+    9: "BiCGSTAB omega denominator is not finite",
+    10: "BiCGSTAB omega is not finite",
+    11: "BiCGSTAB fallback did not converge",
 }
 
 
@@ -138,6 +146,7 @@ SIGNATURES = {
     "hfsi_fluid_divergence_report": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P]),
     "hfsi_fluid_set_interface_couplings": (ctypes.c_int, [_P, _P, ctypes.POINTER(InterfaceFields), _P, _P, _P]),
     "hfsi_fluid_interface_apply": (ctypes.c_int, [_P, _P, _P, _D, _P]),
+    "hfsi_fluid_solve_bicgstab": (ctypes.c_int, [_P, _P, _P, _I, _I, _D, _I, _D, _I, ctypes.POINTER(CgStats), _P]),
     "hfsi_fluid_predict": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(AxisMetrics), _F, _F, _I, _P]),
     "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
     "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),

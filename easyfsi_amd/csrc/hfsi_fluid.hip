@@ -1233,6 +1233,9 @@ struct hfsi_fluid {
   hfsi::CoupleList cpl{};
   uint32_t cpl_n = 0, cpl_cap = 0;
   hfsi::CoupleStats* cpl_stats = nullptr;
+  // BiCGSTAB fallback: shadow residual and s vectors (float64, allocated on first use, freed with allocs)
+  double* bi_rold = nullptr;
+  double* bi_s = nullptr;
 };
 
 namespace hfsi {
@@ -3134,6 +3137,277 @@ int hfsi_fluid_interface_apply(hfsi_fluid* h, const double* p, double* out, doub
     return HFSI_ERR_ARG;
   }
   return launch_couple<double, double>(h, p, out, scale, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB fallback (_continue_pressure_poisson_fv_bicgstab, fluid.py:6264-6533)
+// ---------------------------------------------------------------------------
+
+namespace hfsi {
+
+struct OpBiResidual {  // r = b - A x (obstacle == 1: 0), the axpby of fluid.py:4519-4531
+  double* r;
+  const double* b;
+  const double* ax;
+  __device__ void operator()(uint32_t c, bool, uint8_t f) const { r[c] = (f & kObst) ? 0.0 : b[c] - ax[c]; }
+};
+struct OpBiReset {  // fluid.py:4614-4628
+  const double* r;
+  double *rold, *d, *ad, *s, *t;
+  __device__ void operator()(uint32_t c, bool, uint8_t f) const {
+    rold[c] = (f & kObst) ? 0.0 : r[c];
+    d[c] = 0.0;
+    ad[c] = 0.0;
+    s[c] = 0.0;
+    t[c] = 0.0;
+  }
+};
+struct OpBiDirection {  // fluid.py:4630-4638
+  double* d;
+  const double *r, *ad;
+  double beta, omega;
+  __device__ void operator()(uint32_t c, bool, uint8_t f) const {
+    d[c] = (f & kObst) ? 0.0 : r[c] + beta * (d[c] - omega * ad[c]);
+  }
+};
+struct OpBiAlpha {  // fluid.py:4640-4650
+  double* x;
+  const double *z, *r, *ad;
+  double* s;
+  double alpha;
+  __device__ void operator()(uint32_t c, bool, uint8_t f) const {
+    if (f & kObst) {
+      x[c] = 0.0;
+      s[c] = 0.0;
+    } else {
+      x[c] = x[c] + alpha * z[c];
+      s[c] = r[c] - alpha * ad[c];
+    }
+  }
+};
+struct OpBiOmega {  // fluid.py:4652-4661
+  double* x;
+  const double *z, *s, *t;
+  double* r;
+  double omega;
+  __device__ void operator()(uint32_t c, bool, uint8_t f) const {
+    if (f & kObst) {
+      x[c] = 0.0;
+      r[c] = 0.0;
+    } else {
+      x[c] = x[c] + omega * z[c];
+      r[c] = s[c] - omega * t[c];
+    }
+  }
+};
+struct OpSubConstant {  // field - mean on free cells, 0 elsewhere (fluid.py:4716-4722)
+  double* f;
+  double mean;
+  __device__ void operator()(uint32_t c, bool free, uint8_t) const { f[c] = free ? f[c] - mean : 0.0; }
+};
+struct OpCopy64 {
+  double* dst;
+  const double* src;
+  __device__ void operator()(uint32_t c, bool, uint8_t) const { dst[c] = src[c]; }
+};
+struct OpSumVol {
+  const double* f;
+  __device__ void operator()(uint32_t c, double V, double (&acc)[2]) const {
+    acc[0] += f[c] * V;
+    acc[1] += V;
+  }
+};
+
+}  // namespace hfsi
+
+int hfsi_fluid_solve_bicgstab(hfsi_fluid* h, const double* rhs, double* x, int32_t start_iteration,
+                              int32_t max_iterations, double b_norm, int32_t outlet_zmin, double tolerance,
+                              int32_t remove_mean, hfsi_cg_stats* S, void* stream) {
+  if (!h || !x || !S || !h->r64) {
+    set_error("hfsi_fluid_solve_bicgstab: invalid arguments (the handle needs the float64 CG vectors)");
+    return HFSI_ERR_ARG;
+  }
+  const double* b = rhs ? rhs : h->b;
+  hipStream_t st = (hipStream_t)stream;
+  const LevelMem& m = h->lv[0];
+  if (!h->bi_rold) HFSI_TRY(dalloc(h, &h->bi_rold, m.n));
+  if (!h->bi_s) HFSI_TRY(dalloc(h, &h->bi_s, m.n));
+  const int outlet = outlet_zmin ? 1 : 0;
+  OpGeo<double> g = geo<double>(h, 0, outlet);
+  double *r = h->r64, *d = h->d64, *ad = h->q64, *z = h->z64, *t = h->t64, *rold = h->bi_rold, *sv = h->bi_s;
+  const int iters = std::max(1, (int)max_iterations);
+  const double tol = std::max(0.0, tolerance);
+  const double floor = 1.0e-300;
+  S->breakdown_kind = HFSI_BREAKDOWN_NONE;
+  S->converged = 0;
+
+  auto wdot = [&](const double* a, const double* c2, double* out) -> int {
+    HFSI_TRY(launch_sums<1>(h, g, OpDot<double, double>{a, c2}, FinStore{}, st));
+    HFSI_CHECK(hipMemcpyAsync(h->host_totals, h->totals, sizeof(double), hipMemcpyDeviceToHost, st));
+    HFSI_CHECK(hipStreamSynchronize(st));
+    *out = h->host_totals[0];
+    return HFSI_OK;
+  };
+  auto project = [&](double* f) -> int {
+    if (!remove_mean) return HFSI_OK;
+    HFSI_TRY(launch_sums<2>(h, g, OpSumVol{f}, FinStore{}, st));
+    HFSI_CHECK(hipMemcpyAsync(h->host_totals, h->totals, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HFSI_CHECK(hipStreamSynchronize(st));
+    const double mean = h->host_totals[1] > 0.0 ? h->host_totals[0] / h->host_totals[1] : 0.0;
+    HFSI_TRY(launch_pointwise(h, 0, OpSubConstant{f, mean}, st));
+    S->mean_projection_count += 1;
+    return HFSI_OK;
+  };
+  auto apply = [&](const double* p, double* q) -> int {
+    hipLaunchKernelGGL((k_apply<double, double, double>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g, p, q);
+    HFSI_LAUNCH_CHECK();
+    return launch_couple<double, double>(h, p, q, 1.0, st);
+  };
+  auto jacobi = [&](const double* src, double* dst) -> int {
+    hipLaunchKernelGGL((k_jacobi_precond<double, double, double>), dim3(blocks_for(g.n)), dim3(kBlock), 0, st, g,
+                       src, dst);
+    HFSI_LAUNCH_CHECK();
+    return HFSI_OK;
+  };
+  auto residual = [&]() -> int {  // r = b - A x, projected
+    HFSI_TRY(apply(x, ad));
+    HFSI_TRY(launch_pointwise(h, 0, OpBiResidual{r, b, ad}, st));
+    return project(r);
+  };
+  auto reset = [&]() -> int { return launch_pointwise(h, 0, OpBiReset{r, rold, d, ad, sv, t}, st); };
+  auto norm_check = [&](const double* v, int it, double& rel) -> int {
+    double vv = 0.0;
+    HFSI_TRY(wdot(v, v, &vv));
+    S->host_residual_checks += 1;
+    rel = std::sqrt(std::max(vv, 0.0)) / b_norm;
+    if (it >= 0) S->iterations = start_iteration + it;
+    S->relative_residual = rel;
+    return HFSI_OK;
+  };
+
+  double rel = 0.0;
+  HFSI_TRY(residual());
+  HFSI_TRY(norm_check(r, -1, rel));
+  if (rel <= tol) {
+    S->converged = 1;
+    return HFSI_OK;
+  }
+  HFSI_TRY(reset());
+  double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+  // a failed scalar restarts from the true residual (or from s for the omega scalars), or ends
+  // the solve with its message on the last iteration
+  auto restart_from_x = [&]() -> int {
+    HFSI_TRY(residual());
+    HFSI_TRY(reset());
+    rho_prev = alpha = omega = 1.0;
+    S->restart_count += 1;
+    return HFSI_OK;
+  };
+  for (int it = 1; it <= iters; ++it) {
+    double rho = 0.0;
+    HFSI_TRY(wdot(rold, r, &rho));
+    if (!std::isfinite(rho) || std::fabs(rho) <= floor) {
+      if (it >= iters) {
+        S->breakdown_kind = HFSI_BREAKDOWN_BICGSTAB_SHADOW;
+        return HFSI_OK;
+      }
+      HFSI_TRY(residual());
+      HFSI_TRY(norm_check(r, it, rel));
+      if (rel <= tol) {
+        S->converged = 1;
+        return HFSI_OK;
+      }
+      HFSI_TRY(reset());
+      rho_prev = alpha = omega = 1.0;
+      S->restart_count += 1;
+      continue;
+    }
+    double beta = 0.0;
+    if (it > 1) {
+      if (!std::isfinite(rho_prev) || std::fabs(rho_prev) <= floor || !std::isfinite(omega) ||
+          std::fabs(omega) <= floor) {
+        if (it >= iters) {
+          S->breakdown_kind = HFSI_BREAKDOWN_BICGSTAB_RECURRENCE;
+          return HFSI_OK;
+        }
+        HFSI_TRY(restart_from_x());
+        continue;
+      }
+      beta = (rho / rho_prev) * (alpha / omega);
+      if (!std::isfinite(beta)) {
+        if (it >= iters) {
+          S->breakdown_kind = HFSI_BREAKDOWN_BICGSTAB_BETA;
+          return HFSI_OK;
+        }
+        HFSI_TRY(restart_from_x());
+        continue;
+      }
+    }
+    HFSI_TRY(launch_pointwise(h, 0, OpBiDirection{d, r, ad, beta, omega}, st));
+    HFSI_TRY(jacobi(d, z));
+    HFSI_TRY(project(z));
+    HFSI_TRY(apply(z, ad));
+    double den = 0.0;
+    HFSI_TRY(wdot(rold, ad, &den));
+    if (!std::isfinite(den) || std::fabs(den) <= floor) {
+      if (it >= iters) {
+        S->breakdown_kind = HFSI_BREAKDOWN_BICGSTAB_ALPHA_DEN;
+        return HFSI_OK;
+      }
+      HFSI_TRY(restart_from_x());
+      continue;
+    }
+    alpha = rho / den;
+    if (!std::isfinite(alpha)) {
+      if (it >= iters) {
+        S->breakdown_kind = HFSI_BREAKDOWN_BICGSTAB_ALPHA;
+        return HFSI_OK;
+      }
+      HFSI_TRY(restart_from_x());
+      continue;
+    }
+    HFSI_TRY(launch_pointwise(h, 0, OpBiAlpha{x, z, r, ad, sv, alpha}, st));
+    HFSI_TRY(project(sv));
+    HFSI_TRY(norm_check(sv, it, rel));
+    if (rel <= tol) {
+      HFSI_TRY(launch_pointwise(h, 0, OpCopy64{r, sv}, st));
+      S->converged = 1;
+      return HFSI_OK;
+    }
+    HFSI_TRY(jacobi(sv, z));
+    HFSI_TRY(project(z));
+    HFSI_TRY(apply(z, t));
+    double ts = 0.0, tt = 0.0;
+    HFSI_TRY(wdot(t, sv, &ts));
+    HFSI_TRY(wdot(t, t, &tt));
+    const bool tt_bad = !std::isfinite(tt) || std::fabs(tt) <= floor;
+    if (!tt_bad) omega = ts / tt;
+    if (tt_bad || !std::isfinite(omega) || std::fabs(omega) <= floor) {
+      if (it >= iters) {
+        S->breakdown_kind = tt_bad ? HFSI_BREAKDOWN_BICGSTAB_OMEGA_DEN : HFSI_BREAKDOWN_BICGSTAB_OMEGA;
+        return HFSI_OK;
+      }
+      HFSI_TRY(launch_pointwise(h, 0, OpCopy64{r, sv}, st));
+      HFSI_TRY(project(r));
+      HFSI_TRY(reset());
+      rho_prev = alpha = omega = 1.0;
+      S->restart_count += 1;
+      continue;
+    }
+    HFSI_TRY(launch_pointwise(h, 0, OpBiOmega{x, z, sv, t, r, omega}, st));
+    if (it % 16 == 0) {
+      HFSI_TRY(project(x));
+      HFSI_TRY(project(r));
+    }
+    HFSI_TRY(norm_check(r, it, rel));
+    if (rel <= tol) {
+      S->converged = 1;
+      return HFSI_OK;
+    }
+    rho_prev = rho;
+  }
+  S->breakdown_kind = HFSI_BREAKDOWN_BICGSTAB_NOT_CONVERGED;
+  return HFSI_OK;
 }
 
 int hfsi_fluid_diagonal(hfsi_fluid* h, double* diag, int32_t outlet_zmin, void* stream) {

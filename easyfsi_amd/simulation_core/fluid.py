@@ -10,9 +10,10 @@ libhfsi (include/hfsi.h).
 The velocity predictor (``predict``, fluid.py:2447-2527) and the
 sharp-interface couplings of the operator (row list and per-owner slots,
 fluid.py:4346-4456, with their report and projection divergence) run on
-libhfsi too.  Out of this build (raise ``NotImplementedError``): the BiCGSTAB
-fallback of row-list solves (fluid.py:6264-6533), velocity constraints and
-the local divergence cleanup.
+libhfsi too, as does the BiCGSTAB fallback of row-list solves
+(fluid.py:6264-6533).  Out of this build (raise ``NotImplementedError``):
+HIBM's unreached-set anchoring, velocity constraints and the local divergence
+cleanup.
 """
 
 from __future__ import annotations
@@ -207,6 +208,7 @@ class CartesianFluidSolver:
         "pressure_interface_coupling_extra_neighbor": ((_lib.INTERFACE_SLOTS - 1,), torch.int32, (3,)),
         "pressure_interface_coupling_extra_coefficient": ((_lib.INTERFACE_SLOTS - 1,), torch.float64, ()),
         "pressure_interface_row_diagonal_density": ((), torch.float64, ()),
+        "cg_rhs": ((), torch.float64, ()),  # right-hand side of a direct BiCGSTAB call
     }
     _INTERFACE_ROW_FIELDS = {
         "pressure_interface_row_owner": (torch.int32, (3,)),
@@ -556,6 +558,58 @@ class CartesianFluidSolver:
         self.last_cg_true_relative_residual = float(stats.true_relative_residual)
         self.last_cg_used_multigrid = bool(stats.used_multigrid)
         self.last_cg_stats = stats.as_dict()
+        # an unconverged row-list solve continues with BiCGSTAB (fluid.py:6799-6811)
+        if (not stats.converged and int(stats.breakdown_kind) == 0 and pressure_interface_matrix_active
+                and int(self.pressure_interface_row_count.tensor.item()) > 0):
+            self._continue_pressure_poisson_fv_bicgstab(
+                start_iteration=opt.max_iterations, max_iters=opt.max_iterations, b_norm=float(stats.b_norm),
+                outlet=opt.outlet_zmin, tolerance=opt.tolerance, anchor_unreached=False,
+                remove_nullspace_mean=bool(remove_nullspace_mean), _stats=stats)
+
+    def _continue_pressure_poisson_fv_bicgstab(self, *, start_iteration: int, max_iters: int, b_norm: float,
+                                               outlet: int, tolerance: float, anchor_unreached: bool,
+                                               remove_nullspace_mean: bool, _stats=None) -> None:
+        """Jacobi-preconditioned BiCGSTAB fallback (fluid.py:6264-6533) on libhfsi.
+
+        Called by the CG solve it continues (then on that solve's right-hand side) or
+        directly on ``cg_rhs``."""
+        if anchor_unreached:
+            raise NotImplementedError("unreached-set anchoring (HIBM) is not in this build")
+        stats = _stats if _stats is not None else _lib.CgStats()
+        if _stats is None:
+            stats.iterations = int(self.last_cg_iterations)
+            stats.host_residual_checks = int(self.last_cg_host_residual_checks)
+            stats.mean_projection_count = int(self.last_cg_mean_projection_count)
+            stats.restart_count = int(getattr(self, "last_cg_restart_count", 0))
+            self._set_masks()
+        rhs_ptr = None if _stats is not None else self.cg_rhs.ptr
+        _lib.check(self._lib.hfsi_fluid_solve_bicgstab(
+            self._handle, rhs_ptr, self.pressure.ptr, int(start_iteration), max(1, int(max_iters)), float(b_norm),
+            1 if outlet else 0, max(0.0, float(tolerance)), 1 if remove_nullspace_mean else 0, ctypes.byref(stats),
+            self._stream), "hfsi_fluid_solve_bicgstab")
+        self.last_cg_restart_policy = "periodic_exact_residual+bicgstab_fallback"
+        self.last_cg_iterations = int(stats.iterations)
+        self.last_cg_relative_residual = float(stats.relative_residual)
+        self.last_cg_converged = bool(stats.converged)
+        self.last_cg_breakdown = _lib.BREAKDOWN_MESSAGES.get(int(stats.breakdown_kind), "")
+        self.last_cg_host_residual_checks = int(stats.host_residual_checks)
+        self.last_cg_mean_projection_count = int(stats.mean_projection_count)
+        self.last_cg_restart_count = int(stats.restart_count)
+        self.last_cg_stats = stats.as_dict()
+
+    def _fv_diagonal_kernel(self, output: Field, pressure_outlet_zmin: int = 0) -> None:
+        """fluid.py:4280-4309 (diagonal of the operator, interface diagonal included)."""
+        self._set_masks()
+        _lib.check(self._lib.hfsi_fluid_diagonal(self._handle, output.ptr, 1 if pressure_outlet_zmin else 0,
+                                                 self._stream), "hfsi_fluid_diagonal")
+
+    def _weighted_dot_kernel(self, a: Field, b: Field) -> float:
+        """fluid.py:4533-4549: sum over cells with obstacle == 0 of V a b (float64)."""
+        out = ctypes.c_double()
+        _lib.check(self._lib.hfsi_fluid_weighted_dot(self._handle, a.tensor.double().contiguous().data_ptr(),
+                                                     b.tensor.double().contiguous().data_ptr(), ctypes.byref(out),
+                                                     self._stream), "hfsi_fluid_weighted_dot")
+        return float(out.value)
 
     # ------------------------------------------------------------------
     # projection

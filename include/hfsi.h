@@ -49,6 +49,14 @@ extern "C" {
 #define HFSI_BREAKDOWN_INITIAL_DOT 1   /* "initial residual/preconditioner dot is not positive finite"   */
 #define HFSI_BREAKDOWN_DEVICE_SCALAR 2 /* "device-side CG scalar update failed"                          */
 #define HFSI_BREAKDOWN_RESTART_DOT 3   /* "restarted residual/preconditioner dot is not positive finite" */
+#define HFSI_BREAKDOWN_BICGSTAB_SHADOW 4        /* "BiCGSTAB shadow residual dot is not finite" */
+#define HFSI_BREAKDOWN_BICGSTAB_RECURRENCE 5    /* "BiCGSTAB recurrence scalar is not finite"   */
+#define HFSI_BREAKDOWN_BICGSTAB_BETA 6          /* "BiCGSTAB beta is not finite"                */
+#define HFSI_BREAKDOWN_BICGSTAB_ALPHA_DEN 7     /* "BiCGSTAB alpha denominator is not finite"   */
+#define HFSI_BREAKDOWN_BICGSTAB_ALPHA 8         /* "BiCGSTAB alpha is not finite"               */
+#define HFSI_BREAKDOWN_BICGSTAB_OMEGA_DEN 9     /* "BiCGSTAB omega denominator is not finite"   */
+#define HFSI_BREAKDOWN_BICGSTAB_OMEGA 10        /* "BiCGSTAB omega is not finite"               */
+#define HFSI_BREAKDOWN_BICGSTAB_NOT_CONVERGED 11 /* "BiCGSTAB fallback did not converge"        */
 
 typedef struct hfsi_fluid hfsi_fluid;
 
@@ -355,6 +363,18 @@ int hfsi_fluid_set_interface_couplings(hfsi_fluid* h, const int32_t* obstacle, c
  * _update_pressure_interface_projection_divergence_kernel (fluid.py:5625-5693) with
  * scale = dt/rho.  p and out are [n] float64. */
 int hfsi_fluid_interface_apply(hfsi_fluid* h, const double* p, double* out, double scale, void* stream);
+
+/* Jacobi-preconditioned BiCGSTAB continuing from `pressure` on A p = rhs
+ * (_continue_pressure_poisson_fv_bicgstab, fluid.py:6264-6533): the fallback after a
+ * PCG solve with interface rows stops unconverged.  rhs is the float64 right-hand side
+ * (NULL: that of the last hfsi_fluid_solve_cg on this handle), b_norm its V-norm; the
+ * operator includes the interface couplings.  stats is updated in place: iterations
+ * (start_iteration + the BiCGSTAB iteration of the last check), relative_residual,
+ * converged, breakdown_kind (HFSI_BREAKDOWN_BICGSTAB_*), and the restart, residual
+ * check and mean projection counters are incremented.  Synchronises at every scalar. */
+int hfsi_fluid_solve_bicgstab(hfsi_fluid* h, const double* rhs, double* pressure, int32_t start_iteration,
+                              int32_t max_iterations, double b_norm, int32_t outlet_zmin, double tolerance,
+                              int32_t remove_mean, hfsi_cg_stats* stats, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Velocity predictor

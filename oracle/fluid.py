@@ -932,3 +932,142 @@ def interface_projection_divergence(pressure, obstacle, diag, rhs, couplings: Co
     p = np.asarray(pressure, np.float64)
     out = np.where(np.asarray(obstacle) == 1, 0.0, k * (np.asarray(diag, np.float64) * p - np.asarray(rhs, np.float64)))
     return out + k * coupling_apply(p, couplings)
+
+
+def bicgstab(rhs, x0, lvl: FvLevel, *, outlet: bool, b_norm: float, tolerance: float, max_iters: int,
+             start_iteration: int = 0, remove_mean: bool = False, couplings: Couplings | None = None) -> dict:
+    """Jacobi-preconditioned BiCGSTAB fallback (``_continue_pressure_poisson_fv_bicgstab``,
+    fluid.py:6264-6533) with its restarts, breakdown messages and counters.
+
+    Vectors are zero on obstacle rows (obstacle == 1); the Jacobi preconditioner is r /
+    diag on rows with a positive diagonal; the operator includes the couplings.
+    """
+    diag = fv_diagonal(lvl, outlet)
+    obstacle_row = np.asarray(lvl.obstacle) == 1
+
+    def A(v):
+        q = _fv_apply_plain(v, lvl, outlet)
+        return q + coupling_apply(v, couplings) if couplings is not None and len(couplings.a) else q
+
+    def jac(v):
+        return np.where(~obstacle_row & (diag > 0.0), v / np.where(diag > 0.0, diag, 1.0), 0.0)
+
+    st = {"iterations": int(start_iteration), "converged": False, "relative_residual": math.inf, "breakdown": "",
+          "restart_count": 0, "host_residual_checks": 0, "mean_projection_count": 0}
+
+    def project(v):
+        if not remove_mean:
+            return v
+        st["mean_projection_count"] += 1
+        return subtract_weighted_mean(v, lvl)
+
+    def norm_check(v, it):
+        st["host_residual_checks"] += 1
+        rel = math.sqrt(max(weighted_dot(v, v, lvl), 0.0)) / b_norm
+        if it is not None:
+            st["iterations"] = int(start_iteration) + it
+        st["relative_residual"] = rel
+        return rel
+
+    zero = np.zeros_like(np.asarray(x0, np.float64))
+    x = np.asarray(x0, np.float64).copy()
+    b = np.asarray(rhs, np.float64)
+    tol = max(0.0, float(tolerance))
+    floor = 1.0e-300
+    iters = max(1, int(max_iters))
+
+    def residual():
+        return project(np.where(obstacle_row, 0.0, b - A(x)))
+
+    r = residual()
+    if norm_check(r, None) <= tol:
+        st["converged"] = True
+        return {"pressure": x, **st}
+    rold, d, ad, s, t = np.where(obstacle_row, 0.0, r), zero.copy(), zero.copy(), zero.copy(), zero.copy()
+    rho_prev = alpha = omega = 1.0
+
+    def restart():
+        nonlocal r, rold, d, ad, s, t, rho_prev, alpha, omega
+        r = residual()
+        rold, d, ad, s, t = np.where(obstacle_row, 0.0, r), zero.copy(), zero.copy(), zero.copy(), zero.copy()
+        rho_prev = alpha = omega = 1.0
+        st["restart_count"] += 1
+
+    for it in range(1, iters + 1):
+        rho = weighted_dot(rold, r, lvl)
+        if not math.isfinite(rho) or abs(rho) <= floor:
+            if it >= iters:
+                st["breakdown"] = "BiCGSTAB shadow residual dot is not finite"
+                return {"pressure": x, **st}
+            r = residual()
+            if norm_check(r, it) <= tol:
+                st["converged"] = True
+                return {"pressure": x, **st}
+            rold, d, ad, s, t = np.where(obstacle_row, 0.0, r), zero.copy(), zero.copy(), zero.copy(), zero.copy()
+            rho_prev = alpha = omega = 1.0
+            st["restart_count"] += 1
+            continue
+        beta = 0.0
+        if it > 1:
+            if not (math.isfinite(rho_prev) and abs(rho_prev) > floor and math.isfinite(omega) and abs(omega) > floor):
+                if it >= iters:
+                    st["breakdown"] = "BiCGSTAB recurrence scalar is not finite"
+                    return {"pressure": x, **st}
+                restart()
+                continue
+            beta = (rho / rho_prev) * (alpha / omega)
+            if not math.isfinite(beta):
+                if it >= iters:
+                    st["breakdown"] = "BiCGSTAB beta is not finite"
+                    return {"pressure": x, **st}
+                restart()
+                continue
+        d = np.where(obstacle_row, 0.0, r + beta * (d - omega * ad))
+        z = project(jac(d))
+        ad = A(z)
+        den = weighted_dot(rold, ad, lvl)
+        if not math.isfinite(den) or abs(den) <= floor:
+            if it >= iters:
+                st["breakdown"] = "BiCGSTAB alpha denominator is not finite"
+                return {"pressure": x, **st}
+            restart()
+            continue
+        alpha = rho / den
+        if not math.isfinite(alpha):
+            if it >= iters:
+                st["breakdown"] = "BiCGSTAB alpha is not finite"
+                return {"pressure": x, **st}
+            restart()
+            continue
+        x = np.where(obstacle_row, 0.0, x + alpha * z)
+        s = project(np.where(obstacle_row, 0.0, r - alpha * ad))
+        if norm_check(s, it) <= tol:
+            st["converged"] = True
+            return {"pressure": x, **st}
+        z = project(jac(s))
+        t = A(z)
+        ts, tt = weighted_dot(t, s, lvl), weighted_dot(t, t, lvl)
+        tt_bad = not math.isfinite(tt) or abs(tt) <= floor
+        if not tt_bad:
+            omega = ts / tt
+        if tt_bad or not math.isfinite(omega) or abs(omega) <= floor:
+            if it >= iters:
+                st["breakdown"] = ("BiCGSTAB omega denominator is not finite" if tt_bad
+                                   else "BiCGSTAB omega is not finite")
+                return {"pressure": x, **st}
+            r = project(s.copy())
+            rold, d, ad, s, t = np.where(obstacle_row, 0.0, r), zero.copy(), zero.copy(), zero.copy(), zero.copy()
+            rho_prev = alpha = omega = 1.0
+            st["restart_count"] += 1
+            continue
+        x = np.where(obstacle_row, 0.0, x + omega * z)
+        r = np.where(obstacle_row, 0.0, s - omega * t)
+        if it % 16 == 0:
+            x = project(x)
+            r = project(r)
+        if norm_check(r, it) <= tol:
+            st["converged"] = True
+            return {"pressure": x, **st}
+        rho_prev = rho
+    st["breakdown"] = "BiCGSTAB fallback did not converge"
+    return {"pressure": x, **st}

@@ -322,3 +322,81 @@ def test_project_with_row_couplings_reports_interface_rows():
     want = F.interface_projection_divergence(s.pressure.to_numpy(), obst, diag, np.zeros(shape), c, 1e-3 / 1.0)
     got = s.pressure_interface_projection_divergence_s.to_numpy()
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(np.abs(want).max(), 1e-30))
+
+
+# ---------------------------------------------------------------------------
+# BiCGSTAB fallback (fluid.py:6264-6533)
+# ---------------------------------------------------------------------------
+
+def test_reference_bicgstab_fallback_solves_generic_fv_rhs_on_device():  # test_core_fluid.py:3713-3736
+    s = _unit_solver(4)
+    rhs = np.zeros((4, 4, 4))
+    rhs[2, 2, 2] = 1.0
+    s.cg_rhs.from_numpy(rhs)
+    s._fv_diagonal_kernel(s.fv_diag, 1)
+    b_norm = float(np.sqrt(max(s._weighted_dot_kernel(s.cg_rhs, s.cg_rhs), 0.0)))
+    s._continue_pressure_poisson_fv_bicgstab(start_iteration=0, max_iters=80, b_norm=b_norm, outlet=1,
+                                             tolerance=1.0e-8, anchor_unreached=False, remove_nullspace_mean=False)
+    assert s.last_cg_converged, s.last_cg_breakdown
+    assert s.last_cg_relative_residual <= 1.0e-8
+    assert "bicgstab_fallback" in s.last_cg_restart_policy
+
+
+def test_bicgstab_matches_oracle_with_couplings():
+    _say("bicgstab with row couplings")
+    rng = np.random.default_rng(31)
+    shape = (12, 10, 11)
+    w = [rng.uniform(0.6, 1.4, n) / n for n in shape]
+    obst = (rng.random(shape) < 0.06).astype(np.int32)
+    s = _solver(w)
+    s.obstacle.from_numpy(obst)
+    fields = _set_rows(s, _random_entries(shape, rng, obst, 40))
+    c = F.interface_couplings(obst, w, **fields)
+    dens = np.zeros(shape).ravel()
+    np.add.at(dens, c.a, c.ca)
+    np.add.at(dens, c.b, c.cb)
+    diag = 1.5 * dens.reshape(shape)
+    s.pressure_interface_matrix_diagonal.from_numpy(diag)
+    rhs = np.where(obst == 0, rng.normal(size=shape), 0.0)
+    s.cg_rhs.from_numpy(rhs)
+    s.pressure.fill(0)
+    lvl = F.make_level(w, obst, iface_diag=diag)
+    b_norm = float(np.sqrt(F.weighted_dot(rhs, rhs, lvl)))
+    s._continue_pressure_poisson_fv_bicgstab(start_iteration=5, max_iters=300, b_norm=b_norm, outlet=1,
+                                             tolerance=1e-10, anchor_unreached=False, remove_nullspace_mean=False)
+    ref = F.bicgstab(rhs, np.zeros(shape), lvl, outlet=True, b_norm=b_norm, tolerance=1e-10, max_iters=300,
+                     start_iteration=5, couplings=c)
+    _say(f"  device {s.last_cg_iterations} its, oracle {ref['iterations']} its")
+    assert s.last_cg_converged and ref["converged"]
+    assert abs(s.last_cg_iterations - ref["iterations"]) <= 1
+    np.testing.assert_allclose(s.pressure.to_numpy(), ref["pressure"], rtol=0,
+                               atol=1e-8 * np.abs(ref["pressure"]).max())
+
+
+@pytest.mark.parametrize("rows", [False, True])
+def test_pcg_budget_enters_bicgstab_only_with_rows(rows):  # test_core_fluid.py:3586-3711
+    s = _unit_solver(6)
+    shape = (6, 6, 6)
+    rng = np.random.default_rng(3)
+    obst = np.zeros(shape, np.int32)
+    diag = np.zeros(shape)
+    if rows:
+        fields = _set_rows(s, _random_entries(shape, rng, obst, 10))
+        c = F.interface_couplings(obst, [np.full(6, 1 / 6)] * 3, **fields)
+        dens = np.zeros(shape).ravel()
+        np.add.at(dens, c.a, c.ca)
+        np.add.at(dens, c.b, c.cb)
+        diag = 1.5 * dens.reshape(shape)
+    else:
+        diag[2, 2, 2] = 3.0
+    s.pressure_interface_matrix_diagonal.from_numpy(diag)
+    s._set_masks()
+    s.divergence.from_numpy(rng.normal(size=shape).astype(np.float32))
+    s._solve_pressure_poisson_fv_cg(iterations=2, rhs_scale=1.0, pressure_outlet_zmin=True, tolerance=1.0e-6,
+                                    preconditioner="jacobi", pressure_interface_matrix_active=True)
+    if rows:
+        assert "bicgstab_fallback" in s.last_cg_restart_policy
+        assert s.last_cg_iterations > 2
+    else:
+        assert s.last_cg_restart_policy == "periodic_exact_residual"
+        assert not s.last_cg_converged and s.last_cg_iterations == 2

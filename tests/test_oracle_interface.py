@@ -248,3 +248,64 @@ def test_terms_report_counts_rows_like_the_reference():  # fluid.py:2864-3078
     fs["slot_coef"][0, 0, 0] = T
     assert len(F.interface_couplings(obst, w, **fs, **{k: f[k] for k in f}).a) == 3
     assert len(F.interface_couplings(obst, w, **fs).a) == 1
+
+
+# ---------------------------------------------------------------------------
+# BiCGSTAB fallback (fluid.py:6264-6533)
+# ---------------------------------------------------------------------------
+
+def test_bicgstab_solves_generic_fv_rhs():  # test_core_fluid.py:3713-3736
+    shape = (4, 4, 4)
+    w = _unit(4)
+    obst = np.zeros(shape, np.int32)
+    lvl = _lvl(w, obst)
+    rhs = np.zeros(shape)
+    rhs[2, 2, 2] = 1.0
+    b_norm = float(np.sqrt(max(F.weighted_dot(rhs, rhs, lvl), 0.0)))
+    res = F.bicgstab(rhs, np.zeros(shape), lvl, outlet=True, b_norm=b_norm, tolerance=1e-8, max_iters=80)
+    assert res["converged"], res["breakdown"]
+    assert res["relative_residual"] <= 1e-8
+    # the outlet system is nonsingular: the iterate is spsolve's solution
+    n = 64
+    cols = []
+    for m in range(n):
+        e = np.zeros(n)
+        e[m] = 1.0
+        cols.append(F.fv_apply(e.reshape(shape), lvl, True).ravel())
+    want = spla.spsolve(sp.csr_matrix(np.array(cols).T), rhs.ravel()).reshape(shape)
+    np.testing.assert_allclose(res["pressure"], want, rtol=0, atol=1e-7 * np.abs(want).max())
+
+
+def test_bicgstab_with_couplings_matches_spsolve_and_counts():
+    rng = np.random.default_rng(11)
+    shape = (6, 5, 6)
+    w = [rng.uniform(0.6, 1.4, n) / n for n in shape]
+    obst = (rng.random(shape) < 0.08).astype(np.int32)
+    f = _random_couplings(shape, rng, obst, rows=True, count=15)
+    c = F.interface_couplings(obst, w, **f)
+    dens = np.zeros(shape).ravel()
+    np.add.at(dens, c.a, c.ca)
+    np.add.at(dens, c.b, c.cb)
+    diag = 1.5 * dens.reshape(shape)
+    lvl = _lvl(w, obst, diag)
+    rhs = np.where(obst == 0, rng.normal(size=shape), 0.0)
+    b_norm = float(np.sqrt(F.weighted_dot(rhs, rhs, lvl)))
+    res = F.bicgstab(rhs, np.zeros(shape), lvl, outlet=True, b_norm=b_norm, tolerance=1e-11, max_iters=400,
+                     start_iteration=7, couplings=c)
+    assert res["converged"] and res["iterations"] > 7
+    free = (obst == 0).ravel()
+    idx = np.nonzero(free)[0]
+    cols = []
+    for m in idx:
+        e = np.zeros(int(np.prod(shape)))
+        e[m] = 1.0
+        q = F.fv_apply(e.reshape(shape), lvl, True) + F.coupling_apply(e.reshape(shape), c)
+        cols.append(q.ravel()[idx])
+    want = np.zeros(int(np.prod(shape)))
+    want[idx] = spla.spsolve(sp.csr_matrix(np.array(cols).T), rhs.ravel()[idx])
+    np.testing.assert_allclose(res["pressure"].ravel(), want, rtol=0, atol=1e-8 * np.abs(want).max())
+    # a budget too small to converge ends with the reference's message
+    short = F.bicgstab(rhs, np.zeros(shape), lvl, outlet=True, b_norm=b_norm, tolerance=1e-14, max_iters=3,
+                       couplings=c)
+    assert not short["converged"] and short["breakdown"] == "BiCGSTAB fallback did not converge"
+    assert short["iterations"] == 3 and short["host_residual_checks"] >= 4
