@@ -29,7 +29,8 @@ over plain_fp32 is reported beside it.  Also reported: a time-coherent
 sequence (moving sphere, turning velocity) solved from p = 0, from the
 previous step's pressure and from the learned guess; the neo-Hookean MPM step
 (1M particles); the velocity predictor (Euler and RK2 semi-Lagrangian advection
-plus FV diffusion) on the same grid; the CPU baseline (C + OpenMP restatement of the plain
+plus FV diffusion) on the same grid; the sharp-interface coupling pass and the
+operator with and without couplings; the CPU baseline (C + OpenMP restatement of the plain
 algorithm on the host cores).
 
 Prints one JSON line.  --gpus above 1 is refused: this build measures one GPU.
@@ -523,6 +524,69 @@ def run_predict(solver, steps: int, warmup: int, cpu: bool, cpu_budget: float = 
     return out
 
 
+# Algorithmic bytes per interface coupling of one coupling pass: the pair (8 B), T/V on both
+# sides (16 B), the two pressures read (16 B) and the two read-modify-write atomics (32 B).
+COUPLING_BYTES = 72
+
+
+def run_interface(solver, steps: int):
+    """Sharp-interface row couplings on the benchmark grid: every fluid cell of the shell
+    0.06 < r <= 0.075 around the sphere is coupled to its +z neighbour (T = V/2, float64
+    rows), compiled into the operator; times the coupling pass alone (out += C p) and the
+    operator application with and without the couplings (HIP events on the launch stream)."""
+    import ctypes
+
+    import torch
+
+    from easyfsi_amd import _lib
+
+    nx, ny, nz = solver.nx, solver.ny, solver.nz
+    cx, cy, cz = (solver.cell_center_x_m.tensor.double(), solver.cell_center_y_m.tensor.double(),
+                  solver.cell_center_z_m.tensor.double())
+    r = torch.sqrt((cx[:, None, None] - 0.5) ** 2 + (cy[None, :, None] - 0.5) ** 2 + (cz[None, None, :] - 0.5) ** 2)
+    shell = (r > 0.06) & (r <= 0.075) & (solver.obstacle.tensor == 0)
+    shell[:, :, nz - 1] = False
+    own = torch.nonzero(shell).to(torch.int32)
+    nb = own.clone()
+    nb[:, 2] += 1
+    keep = solver.obstacle.tensor[nb[:, 0].long(), nb[:, 1].long(), nb[:, 2].long()] == 0
+    own, nb = own[keep], nb[keep]
+    rows = int(own.shape[0])
+    vol = ((solver.cell_width_x_m.tensor.double()[own[:, 0].long()] * solver.cell_width_y_m.tensor.double()[
+        own[:, 1].long()]) * solver.cell_width_z_m.tensor.double()[own[:, 2].long()])
+    solver.pressure_interface_row_owner.tensor[:rows] = own
+    solver.pressure_interface_row_neighbor.tensor[:rows] = nb
+    solver.pressure_interface_row_transmissibility.tensor[:rows] = 0.5 * vol
+    solver.pressure_interface_row_count[None] = rows
+    solver._set_masks()
+    L, h, st = solver._lib, solver._handle, solver._stream
+    n = nx * ny * nz
+    p = torch.randn(n, dtype=torch.float64, device="cuda")
+    q = torch.zeros_like(p)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / steps * 1e3
+
+    couple_us = timed(lambda: _lib.check(L.hfsi_fluid_interface_apply(h, p.data_ptr(), q.data_ptr(), 1.0, st)))
+    coupled_us = timed(lambda: _lib.check(L.hfsi_fluid_apply(h, p.data_ptr(), q.data_ptr(), 0, st)))
+    couplings = int(solver._iface_row_report[5])
+    solver.pressure_interface_row_count[None] = 0
+    solver._set_masks()
+    plain_us = timed(lambda: _lib.check(L.hfsi_fluid_apply(h, p.data_ptr(), q.data_ptr(), 0, st)))
+    return {"rows": rows, "couplings": couplings, "coupling_pass_us": round(couple_us, 2),
+            "coupling_pass_GBps": round(COUPLING_BYTES * couplings / couple_us / 1e3, 1),
+            "apply_with_couplings_us": round(coupled_us, 2), "apply_stencil_only_us": round(plain_us, 2),
+            "bytes_per_coupling": COUPLING_BYTES}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -542,6 +606,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mpm-steps", type=int, default=20, help="neo-Hookean MPM steps timed beside (0: skip)")
+    ap.add_argument("--interface-steps", type=int, default=20,
+                    help="interface coupling passes and operator applications timed beside (0: skip)")
     ap.add_argument("--predict-steps", type=int, default=20,
                     help="velocity predictor launches timed beside, per advection scheme (0: skip)")
     args = ap.parse_args(argv)
@@ -625,6 +691,13 @@ def main(argv=None) -> int:
         predict = run_predict(solver, args.predict_steps, 3, not args.no_cpu)
         print(f"[bench] predictor: euler {predict['euler']['kernel_avg_us']:.1f} us, "
               f"rk2 {predict['rk2']['kernel_avg_us']:.1f} us per launch", file=sys.stderr, flush=True)
+    interface = None
+    if args.interface_steps > 0:
+        print(f"[bench] interface couplings: {args.interface_steps} passes", file=sys.stderr, flush=True)
+        interface = run_interface(solver, args.interface_steps)
+        print(f"[bench] interface: {interface['couplings']} couplings, pass {interface['coupling_pass_us']:.1f} us, "
+              f"apply {interface['apply_with_couplings_us']:.1f} vs {interface['apply_stencil_only_us']:.1f} us",
+              file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (C oracle)", file=sys.stderr, flush=True)
@@ -670,6 +743,8 @@ def main(argv=None) -> int:
         line["mpm"] = mpm
     if predict is not None:
         line["predictor"] = predict
+    if interface is not None:
+        line["interface"] = interface
     if sequence is not None:
         line["warm_start_sequence"] = sequence
     if project is not None:
