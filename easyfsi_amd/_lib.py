@@ -116,6 +116,14 @@ class InterfaceFields(ctypes.Structure):
                 ("row_count", ctypes.c_int32)]
 
 
+class MarkerScatterReport(ctypes.Structure):
+    """hfsi_marker_scatter_report."""
+
+    _fields_ = [("active_marker_count", ctypes.c_int32), ("invalid_marker_count", ctypes.c_int32),
+                ("active_particle_count", ctypes.c_int64), ("total_marker_force", ctypes.c_double * 3),
+                ("total_external_force", ctypes.c_double * 3)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 _D = ctypes.c_double
@@ -147,6 +155,8 @@ SIGNATURES = {
     "hfsi_fluid_set_interface_couplings": (ctypes.c_int, [_P, _P, ctypes.POINTER(InterfaceFields), _P, _P, _P]),
     "hfsi_fluid_interface_apply": (ctypes.c_int, [_P, _P, _P, _D, _P]),
     "hfsi_fluid_solve_bicgstab": (ctypes.c_int, [_P, _P, _P, _I, _I, _D, _I, _D, _I, ctypes.POINTER(CgStats), _P]),
+    "hfsi_scatter_marker_forces": (ctypes.c_int, [_P, _P, _I, _P, _P, _I, _F, ctypes.POINTER(MarkerScatterReport),
+                                                  _P]),
     "hfsi_fluid_predict": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(AxisMetrics), _F, _F, _I, _P]),
     "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
     "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),

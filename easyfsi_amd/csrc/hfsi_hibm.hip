@@ -1,0 +1,304 @@
+// Marker -> MPM particle force scatter of the HIBM coupling (gfx950):
+// HibmMpmSurfaceMarkers._scatter_marker_forces_to_mpm_particles_kernel
+// (simulation_core/hibm_mpm.py:4899-4960) with the shape weight of 4838-4848.
+//
+// Every marker with a finite position and force spreads its force over the particles
+// with finite positions, in proportion to the tent weight
+//   w = prod_axis max(1 - |x_m - x_p| / h, 0)          (float32, h = support radius)
+// normalised by the marker's weight sum (kept when > 1e-12); each contribution is cast
+// to float32 and added to the particle's external force.  The reference visits every
+// (marker, particle) pair; only particles within h along every axis have w > 0, so
+// here the particles are binned into cells of edge >= h first (counting sort: count,
+// hipCUB exclusive scan, fill) and each marker visits the 27 cells around its own.  One
+// 64-lane wavefront per marker: the lanes stride over the candidates, the weight sum is
+// a wavefront reduction, then the lanes add their contributions with float32 atomics
+// (the reference accumulates with atomics too, so the order of a particle's terms is
+// unspecified in both; the weight sum is reduced in a different order than the
+// reference's particle-index loop).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hfsi.h"
+#include "hfsi_common.hpp"
+
+namespace hfsi {
+
+namespace {
+
+constexpr uint32_t kMaxBinCells = 1u << 22;
+constexpr int kScatterWaves = 4;  // markers (wavefronts) per workgroup
+
+struct BinGrid {
+  float ox, oy, oz;  // origin (min corner of the finite particles)
+  float h;           // cell edge (>= support radius)
+  int nx, ny, nz;
+};
+
+__device__ inline bool finite3(float x, float y, float z) { return isfinite(x) && isfinite(y) && isfinite(z); }
+
+__device__ inline int bin_axis(float v, float o, float h, int n) {
+  const float f = floorf((v - o) / h);
+  return (int)fminf(fmaxf(f, 0.0f), (float)(n - 1));
+}
+
+// Per-workgroup min / max of the finite particle positions (6 floats per workgroup).
+__global__ void __launch_bounds__(256) k_particle_bounds(const float* x, int32_t n, float* partial) {
+  __shared__ float s[6][256];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const float a = x[3 * (size_t)p], b = x[3 * (size_t)p + 1], c = x[3 * (size_t)p + 2];
+    if (!finite3(a, b, c)) continue;
+    lo[0] = fminf(lo[0], a), lo[1] = fminf(lo[1], b), lo[2] = fminf(lo[2], c);
+    hi[0] = fmaxf(hi[0], a), hi[1] = fmaxf(hi[1], b), hi[2] = fmaxf(hi[2], c);
+  }
+  for (int d = 0; d < 3; ++d) {
+    s[d][threadIdx.x] = lo[d];
+    s[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      for (int d = 0; d < 3; ++d) {
+        s[d][threadIdx.x] = fminf(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+        s[3 + d][threadIdx.x] = fmaxf(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_bin_particles(const float* x, int32_t n, BinGrid g, int32_t* cell,
+                                                       uint32_t* count) {
+  for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const float a = x[3 * (size_t)p], b = x[3 * (size_t)p + 1], c = x[3 * (size_t)p + 2];
+    int32_t id = -1;
+    if (finite3(a, b, c)) {
+      id = (bin_axis(a, g.ox, g.h, g.nx) * g.ny + bin_axis(b, g.oy, g.h, g.ny)) * g.nz + bin_axis(c, g.oz, g.h, g.nz);
+      atomicAdd(&count[id], 1u);
+    }
+    cell[p] = id;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill_bins(int32_t n, const int32_t* cell, const uint32_t* start,
+                                                   uint32_t* cursor, int32_t* sorted) {
+  for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const int32_t id = cell[p];
+    if (id >= 0) sorted[start[id] + atomicAdd(&cursor[id], 1u)] = p;
+  }
+}
+
+struct ScatterTotals {
+  uint32_t active_markers, invalid_markers;
+  unsigned long long active_pairs;
+  double marker_force[3];
+  double external_force[3];
+};
+
+__device__ inline float tent_weight(float mx, float my, float mz, float px, float py, float pz, float h) {
+#pragma clang fp contract(off)
+  const float wx = fmaxf(1.0f - fabsf(mx - px) / h, 0.0f);
+  const float wy = fmaxf(1.0f - fabsf(my - py) / h, 0.0f);
+  const float wz = fmaxf(1.0f - fabsf(mz - pz) / h, 0.0f);
+  return wx * wy * wz;
+}
+
+__device__ inline float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One wavefront per marker.
+__global__ void __launch_bounds__(64 * kScatterWaves) k_scatter_markers(
+    const float* xm, const double* fm, int32_t markers, const float* xp, float* ext, BinGrid g,
+    const uint32_t* start, const uint32_t* count, const int32_t* sorted, float h, ScatterTotals* tot) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int32_t m = blockIdx.x * kScatterWaves + (threadIdx.x >> 6);
+  if (m >= markers) return;  // whole wavefronts leave together
+  const float mx = xm[3 * (size_t)m], my = xm[3 * (size_t)m + 1], mz = xm[3 * (size_t)m + 2];
+  const double fx = fm[3 * (size_t)m], fy = fm[3 * (size_t)m + 1], fz = fm[3 * (size_t)m + 2];
+  const bool valid = finite3(mx, my, mz) && isfinite(fx) && isfinite(fy) && isfinite(fz);
+  int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+  if (valid) {
+    const float o[3] = {g.ox, g.oy, g.oz}, c[3] = {mx, my, mz};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    for (int d = 0; d < 3; ++d) {
+      // cells that can hold particles within h of the marker along axis d (clamped to the grid);
+      // the reach is widened by 0.1 % so that rounding of the bin coordinates cannot drop one
+      const float r = 1.001f * h;
+      const float a = floorf((c[d] - r - o[d]) / g.h), b = floorf((c[d] + r - o[d]) / g.h);
+      lo[d] = (int)fminf(fmaxf(a, 0.0f), (float)(n[d] - 1));
+      hi[d] = (int)fminf(fmaxf(b, 0.0f), (float)(n[d] - 1));
+      if (b < 0.0f || a > (float)(n[d] - 1)) hi[d] = lo[d] - 1;  // no cell in reach
+    }
+  }
+  float wsum = 0.0f;
+  for (int i = lo[0]; i <= hi[0]; ++i)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int k = lo[2]; k <= hi[2]; ++k) {
+        const int32_t id = (i * g.ny + j) * g.nz + k;
+        const uint32_t s0 = start[id], s1 = s0 + count[id];
+        for (uint32_t q = s0 + lane; q < s1; q += 64) {
+          const int32_t p = sorted[q];
+          wsum += tent_weight(mx, my, mz, xp[3 * (size_t)p], xp[3 * (size_t)p + 1], xp[3 * (size_t)p + 2], h);
+        }
+      }
+  wsum = wave_sum_f(wsum);
+  const bool active = valid && wsum > 1.0e-12f;
+  double ex = 0.0, ey = 0.0, ez = 0.0;
+  unsigned long long pairs = 0;
+  if (active) {
+    for (int i = lo[0]; i <= hi[0]; ++i)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int k = lo[2]; k <= hi[2]; ++k) {
+          const int32_t id = (i * g.ny + j) * g.nz + k;
+          const uint32_t s0 = start[id], s1 = s0 + count[id];
+          for (uint32_t q = s0 + lane; q < s1; q += 64) {
+            const int32_t p = sorted[q];
+            const float w =
+                tent_weight(mx, my, mz, xp[3 * (size_t)p], xp[3 * (size_t)p + 1], xp[3 * (size_t)p + 2], h);
+            if (w > 0.0f) {
+              const double share = (double)(w / wsum);
+              const float cx = (float)(fx * share), cy = (float)(fy * share), cz = (float)(fz * share);
+              unsafeAtomicAdd(&ext[3 * (size_t)p], cx);
+              unsafeAtomicAdd(&ext[3 * (size_t)p + 1], cy);
+              unsafeAtomicAdd(&ext[3 * (size_t)p + 2], cz);
+              ex += (double)cx;
+              ey += (double)cy;
+              ez += (double)cz;
+              pairs += 1;
+            }
+          }
+        }
+  }
+  ex = wave_sum(ex);
+  ey = wave_sum(ey);
+  ez = wave_sum(ez);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off, 64);
+  if (lane == 0) {
+    if (active) {
+      atomicAdd(&tot->active_markers, 1u);
+      atomicAdd(&tot->active_pairs, pairs);
+      atomicAdd(&tot->marker_force[0], fx);
+      atomicAdd(&tot->marker_force[1], fy);
+      atomicAdd(&tot->marker_force[2], fz);
+      atomicAdd(&tot->external_force[0], ex);
+      atomicAdd(&tot->external_force[1], ey);
+      atomicAdd(&tot->external_force[2], ez);
+    } else {
+      atomicAdd(&tot->invalid_markers, 1u);
+    }
+  }
+}
+
+}  // namespace
+
+}  // namespace hfsi
+
+using namespace hfsi;
+
+extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const double* marker_force,
+                                          int32_t marker_count, const float* particle_position,
+                                          float* external_force, int32_t particle_count, float support_radius,
+                                          hfsi_marker_scatter_report* report, void* stream) {
+  if (marker_count < 0 || particle_count <= 0 || !particle_position || !external_force ||
+      (marker_count > 0 && (!marker_position || !marker_force)) || !(support_radius > 0.0f) ||
+      !std::isfinite(support_radius)) {
+    set_error("hfsi_scatter_marker_forces: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hfsi_marker_scatter_report rep{};
+  if (marker_count == 0) {
+    if (report) *report = rep;
+    return HFSI_OK;
+  }
+  // bounds of the finite particles
+  const int nb = std::max(1, std::min(1024, (particle_count + 255) / 256));
+  float* partial = nullptr;
+  HFSI_CHECK(hipMallocAsync((void**)&partial, 6 * nb * sizeof(float), st));
+  hipLaunchKernelGGL(k_particle_bounds, dim3(nb), dim3(256), 0, st, particle_position, particle_count, partial);
+  HFSI_LAUNCH_CHECK();
+  std::vector<float> hp(6 * nb);
+  HFSI_CHECK(hipMemcpyAsync(hp.data(), partial, hp.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nb; ++b)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], hp[6 * b + d]);
+      hi[d] = std::max(hi[d], hp[6 * b + 3 + d]);
+    }
+  BinGrid g{};
+  const bool any_particle = lo[0] <= hi[0];
+  g.h = support_radius;
+  int dims[3] = {1, 1, 1};
+  if (any_particle) {
+    for (;;) {
+      uint64_t cells = 1;
+      for (int d = 0; d < 3; ++d) {
+        dims[d] = (int)std::min<double>(std::floor((double)(hi[d] - lo[d]) / g.h) + 1.0, 1 << 20);
+        cells *= (uint64_t)dims[d];
+      }
+      if (cells <= kMaxBinCells) break;
+      g.h *= 2.0f;  // cells larger than the support only add candidates
+    }
+  }
+  g.ox = any_particle ? lo[0] : 0.0f;
+  g.oy = any_particle ? lo[1] : 0.0f;
+  g.oz = any_particle ? lo[2] : 0.0f;
+  g.nx = dims[0];
+  g.ny = dims[1];
+  g.nz = dims[2];
+  const uint32_t ncell = (uint32_t)(g.nx * g.ny * g.nz);
+  uint32_t *count = nullptr, *start = nullptr, *cursor = nullptr;
+  int32_t *cell = nullptr, *sorted = nullptr;
+  ScatterTotals* tot = nullptr;
+  void* scan_tmp = nullptr;
+  size_t scan_bytes = 0;
+  HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count, start, (int)ncell, st));
+  HFSI_CHECK(hipMallocAsync((void**)&count, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&start, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&cursor, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&cell, (size_t)particle_count * sizeof(int32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&sorted, (size_t)particle_count * sizeof(int32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&tot, sizeof(ScatterTotals), st));
+  HFSI_CHECK(hipMallocAsync(&scan_tmp, std::max<size_t>(scan_bytes, 1), st));
+  HFSI_CHECK(hipMemsetAsync(count, 0, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMemsetAsync(cursor, 0, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMemsetAsync(tot, 0, sizeof(ScatterTotals), st));
+  const int pb = std::max(1, std::min(4096, (particle_count + 255) / 256));
+  hipLaunchKernelGGL(k_bin_particles, dim3(pb), dim3(256), 0, st, particle_position, particle_count, g, cell, count);
+  HFSI_LAUNCH_CHECK();
+  HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, count, start, (int)ncell, st));
+  hipLaunchKernelGGL(k_fill_bins, dim3(pb), dim3(256), 0, st, particle_count, cell, start, cursor, sorted);
+  HFSI_LAUNCH_CHECK();
+  const uint32_t mb = (uint32_t)((marker_count + kScatterWaves - 1) / kScatterWaves);
+  hipLaunchKernelGGL(k_scatter_markers, dim3(mb), dim3(64 * kScatterWaves), 0, st, marker_position, marker_force,
+                     marker_count, particle_position, external_force, g, start, count, sorted, support_radius, tot);
+  HFSI_LAUNCH_CHECK();
+  ScatterTotals ht{};
+  HFSI_CHECK(hipMemcpyAsync(&ht, tot, sizeof(ht), hipMemcpyDeviceToHost, st));
+  for (void* p : {(void*)partial, (void*)count, (void*)start, (void*)cursor, (void*)cell, (void*)sorted, (void*)tot,
+                  scan_tmp})
+    HFSI_CHECK(hipFreeAsync(p, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  rep.active_marker_count = (int32_t)ht.active_markers;
+  rep.invalid_marker_count = (int32_t)ht.invalid_markers;
+  rep.active_particle_count = (int64_t)ht.active_pairs;
+  for (int d = 0; d < 3; ++d) {
+    rep.total_marker_force[d] = ht.marker_force[d];
+    rep.total_external_force[d] = ht.external_force[d];
+  }
+  if (report) *report = rep;
+  return HFSI_OK;
+}

@@ -1,0 +1,77 @@
+"""HIBM coupling hot spots on libhfsi (MI355X).
+
+``scatter_marker_forces_to_mpm_particles`` is the data-parallel part of
+``HibmMpmSurfaceMarkers.scatter_marker_forces_to_mpm_particles``
+(simulation_core/hibm_mpm.py:4962-5006, kernel 4899-4960): the same argument
+checks, accumulation into the particles' external force and report, with the
+marker arrays passed in (the marker class itself is outside this build).  The
+reference visits every (marker, particle) pair; libhfsi bins the particles by
+the support radius and visits 27 cells per marker (hfsi_scatter_marker_forces).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+
+
+@dataclass(frozen=True)
+class HibmMpmMpmForceScatterReport:
+    """hibm_mpm.py:99-105."""
+
+    active_marker_count: int
+    invalid_marker_count: int
+    active_particle_count: int
+    total_marker_force_n: tuple[float, float, float]
+    total_mpm_external_force_n: tuple[float, float, float]
+    action_reaction_residual_n: float
+
+
+def _tensor(field):
+    return field.tensor if hasattr(field, "tensor") else field
+
+
+def scatter_marker_forces_to_mpm_particles(marker_position_m, marker_force_n, external_force_n, particle_position_m,
+                                           *, particle_count: int, support_radius_m: float,
+                                           marker_count: int | None = None) -> HibmMpmMpmForceScatterReport:
+    """Add every marker's force to the particles within its support (hibm_mpm.py:4899-5006).
+
+    ``marker_position_m`` ([M, 3] float32) and ``marker_force_n`` ([M, 3] float64) are the
+    markers' ``x_gamma_m`` and ``F_gamma_n``; ``external_force_n`` ([P, 3] float32) is
+    accumulated in place, ``particle_position_m`` ([P, 3] float32) is read; Field objects
+    or CUDA tensors.  ``marker_count`` defaults to all rows of the marker arrays.
+    """
+    particles = int(particle_count)
+    if particles <= 0:
+        raise ValueError("particle_count must be positive")
+    support_radius = float(support_radius_m)
+    if not math.isfinite(support_radius) or support_radius <= 0.0:
+        raise ValueError("support_radius_m must be a finite positive number")
+    xm = _tensor(marker_position_m)
+    fm = _tensor(marker_force_n)
+    ext = _tensor(external_force_n)
+    xp = _tensor(particle_position_m)
+    m = int(xm.shape[0]) if marker_count is None else int(marker_count)
+    if xm.dtype != torch.float32 or fm.dtype != torch.float64 or ext.dtype != torch.float32 or xp.dtype != torch.float32:
+        raise TypeError("marker positions, external forces and particle positions are float32, marker forces float64")
+    if m > xm.shape[0] or m > fm.shape[0] or particles > xp.shape[0] or particles > ext.shape[0]:
+        raise ValueError("marker_count / particle_count exceed the arrays")
+    if not (xm.is_contiguous() and fm.is_contiguous() and ext.is_contiguous() and xp.is_contiguous()):
+        raise ValueError("marker and particle arrays must be contiguous")
+    lib = _lib.lib()
+    rep = _lib.MarkerScatterReport()
+    _lib.check(lib.hfsi_scatter_marker_forces(xm.data_ptr(), fm.data_ptr(), m, xp.data_ptr(), ext.data_ptr(),
+                                              particles, ctypes.c_float(support_radius), ctypes.byref(rep),
+                                              _lib.stream_handle()), "hfsi_scatter_marker_forces")
+    total_marker = tuple(float(v) for v in rep.total_marker_force)
+    total_external = tuple(float(v) for v in rep.total_external_force)
+    residual = math.sqrt(sum((a - b) * (a - b) for a, b in zip(total_marker, total_external)))
+    return HibmMpmMpmForceScatterReport(
+        active_marker_count=int(rep.active_marker_count), invalid_marker_count=int(rep.invalid_marker_count),
+        active_particle_count=int(rep.active_particle_count), total_marker_force_n=total_marker,
+        total_mpm_external_force_n=total_external, action_reaction_residual_n=residual)

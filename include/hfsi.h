@@ -406,6 +406,31 @@ typedef struct hfsi_axis_metrics {
 int hfsi_fluid_predict(hfsi_fluid* h, const float* velocity_prev, const int32_t* obstacle, float* velocity,
                        const hfsi_axis_metrics* metrics, float dt_s, float nu_m2_s, int32_t scheme, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * HIBM: marker -> MPM particle force scatter
+ * ------------------------------------------------------------------------- */
+
+typedef struct hfsi_marker_scatter_report {
+  int32_t active_marker_count;   /* markers whose force was spread                        */
+  int32_t invalid_marker_count;  /* non-finite markers and markers with weight sum <= 1e-12 */
+  int64_t active_particle_count; /* (marker, particle) pairs with a positive weight        */
+  double total_marker_force[3];  /* sum of the spread marker forces                       */
+  double total_external_force[3];/* sum of the float32 contributions added to particles   */
+} hfsi_marker_scatter_report;
+
+/* external_force[p] += sum over markers m of float32(F_m * (w_mp / sum_q w_mq)) with the
+ * tent weight w = prod_axis max(1 - |x_m - x_p| / support_radius, 0) in float32, over
+ * markers with finite position and force and a weight sum above 1e-12 and particles with
+ * finite positions.  Replaces
+ * HibmMpmSurfaceMarkers._scatter_marker_forces_to_mpm_particles_kernel
+ * (simulation_core/hibm_mpm.py:4899-4960); the particles are binned into cells of edge
+ * >= support_radius so each marker visits 27 cells instead of every particle.
+ * marker_position [M][3] float32, marker_force [M][3] float64, particle_position and
+ * external_force [P][3] float32 (device).  report (host) may be NULL.  Synchronises. */
+int hfsi_scatter_marker_forces(const float* marker_position, const double* marker_force, int32_t marker_count,
+                               const float* particle_position, float* external_force, int32_t particle_count,
+                               float support_radius, hfsi_marker_scatter_report* report, void* stream);
+
 /* Live kernel timing (no reference counterpart; measurement support):
  * while enabled, HIP events are recorded on the launch stream around every
  * launch of the selected level-0 kernel and their elapsed times summed.

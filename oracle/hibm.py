@@ -1,0 +1,96 @@
+"""NumPy restatement of the HIBM marker -> MPM particle force scatter (test oracle).
+
+TEST INFRASTRUCTURE ONLY -- never imported by the product package.
+
+Follows ``HibmMpmSurfaceMarkers._scatter_marker_forces_to_mpm_particles_kernel``
+(simulation_core/hibm_mpm.py:4899-4960) and its tent weight (4838-4848): float32
+positions and weights, float64 marker forces, the weight sum accumulated in float32
+in particle order, contributions cast to float32 and added in marker order.
+
+Pinning (tests/test_oracle_hibm.py): the reference's scatter tests (one marker on
+one particle, equal split between two symmetric particles, a non-finite marker
+leaving the forces untouched), force conservation (the normalised weights of a
+marker sum to one), the support cut-off, and a scalar per-pair loop.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+
+
+def tent_weight(marker, particles, support) -> np.ndarray:
+    """max(1 - |d| / h, 0) per axis, multiplied (hibm_mpm.py:4838-4848), float32."""
+    h = F32(support)
+    rel = np.asarray(marker, F32)[None, :] - np.asarray(particles, F32)
+    w = np.fmax(F32(1.0) - np.abs(rel) / h, F32(0.0))
+    return (w[:, 0] * w[:, 1]) * w[:, 2]
+
+
+def scatter_marker_forces(marker_position, marker_force, particle_position, external_force, support_radius):
+    """Returns (new external force [P, 3] float32, report dict)."""
+    xm = np.asarray(marker_position, F32)
+    fm = np.asarray(marker_force, np.float64)
+    xp = np.asarray(particle_position, F32)
+    ext = np.array(external_force, F32, copy=True)
+    finite_p = np.isfinite(xp).all(axis=1)
+    rep = {"active_marker_count": 0, "invalid_marker_count": 0, "active_particle_count": 0,
+           "total_marker_force_n": np.zeros(3), "total_mpm_external_force_n": np.zeros(3)}
+    for m in range(xm.shape[0]):
+        valid = bool(np.isfinite(xm[m]).all() and np.isfinite(fm[m]).all())
+        w = np.zeros(xp.shape[0], F32)
+        if valid:
+            w[finite_p] = tent_weight(xm[m], xp[finite_p], support_radius)
+        wsum = np.add.accumulate(w, dtype=F32)[-1] if valid and w.size else F32(0.0)
+        if valid and wsum > F32(1.0e-12):
+            rep["active_marker_count"] += 1
+            rep["total_marker_force_n"] += fm[m]
+            idx = np.nonzero(w > 0.0)[0]
+            share = (w[idx] / wsum).astype(F32).astype(np.float64)
+            contrib = (fm[m][None, :] * share[:, None]).astype(F32)
+            np.add.at(ext, idx, contrib)
+            rep["total_mpm_external_force_n"] += contrib.astype(np.float64).sum(axis=0)
+            rep["active_particle_count"] += int(idx.size)
+        else:
+            rep["invalid_marker_count"] += 1
+    rep["action_reaction_residual_n"] = float(np.linalg.norm(rep["total_marker_force_n"]
+                                                             - rep["total_mpm_external_force_n"]))
+    return ext, rep
+
+
+def scatter_scalar_loop(marker_position, marker_force, particle_position, external_force, support_radius):
+    """Pair-by-pair loop of the same kernel (small inputs only)."""
+    xm = np.asarray(marker_position, F32)
+    fm = np.asarray(marker_force, np.float64)
+    xp = np.asarray(particle_position, F32)
+    ext = np.array(external_force, F32, copy=True)
+    h = F32(support_radius)
+
+    def weight(a, b):
+        w = F32(1.0)
+        out = []
+        for d in range(3):
+            out.append(max(F32(F32(1.0) - F32(abs(F32(a[d] - b[d])) / h)), F32(0.0)))
+        w = F32(F32(out[0] * out[1]) * out[2])
+        return w
+
+    counts = [0, 0, 0]
+    for m in range(xm.shape[0]):
+        valid = all(np.isfinite(xm[m])) and all(np.isfinite(fm[m]))
+        wsum = F32(0.0)
+        if valid:
+            for p in range(xp.shape[0]):
+                if all(np.isfinite(xp[p])):
+                    wsum = F32(wsum + weight(xm[m], xp[p]))
+        if valid and wsum > F32(1.0e-12):
+            counts[0] += 1
+            for p in range(xp.shape[0]):
+                w = weight(xm[m], xp[p]) if all(np.isfinite(xp[p])) else F32(0.0)
+                if w > 0.0:
+                    share = float(F32(w / wsum))
+                    ext[p] = (ext[p] + np.array([fm[m][d] * share for d in range(3)]).astype(F32)).astype(F32)
+                    counts[2] += 1
+        else:
+            counts[1] += 1
+    return ext, counts

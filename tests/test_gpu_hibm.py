@@ -1,0 +1,142 @@
+"""GPU parity of the HIBM marker -> MPM particle force scatter
+(hfsi_scatter_marker_forces through easyfsi_amd.hibm) against the oracle
+(oracle/hibm.py).
+
+Tolerances: counts exact; external forces within 1e-5 of the largest force
+increment (the device reduces each marker's float32 weight sum across a
+wavefront, the oracle in particle order, and particles receive their float32
+terms through atomics); report totals rtol 1e-6.  The reference's scatter tests
+(tests/test_hibm.py:605-734, 4040-4070) run through the facade unchanged.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hibm as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _t(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def _run(xm, fm, xp, ext0, h):
+    from easyfsi_amd import hibm
+
+    ext = _t(ext0, torch.float32)
+    rep = hibm.scatter_marker_forces_to_mpm_particles(_t(xm, torch.float32), _t(fm, torch.float64), ext,
+                                                      _t(xp, torch.float32), particle_count=len(xp),
+                                                      support_radius_m=h)
+    torch.cuda.synchronize()
+    return ext.cpu().numpy(), rep
+
+
+def test_reference_scatter_one_marker_one_particle():  # test_hibm.py:605-642
+    ext, rep = _run([[0.5, 0.5, 0.5]], [[1.0, -2.0, 3.0]], [[0.5, 0.5, 0.5]], np.zeros((1, 3)), 0.25)
+    assert (rep.active_marker_count, rep.invalid_marker_count, rep.active_particle_count) == (1, 0, 1)
+    assert tuple(float(v) for v in ext[0]) == (1.0, -2.0, 3.0)
+    assert rep.total_marker_force_n == (1.0, -2.0, 3.0)
+    assert rep.total_mpm_external_force_n == (1.0, -2.0, 3.0)
+    assert rep.action_reaction_residual_n == pytest.approx(0.0)
+
+
+def test_reference_scatter_partitions_force():  # test_hibm.py:644-682
+    ext, rep = _run([[0.5, 0.5, 0.5]], [[0.0, 0.0, 4.0]], [[0.375, 0.5, 0.5], [0.625, 0.5, 0.5]], np.zeros((2, 3)),
+                    0.4)
+    assert rep.active_marker_count == 1 and rep.active_particle_count == 2
+    assert ext[0, 2] == pytest.approx(2.0, abs=1e-5) and ext[1, 2] == pytest.approx(2.0, abs=1e-5)
+    assert rep.total_mpm_external_force_n[2] == pytest.approx(4.0, abs=1e-5)
+
+
+def test_reference_non_finite_marker_force_is_invalid():  # test_hibm.py:4040-4070
+    ext, rep = _run([[0.5, 0.5, 0.5]], [[np.nan, 0.0, 0.0]], [[0.5, 0.5, 0.5]], [[1.0, 2.0, 3.0]], 0.5)
+    assert rep.active_marker_count == 0 and rep.invalid_marker_count == 1
+    assert rep.total_marker_force_n == (0.0, 0.0, 0.0) and rep.total_mpm_external_force_n == (0.0, 0.0, 0.0)
+    np.testing.assert_allclose(ext[0], (1.0, 2.0, 3.0), rtol=0, atol=1e-7)
+
+
+def test_argument_checks():
+    from easyfsi_amd import hibm
+
+    z = torch.zeros((1, 3), dtype=torch.float32, device="cuda")
+    f = torch.zeros((1, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        hibm.scatter_marker_forces_to_mpm_particles(z, f, z, z, particle_count=0, support_radius_m=0.1)
+    with pytest.raises(ValueError):
+        hibm.scatter_marker_forces_to_mpm_particles(z, f, z, z, particle_count=1, support_radius_m=float("nan"))
+
+
+def _cloud(seed, n_side, markers):
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n_side) + 0.5) / n_side
+    xp = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    xp = xp[np.linalg.norm(xp - 0.5, axis=1) < 0.4]
+    xp = (xp + rng.normal(0.0, 0.1 / n_side, xp.shape)).astype(np.float32)
+    d = rng.normal(size=(markers, 3))
+    xm = (0.5 + 0.38 * d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    fm = rng.normal(size=(markers, 3))
+    return rng, xp, xm, fm
+
+
+def test_scatter_matches_oracle_on_a_particle_ball():
+    _say("hibm scatter parity: ball")
+    rng, xp, xm, fm = _cloud(4, 34, 400)
+    xp[11] = np.nan
+    xm[5] = (3.0, 3.0, 3.0)  # no particle within reach
+    fm[8, 2] = np.inf  # invalid force
+    ext0 = rng.normal(size=xp.shape).astype(np.float32)
+    h = 2.2 / 34
+    ext, rep = _run(xm, fm, xp, ext0, h)
+    want, ref = H.scatter_marker_forces(xm, fm, xp, ext0, h)
+    assert rep.active_marker_count == ref["active_marker_count"] == 398
+    assert rep.invalid_marker_count == ref["invalid_marker_count"] == 2
+    assert rep.active_particle_count == ref["active_particle_count"]
+    scale = float(np.abs(want.astype(np.float64) - ext0).max())
+    np.testing.assert_allclose(ext, want, rtol=0, atol=1e-5 * scale)
+    np.testing.assert_allclose(rep.total_marker_force_n, ref["total_marker_force_n"], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(rep.total_mpm_external_force_n, ref["total_mpm_external_force_n"], rtol=1e-6,
+                               atol=1e-6)
+
+
+def test_scatter_conserves_force_at_million_particles():
+    _say("hibm scatter: 1M particles")
+    rng = np.random.default_rng(8)
+    n = 100
+    g = 0.35 + (np.arange(n) + 0.5) * (0.3 / n)
+    xp = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    k = 64
+    u = 0.35 + (np.arange(k) + 0.5) * (0.3 / k)
+    uu, vv = np.meshgrid(u, u, indexing="ij")
+    faces = []
+    for axis in range(3):
+        for side in (0.35, 0.65):
+            f = np.zeros((k * k, 3))
+            f[:, axis] = side
+            f[:, (axis + 1) % 3] = uu.ravel()
+            f[:, (axis + 2) % 3] = vv.ravel()
+            faces.append(f)
+    xm = np.concatenate(faces).astype(np.float32)
+    fm = rng.normal(size=xm.shape)
+    ext0 = np.zeros(xp.shape, np.float32)
+    h = 2.0 * 0.3 / n
+    ext, rep = _run(xm, fm, xp, ext0, h)
+    assert rep.active_marker_count == len(xm) and rep.invalid_marker_count == 0
+    np.testing.assert_allclose(rep.total_mpm_external_force_n, rep.total_marker_force_n, rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(ext.astype(np.float64).sum(axis=0), rep.total_marker_force_n, rtol=1e-4, atol=1e-3)
+    # interior particles farther than the support from every face receive nothing
+    inner = (np.abs(xp - 0.5) < 0.15 - h).all(axis=1)
+    assert np.all(ext[inner] == 0.0)
+    # a sample of markers checked against the oracle's pair count
+    sample = rng.choice(len(xm), 40, replace=False)
+    _, ref = H.scatter_marker_forces(xm[sample], fm[sample], xp, ext0, h)
+    _, rep_s = _run(xm[sample], fm[sample], xp, ext0, h)
+    assert rep_s.active_particle_count == ref["active_particle_count"]
