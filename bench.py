@@ -30,7 +30,8 @@ sequence (moving sphere, turning velocity) solved from p = 0, from the
 previous step's pressure and from the learned guess; the neo-Hookean MPM step
 (1M particles); the velocity predictor (Euler and RK2 semi-Lagrangian advection
 plus FV diffusion) on the same grid; the sharp-interface coupling pass and the
-operator with and without couplings; the CPU baseline (C + OpenMP restatement of the plain
+operator with and without couplings; the HIBM marker -> particle force scatter
+(1M particles, 24576 markers); the CPU baseline (C + OpenMP restatement of the plain
 algorithm on the host cores).
 
 Prints one JSON line.  --gpus above 1 is refused: this build measures one GPU.
@@ -587,6 +588,61 @@ def run_interface(solver, steps: int):
             "bytes_per_coupling": COUPLING_BYTES}
 
 
+def run_marker_scatter(steps: int, cpu: bool, lattice: int = 100, per_face: int = 64):
+    """HIBM marker -> particle force scatter: `lattice`^3 particles filling the MPM benchmark block
+    (0.35..0.65 cube), `per_face`^2 markers on each of its six faces, support radius two particle
+    spacings; each call bins the particles and spreads every marker's force (hfsi_scatter_marker_forces,
+    which synchronises).  CPU baseline: the reference's all-pairs loop (NumPy oracle) on a marker sample."""
+    import torch
+
+    from easyfsi_amd import hibm
+
+    g = 0.35 + (np.arange(lattice) + 0.5) * (0.3 / lattice)
+    xp = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    u = 0.35 + (np.arange(per_face) + 0.5) * (0.3 / per_face)
+    uu, vv = np.meshgrid(u, u, indexing="ij")
+    faces = []
+    for axis in range(3):
+        for side in (0.35, 0.65):
+            f = np.zeros((per_face * per_face, 3))
+            f[:, axis] = side
+            f[:, (axis + 1) % 3] = uu.ravel()
+            f[:, (axis + 2) % 3] = vv.ravel()
+            faces.append(f)
+    xm = np.concatenate(faces).astype(np.float32)
+    fm = np.random.default_rng(3).normal(size=xm.shape)
+    h = 2.0 * 0.3 / lattice
+    txp = torch.from_numpy(xp).cuda()
+    txm = torch.from_numpy(xm).cuda()
+    tfm = torch.from_numpy(fm).cuda()
+    ext = torch.zeros_like(txp)
+    for _ in range(2):
+        rep = hibm.scatter_marker_forces_to_mpm_particles(txm, tfm, ext, txp, particle_count=len(xp),
+                                                          support_radius_m=h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rep = hibm.scatter_marker_forces_to_mpm_particles(txm, tfm, ext, txp, particle_count=len(xp),
+                                                          support_radius_m=h)
+    dt = (time.perf_counter() - t0) / steps
+    out = {"particles": len(xp), "markers": len(xm), "support_radius_m": h, "ms_per_scatter": round(1e3 * dt, 4),
+           "pairs_per_scatter": rep.active_particle_count,
+           "action_reaction_residual_n": rep.action_reaction_residual_n}
+    if cpu:
+        from oracle import hibm as H
+
+        sample = 8
+        t0 = time.perf_counter()
+        H.scatter_marker_forces(xm[:sample], fm[:sample], xp, np.zeros_like(xp), h)
+        per_marker = (time.perf_counter() - t0) / sample
+        out["cpu_baseline"] = {"value": round(1.0 / (per_marker * len(xm)), 6), "unit": "scatters/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"NumPy oracle (oracle/hibm.py), the reference's all-pairs loop, on {sample} "
+                                         f"markers x {len(xp)} particles ({per_marker:.3f} s per marker), scaled "
+                                         f"to {len(xm)} markers"}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -606,6 +662,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mpm-steps", type=int, default=20, help="neo-Hookean MPM steps timed beside (0: skip)")
+    ap.add_argument("--scatter-steps", type=int, default=10,
+                    help="HIBM marker -> particle force scatters timed beside (0: skip)")
     ap.add_argument("--interface-steps", type=int, default=20,
                     help="interface coupling passes and operator applications timed beside (0: skip)")
     ap.add_argument("--predict-steps", type=int, default=20,
@@ -698,6 +756,11 @@ def main(argv=None) -> int:
         print(f"[bench] interface: {interface['couplings']} couplings, pass {interface['coupling_pass_us']:.1f} us, "
               f"apply {interface['apply_with_couplings_us']:.1f} vs {interface['apply_stencil_only_us']:.1f} us",
               file=sys.stderr, flush=True)
+    scatter = None
+    if args.scatter_steps > 0:
+        print(f"[bench] HIBM marker force scatter: {args.scatter_steps} calls", file=sys.stderr, flush=True)
+        scatter = run_marker_scatter(args.scatter_steps, not args.no_cpu)
+        print(f"[bench] scatter: {scatter['ms_per_scatter']:.3f} ms per call", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (C oracle)", file=sys.stderr, flush=True)
@@ -745,6 +808,8 @@ def main(argv=None) -> int:
         line["predictor"] = predict
     if interface is not None:
         line["interface"] = interface
+    if scatter is not None:
+        line["marker_scatter"] = scatter
     if sequence is not None:
         line["warm_start_sequence"] = sequence
     if project is not None:

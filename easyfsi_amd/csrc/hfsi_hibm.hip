@@ -9,7 +9,7 @@
 // to float32 and added to the particle's external force.  The reference visits every
 // (marker, particle) pair; only particles within h along every axis have w > 0, so
 // here the particles are binned into cells of edge >= h first (counting sort: count,
-// hipCUB exclusive scan, fill) and each marker visits the 27 cells around its own.  One
+// hipCUB exclusive scan, fill) and each marker visits the few cells around its own.  One
 // 64-lane wavefront per marker: the lanes stride over the candidates, the weight sum is
 // a wavefront reduction, then the lanes add their contributions with float32 atomics
 // (the reference accumulates with atomics too, so the order of a particle's terms is
@@ -116,14 +116,93 @@ __device__ inline float wave_sum_f(float v) {
   return v;
 }
 
-// One wavefront per marker.
+__device__ void scatter_one_marker(int32_t m, int lane, int wv, const float* xm, const double* fm, const float* xp,
+                                   float* ext, const BinGrid& g, const uint32_t* start, const uint32_t* count,
+                                   const int32_t* sorted, float h, uint32_t (*s_excl)[64], uint32_t (*s_start)[64],
+                                   ScatterTotals* s_tot);
+
+// Wavefront prefix sum (inclusive) of one int per lane.
+__device__ inline uint32_t wave_scan_incl(uint32_t v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// One wavefront per marker.  The marker's reach covers at most 4 cells per axis (2.002 h
+// over cells of edge >= h), so lane b takes cell b of that 4 x 4 x 4 block; a wavefront
+// prefix sum of the cell counts turns the candidates into one flat list that the lanes
+// stride over, finding their cell by a binary search over the prefix sums in LDS.
 __global__ void __launch_bounds__(64 * kScatterWaves) k_scatter_markers(
     const float* xm, const double* fm, int32_t markers, const float* xp, float* ext, BinGrid g,
     const uint32_t* start, const uint32_t* count, const int32_t* sorted, float h, ScatterTotals* tot) {
 #pragma clang fp contract(off)
+  __shared__ uint32_t s_excl[kScatterWaves][64];
+  __shared__ uint32_t s_start[kScatterWaves][64];
+  __shared__ ScatterTotals s_tot[kScatterWaves];
   const int lane = threadIdx.x & 63;
-  const int32_t m = blockIdx.x * kScatterWaves + (threadIdx.x >> 6);
-  if (m >= markers) return;  // whole wavefronts leave together
+  const int wv = threadIdx.x >> 6;
+  const int32_t m = blockIdx.x * kScatterWaves + wv;
+  if (lane == 0) s_tot[wv] = ScatterTotals{};
+  if (m < markers) scatter_one_marker(m, lane, wv, xm, fm, xp, ext, g, start, count, sorted, h, s_excl, s_start,
+                                      s_tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {  // this workgroup's totals, summed over its wavefronts in order
+    ScatterTotals t{};
+    for (int w = 0; w < kScatterWaves; ++w) {
+      t.active_markers += s_tot[w].active_markers;
+      t.invalid_markers += s_tot[w].invalid_markers;
+      t.active_pairs += s_tot[w].active_pairs;
+      for (int d = 0; d < 3; ++d) {
+        t.marker_force[d] += s_tot[w].marker_force[d];
+        t.external_force[d] += s_tot[w].external_force[d];
+      }
+    }
+    tot[blockIdx.x] = t;
+  }
+}
+
+// Sum of the per-workgroup totals (one workgroup, fixed order per thread then a tree).
+__global__ void __launch_bounds__(256) k_scatter_finish(const ScatterTotals* parts, uint32_t nparts,
+                                                        ScatterTotals* out) {
+  __shared__ ScatterTotals s[256];
+  ScatterTotals t{};
+  for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+    t.active_markers += parts[b].active_markers;
+    t.invalid_markers += parts[b].invalid_markers;
+    t.active_pairs += parts[b].active_pairs;
+    for (int d = 0; d < 3; ++d) {
+      t.marker_force[d] += parts[b].marker_force[d];
+      t.external_force[d] += parts[b].external_force[d];
+    }
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ScatterTotals& a = s[threadIdx.x];
+      const ScatterTotals& b = s[threadIdx.x + w];
+      a.active_markers += b.active_markers;
+      a.invalid_markers += b.invalid_markers;
+      a.active_pairs += b.active_pairs;
+      for (int d = 0; d < 3; ++d) {
+        a.marker_force[d] += b.marker_force[d];
+        a.external_force[d] += b.external_force[d];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+// One marker on one wavefront (see k_scatter_markers).
+__device__ void scatter_one_marker(int32_t m, int lane, int wv, const float* xm, const double* fm, const float* xp,
+                                   float* ext, const BinGrid& g, const uint32_t* start, const uint32_t* count,
+                                   const int32_t* sorted, float h, uint32_t (*s_excl)[64], uint32_t (*s_start)[64],
+                                   ScatterTotals* s_tot) {
+#pragma clang fp contract(off)
   const float mx = xm[3 * (size_t)m], my = xm[3 * (size_t)m + 1], mz = xm[3 * (size_t)m + 2];
   const double fx = fm[3 * (size_t)m], fy = fm[3 * (size_t)m + 1], fz = fm[3 * (size_t)m + 2];
   const bool valid = finite3(mx, my, mz) && isfinite(fx) && isfinite(fy) && isfinite(fz);
@@ -141,44 +220,53 @@ __global__ void __launch_bounds__(64 * kScatterWaves) k_scatter_markers(
       if (b < 0.0f || a > (float)(n[d] - 1)) hi[d] = lo[d] - 1;  // no cell in reach
     }
   }
+  // lane b: cell (lo + (b >> 4, (b >> 2) & 3, b & 3)) when inside the reach
+  const int ci = lo[0] + (lane >> 4), cj = lo[1] + ((lane >> 2) & 3), ck = lo[2] + (lane & 3);
+  uint32_t cnt = 0, st0 = 0;
+  if (ci <= hi[0] && cj <= hi[1] && ck <= hi[2]) {
+    const int32_t id = (ci * g.ny + cj) * g.nz + ck;
+    cnt = count[id];
+    st0 = start[id];
+  }
+  const uint32_t incl = wave_scan_incl(cnt, lane);
+  const uint32_t total = __shfl(incl, 63, 64);
+  s_excl[wv][lane] = incl - cnt;
+  s_start[wv][lane] = st0;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  auto particle_of = [&](uint32_t q) -> int32_t {
+    int b = 0;  // last cell with exclusive prefix <= q (empty cells share their successor's prefix)
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+      if (b + step < 64 && s_excl[wv][b + step] <= q) b += step;
+    return sorted[s_start[wv][b] + (q - s_excl[wv][b])];
+  };
   float wsum = 0.0f;
-  for (int i = lo[0]; i <= hi[0]; ++i)
-    for (int j = lo[1]; j <= hi[1]; ++j)
-      for (int k = lo[2]; k <= hi[2]; ++k) {
-        const int32_t id = (i * g.ny + j) * g.nz + k;
-        const uint32_t s0 = start[id], s1 = s0 + count[id];
-        for (uint32_t q = s0 + lane; q < s1; q += 64) {
-          const int32_t p = sorted[q];
-          wsum += tent_weight(mx, my, mz, xp[3 * (size_t)p], xp[3 * (size_t)p + 1], xp[3 * (size_t)p + 2], h);
-        }
-      }
+  for (uint32_t q = lane; q < total; q += 64) {
+    const int32_t p = particle_of(q);
+    wsum += tent_weight(mx, my, mz, xp[3 * (size_t)p], xp[3 * (size_t)p + 1], xp[3 * (size_t)p + 2], h);
+  }
   wsum = wave_sum_f(wsum);
   const bool active = valid && wsum > 1.0e-12f;
   double ex = 0.0, ey = 0.0, ez = 0.0;
   unsigned long long pairs = 0;
   if (active) {
-    for (int i = lo[0]; i <= hi[0]; ++i)
-      for (int j = lo[1]; j <= hi[1]; ++j)
-        for (int k = lo[2]; k <= hi[2]; ++k) {
-          const int32_t id = (i * g.ny + j) * g.nz + k;
-          const uint32_t s0 = start[id], s1 = s0 + count[id];
-          for (uint32_t q = s0 + lane; q < s1; q += 64) {
-            const int32_t p = sorted[q];
-            const float w =
-                tent_weight(mx, my, mz, xp[3 * (size_t)p], xp[3 * (size_t)p + 1], xp[3 * (size_t)p + 2], h);
-            if (w > 0.0f) {
-              const double share = (double)(w / wsum);
-              const float cx = (float)(fx * share), cy = (float)(fy * share), cz = (float)(fz * share);
-              unsafeAtomicAdd(&ext[3 * (size_t)p], cx);
-              unsafeAtomicAdd(&ext[3 * (size_t)p + 1], cy);
-              unsafeAtomicAdd(&ext[3 * (size_t)p + 2], cz);
-              ex += (double)cx;
-              ey += (double)cy;
-              ez += (double)cz;
-              pairs += 1;
-            }
-          }
-        }
+    for (uint32_t q = lane; q < total; q += 64) {
+      const int32_t p = particle_of(q);
+      const float w = tent_weight(mx, my, mz, xp[3 * (size_t)p], xp[3 * (size_t)p + 1], xp[3 * (size_t)p + 2], h);
+      if (w > 0.0f) {
+        const double share = (double)(w / wsum);
+        const float cx = (float)(fx * share), cy = (float)(fy * share), cz = (float)(fz * share);
+        unsafeAtomicAdd(&ext[3 * (size_t)p], cx);
+        unsafeAtomicAdd(&ext[3 * (size_t)p + 1], cy);
+        unsafeAtomicAdd(&ext[3 * (size_t)p + 2], cz);
+        ex += (double)cx;
+        ey += (double)cy;
+        ez += (double)cz;
+        pairs += 1;
+      }
+    }
   }
   ex = wave_sum(ex);
   ey = wave_sum(ey);
@@ -186,17 +274,18 @@ __global__ void __launch_bounds__(64 * kScatterWaves) k_scatter_markers(
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off, 64);
   if (lane == 0) {
+    ScatterTotals& t = s_tot[wv];
     if (active) {
-      atomicAdd(&tot->active_markers, 1u);
-      atomicAdd(&tot->active_pairs, pairs);
-      atomicAdd(&tot->marker_force[0], fx);
-      atomicAdd(&tot->marker_force[1], fy);
-      atomicAdd(&tot->marker_force[2], fz);
-      atomicAdd(&tot->external_force[0], ex);
-      atomicAdd(&tot->external_force[1], ey);
-      atomicAdd(&tot->external_force[2], ez);
+      t.active_markers = 1;
+      t.active_pairs = pairs;
+      t.marker_force[0] = fx;
+      t.marker_force[1] = fy;
+      t.marker_force[2] = fz;
+      t.external_force[0] = ex;
+      t.external_force[1] = ey;
+      t.external_force[2] = ez;
     } else {
-      atomicAdd(&tot->invalid_markers, 1u);
+      t.invalid_markers = 1;
     }
   }
 }
@@ -271,25 +360,28 @@ extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const do
   HFSI_CHECK(hipMallocAsync((void**)&cursor, ncell * sizeof(uint32_t), st));
   HFSI_CHECK(hipMallocAsync((void**)&cell, (size_t)particle_count * sizeof(int32_t), st));
   HFSI_CHECK(hipMallocAsync((void**)&sorted, (size_t)particle_count * sizeof(int32_t), st));
+  const uint32_t mb = (uint32_t)((marker_count + kScatterWaves - 1) / kScatterWaves);
+  ScatterTotals* parts = nullptr;
   HFSI_CHECK(hipMallocAsync((void**)&tot, sizeof(ScatterTotals), st));
+  HFSI_CHECK(hipMallocAsync((void**)&parts, mb * sizeof(ScatterTotals), st));
   HFSI_CHECK(hipMallocAsync(&scan_tmp, std::max<size_t>(scan_bytes, 1), st));
   HFSI_CHECK(hipMemsetAsync(count, 0, ncell * sizeof(uint32_t), st));
   HFSI_CHECK(hipMemsetAsync(cursor, 0, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMemsetAsync(tot, 0, sizeof(ScatterTotals), st));
   const int pb = std::max(1, std::min(4096, (particle_count + 255) / 256));
   hipLaunchKernelGGL(k_bin_particles, dim3(pb), dim3(256), 0, st, particle_position, particle_count, g, cell, count);
   HFSI_LAUNCH_CHECK();
   HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, count, start, (int)ncell, st));
   hipLaunchKernelGGL(k_fill_bins, dim3(pb), dim3(256), 0, st, particle_count, cell, start, cursor, sorted);
   HFSI_LAUNCH_CHECK();
-  const uint32_t mb = (uint32_t)((marker_count + kScatterWaves - 1) / kScatterWaves);
   hipLaunchKernelGGL(k_scatter_markers, dim3(mb), dim3(64 * kScatterWaves), 0, st, marker_position, marker_force,
-                     marker_count, particle_position, external_force, g, start, count, sorted, support_radius, tot);
+                     marker_count, particle_position, external_force, g, start, count, sorted, support_radius, parts);
+  HFSI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scatter_finish, dim3(1), dim3(256), 0, st, parts, mb, tot);
   HFSI_LAUNCH_CHECK();
   ScatterTotals ht{};
   HFSI_CHECK(hipMemcpyAsync(&ht, tot, sizeof(ht), hipMemcpyDeviceToHost, st));
   for (void* p : {(void*)partial, (void*)count, (void*)start, (void*)cursor, (void*)cell, (void*)sorted, (void*)tot,
-                  scan_tmp})
+                  (void*)parts, scan_tmp})
     HFSI_CHECK(hipFreeAsync(p, st));
   HFSI_CHECK(hipStreamSynchronize(st));
   rep.active_marker_count = (int32_t)ht.active_markers;
