@@ -409,9 +409,16 @@ class CartesianFluidSolver:
                                               1 if pressure_outlet_zmin else 0, self._stream),
                    "hfsi_fluid_apply")
 
-    def _update_pressure_interface_projection_divergence(self, dt_over_rho: float, accumulate: bool = False) -> None:
+    def _update_pressure_interface_projection_divergence(self, dt_over_rho: float, accumulate: bool = False,
+                                                         terms_present: bool = True) -> None:
         """fluid.py:5625-5693 (accumulate=False) and 5695-5759 (True): the interface part
-        of the operator applied to the pressure, times dt/rho."""
+        of the operator applied to the pressure, times dt/rho.  ``terms_present=False``
+        (the caller has just found no interface diagonal, right-hand side or coupling)
+        gives the same all-zero field without the passes over the grid."""
+        if not terms_present:
+            if not accumulate:
+                self.pressure_interface_projection_divergence_s.tensor.zero_()
+            return
         k = float(np.float32(dt_over_rho))
         p = self.pressure.tensor
         out = self.pressure_interface_projection_divergence_s.tensor
@@ -659,6 +666,15 @@ class CartesianFluidSolver:
         self._set_interface_couplings()
         d = self.pressure_interface_matrix_diagonal.tensor
         r = self.pressure_interface_matrix_rhs.tensor
+        dmax = float(d.abs().max().item())
+        self._iface_terms_present = bool(dmax > 0.0 or self._iface_compiled or self._iface_row_report[1] > 0
+                                         or bool(torch.any(r != 0).item()))
+        if not self._iface_terms_present:  # every integral, count and mismatch is zero
+            return {"diagonal_integral": 0.0, "rhs_integral": 0.0, "max_abs_diagonal": dmax, "active_cells": 0,
+                    "row_count": int(self.pressure_interface_row_count.tensor.item()), "row_active_count": 0,
+                    "row_invalid_count": 0, "row_overflow_count": int(self._iface_row_report[2]),
+                    "row_diagonal_integral": 0.0, "row_diagonal_integral_abs_mismatch": 0.0,
+                    "row_diagonal_max_abs_density_mismatch": 0.0, "row_max_transmissibility": 0.0}
         vol = ((self.cell_width_x_m.tensor[:, None, None] * self.cell_width_y_m.tensor[None, :, None])
                * self.cell_width_z_m.tensor[None, None, :]).double()
         dens = (self.__dict__["pressure_interface_row_diagonal_density"].tensor
@@ -836,7 +852,8 @@ class CartesianFluidSolver:
         record_cg_stats()
         handle_failure()
         self._subtract_pressure_gradient(step_dt_s / self.rho, outlet)
-        self._update_pressure_interface_projection_divergence(step_dt_s / self.rho)
+        self._update_pressure_interface_projection_divergence(step_dt_s / self.rho,
+                                                              terms_present=self._iface_terms_present)
         apply_bcs()
         self._divergence_only(outlet)
         target_l2_sq = 0.0
@@ -868,7 +885,8 @@ class CartesianFluidSolver:
                 record_cg_stats()
                 handle_failure()
                 self._subtract_pressure_gradient(step_dt_s / self.rho, outlet)
-                self._update_pressure_interface_projection_divergence(step_dt_s / self.rho, accumulate=True)
+                self._update_pressure_interface_projection_divergence(step_dt_s / self.rho, accumulate=True,
+                                                                      terms_present=self._iface_terms_present)
                 self.pressure_accum.tensor.add_(self.pressure.tensor)
                 self.pressure.copy_from(self.pressure_accum)
                 self.pressure_tmp.copy_from(self.pressure_accum)

@@ -5,7 +5,8 @@
 Tolerances: counts exact; external forces within 1e-5 of the largest force
 increment (the device reduces each marker's float32 weight sum across a
 wavefront, the oracle in particle order, and particles receive their float32
-terms through atomics); report totals rtol 1e-6.  The reference's scatter tests
+terms through atomics); marker-force total rtol 1e-6; external-force total
+within 1e-6 of the summed marker-force magnitudes (float32 contributions).  The reference's scatter tests
 (tests/test_hibm.py:605-734, 4040-4070) run through the facade unchanged.
 """
 
@@ -103,8 +104,11 @@ def test_scatter_matches_oracle_on_a_particle_ball():
     scale = float(np.abs(want.astype(np.float64) - ext0).max())
     np.testing.assert_allclose(ext, want, rtol=0, atol=1e-5 * scale)
     np.testing.assert_allclose(rep.total_marker_force_n, ref["total_marker_force_n"], rtol=1e-6, atol=1e-9)
-    np.testing.assert_allclose(rep.total_mpm_external_force_n, ref["total_mpm_external_force_n"], rtol=1e-6,
-                               atol=1e-6)
+    # each float32 contribution carries a relative rounding of ~1e-7 and a marker's shares sum
+    # to one, so the total can move by ~1e-7 of the summed force magnitudes
+    mag = np.abs(fm[np.isfinite(fm).all(axis=1)]).sum(axis=0)
+    np.testing.assert_allclose(rep.total_mpm_external_force_n, ref["total_mpm_external_force_n"], rtol=0,
+                               atol=1e-6 * mag.max())
 
 
 def test_scatter_conserves_force_at_million_particles():
