@@ -124,6 +124,13 @@ class MarkerScatterReport(ctypes.Structure):
                 ("total_external_force", ctypes.c_double * 3)]
 
 
+class MarkerFeedbackReport(ctypes.Structure):
+    """hfsi_marker_feedback_report."""
+
+    _fields_ = [("updated_marker_count", ctypes.c_int32), ("invalid_marker_count", ctypes.c_int32),
+                ("max_marker_displacement", ctypes.c_double), ("max_marker_speed", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 _D = ctypes.c_double
@@ -157,6 +164,8 @@ SIGNATURES = {
     "hfsi_fluid_solve_bicgstab": (ctypes.c_int, [_P, _P, _P, _I, _I, _D, _I, _D, _I, ctypes.POINTER(CgStats), _P]),
     "hfsi_scatter_marker_forces": (ctypes.c_int, [_P, _P, _I, _P, _P, _I, _F, ctypes.POINTER(MarkerScatterReport),
                                                   _P]),
+    "hfsi_marker_feedback": (ctypes.c_int, [_P, _P, _I, _P, _P, _I, _F, _F, ctypes.POINTER(MarkerFeedbackReport),
+                                            _P]),
     "hfsi_fluid_predict": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(AxisMetrics), _F, _F, _I, _P]),
     "hfsi_fluid_kernel_timer": (ctypes.c_int, [_P, _I]),
     "hfsi_fluid_kernel_timer_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),

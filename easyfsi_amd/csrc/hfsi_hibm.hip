@@ -27,6 +27,12 @@
 #include "../../include/hfsi.h"
 #include "hfsi_common.hpp"
 
+#define HFSI_TRY_BINS(expr)         \
+  do {                              \
+    const int _rc = (expr);         \
+    if (_rc != HFSI_OK) return _rc; \
+  } while (0)
+
 namespace hfsi {
 
 namespace {
@@ -116,13 +122,68 @@ __device__ inline float wave_sum_f(float v) {
   return v;
 }
 
+// Cells in reach of a marker: along each axis the cells that can hold particles within h
+// (clamped to the grid; widened by 0.1 % so that rounding of the bin coordinates cannot
+// drop one).  At most 4 per axis.  hi < lo: none.
+__device__ inline void marker_reach(float mx, float my, float mz, bool valid, const BinGrid& g, float h, int (&lo)[3],
+                                    int (&hi)[3]) {
+#pragma clang fp contract(off)
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = 0;
+    hi[d] = -1;
+  }
+  if (!valid) return;
+  const float o[3] = {g.ox, g.oy, g.oz}, c[3] = {mx, my, mz};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const float r = 1.001f * h;
+  for (int d = 0; d < 3; ++d) {
+    const float a = floorf((c[d] - r - o[d]) / g.h), b = floorf((c[d] + r - o[d]) / g.h);
+    lo[d] = (int)fminf(fmaxf(a, 0.0f), (float)(n[d] - 1));
+    hi[d] = (int)fminf(fmaxf(b, 0.0f), (float)(n[d] - 1));
+    if (b < 0.0f || a > (float)(n[d] - 1)) hi[d] = lo[d] - 1;
+  }
+}
+
+__device__ uint32_t wave_scan_incl(uint32_t v, int lane);
+
+// Lane b takes cell b of the 4 x 4 x 4 block at lo; the exclusive prefix sums of the cell
+// counts and the cells' first sorted index go to LDS; returns the number of candidates.
+__device__ inline uint32_t flat_candidates(const int (&lo)[3], const int (&hi)[3], const BinGrid& g,
+                                           const uint32_t* start, const uint32_t* count, int lane, uint32_t* excl,
+                                           uint32_t* first) {
+  const int ci = lo[0] + (lane >> 4), cj = lo[1] + ((lane >> 2) & 3), ck = lo[2] + (lane & 3);
+  uint32_t cnt = 0, st0 = 0;
+  if (ci <= hi[0] && cj <= hi[1] && ck <= hi[2]) {
+    const int32_t id = (ci * g.ny + cj) * g.nz + ck;
+    cnt = count[id];
+    st0 = start[id];
+  }
+  const uint32_t incl = wave_scan_incl(cnt, lane);
+  excl[lane] = incl - cnt;
+  first[lane] = st0;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return __shfl(incl, 63, 64);
+}
+
+// Particle of flat candidate q: the last cell whose exclusive prefix is <= q holds it.
+__device__ inline int32_t candidate_particle(uint32_t q, const uint32_t* excl, const uint32_t* first,
+                                             const int32_t* sorted) {
+  int b = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1)
+    if (b + step < 64 && excl[b + step] <= q) b += step;
+  return sorted[first[b] + (q - excl[b])];
+}
+
 __device__ void scatter_one_marker(int32_t m, int lane, int wv, const float* xm, const double* fm, const float* xp,
                                    float* ext, const BinGrid& g, const uint32_t* start, const uint32_t* count,
                                    const int32_t* sorted, float h, uint32_t (*s_excl)[64], uint32_t (*s_start)[64],
                                    ScatterTotals* s_tot);
 
 // Wavefront prefix sum (inclusive) of one int per lane.
-__device__ inline uint32_t wave_scan_incl(uint32_t v, int lane) {
+__device__ uint32_t wave_scan_incl(uint32_t v, int lane) {
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const uint32_t u = __shfl_up(v, off, 64);
@@ -290,36 +351,137 @@ __device__ void scatter_one_marker(int32_t m, int lane, int wv, const float* xm,
   }
 }
 
+struct FeedbackTotals {
+  uint32_t updated, invalid;
+  float max_disp, max_speed;
+};
+
+// Marker surface feedback (hibm_mpm.py:5182-5224): one wavefront per marker; the weighted
+// velocity of the particles within the support becomes the marker velocity and moves the
+// marker by dt times it.  float32 throughout, as the reference.
+__global__ void __launch_bounds__(64 * kScatterWaves) k_feedback_markers(
+    float* xm, float* vm, int32_t markers, const float* xp, const float* vp, BinGrid g, const uint32_t* start,
+    const uint32_t* count, const int32_t* sorted, float h, float dt, FeedbackTotals* parts) {
+#pragma clang fp contract(off)
+  __shared__ uint32_t s_excl[kScatterWaves][64];
+  __shared__ uint32_t s_first[kScatterWaves][64];
+  __shared__ FeedbackTotals s_tot[kScatterWaves];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int32_t m = blockIdx.x * kScatterWaves + wv;
+  if (lane == 0) s_tot[wv] = FeedbackTotals{};
+  if (m < markers) {
+    const float mx = xm[3 * (size_t)m], my = xm[3 * (size_t)m + 1], mz = xm[3 * (size_t)m + 2];
+    int lo[3], hi[3];
+    // a non-finite marker has weight 0 with every particle (the reference's max(NaN, 0) = 0)
+    marker_reach(mx, my, mz, finite3(mx, my, mz), g, h, lo, hi);
+    const uint32_t total = flat_candidates(lo, hi, g, start, count, lane, s_excl[wv], s_first[wv]);
+    float ws = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t q = lane; q < total; q += 64) {
+      const int32_t p = candidate_particle(q, s_excl[wv], s_first[wv], sorted);
+      const float w = tent_weight(mx, my, mz, xp[3 * (size_t)p], xp[3 * (size_t)p + 1], xp[3 * (size_t)p + 2], h);
+      if (w > 0.0f) {
+        sx = sx + w * vp[3 * (size_t)p];
+        sy = sy + w * vp[3 * (size_t)p + 1];
+        sz = sz + w * vp[3 * (size_t)p + 2];
+        ws = ws + w;
+      }
+    }
+    ws = wave_sum_f(ws);
+    sx = wave_sum_f(sx);
+    sy = wave_sum_f(sy);
+    sz = wave_sum_f(sz);
+    if (lane == 0) {
+      FeedbackTotals& t = s_tot[wv];
+      if (ws > 1.0e-12f) {
+        const float nvx = sx / ws, nvy = sy / ws, nvz = sz / ws;
+        const float nx = mx + dt * nvx, ny = my + dt * nvy, nz = mz + dt * nvz;
+        xm[3 * (size_t)m] = nx;
+        xm[3 * (size_t)m + 1] = ny;
+        xm[3 * (size_t)m + 2] = nz;
+        vm[3 * (size_t)m] = nvx;
+        vm[3 * (size_t)m + 1] = nvy;
+        vm[3 * (size_t)m + 2] = nvz;
+        const float dx = nx - mx, dy = ny - my, dz = nz - mz;
+        t.updated = 1;
+        t.max_disp = sqrtf(dx * dx + dy * dy + dz * dz);
+        t.max_speed = sqrtf(nvx * nvx + nvy * nvy + nvz * nvz);
+      } else {
+        t.invalid = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    FeedbackTotals t{};
+    for (int w = 0; w < kScatterWaves; ++w) {
+      t.updated += s_tot[w].updated;
+      t.invalid += s_tot[w].invalid;
+      t.max_disp = fmaxf(t.max_disp, s_tot[w].max_disp);
+      t.max_speed = fmaxf(t.max_speed, s_tot[w].max_speed);
+    }
+    parts[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_feedback_finish(const FeedbackTotals* parts, uint32_t nparts,
+                                                         FeedbackTotals* out) {
+  __shared__ FeedbackTotals s[256];
+  FeedbackTotals t{};
+  for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+    t.updated += parts[b].updated;
+    t.invalid += parts[b].invalid;
+    t.max_disp = fmaxf(t.max_disp, parts[b].max_disp);
+    t.max_speed = fmaxf(t.max_speed, parts[b].max_speed);
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[threadIdx.x].updated += s[threadIdx.x + w].updated;
+      s[threadIdx.x].invalid += s[threadIdx.x + w].invalid;
+      s[threadIdx.x].max_disp = fmaxf(s[threadIdx.x].max_disp, s[threadIdx.x + w].max_disp);
+      s[threadIdx.x].max_speed = fmaxf(s[threadIdx.x].max_speed, s[threadIdx.x + w].max_speed);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
 }  // namespace
 
 }  // namespace hfsi
 
 using namespace hfsi;
 
-extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const double* marker_force,
-                                          int32_t marker_count, const float* particle_position,
-                                          float* external_force, int32_t particle_count, float support_radius,
-                                          hfsi_marker_scatter_report* report, void* stream) {
-  if (marker_count < 0 || particle_count <= 0 || !particle_position || !external_force ||
-      (marker_count > 0 && (!marker_position || !marker_force)) || !(support_radius > 0.0f) ||
-      !std::isfinite(support_radius)) {
-    set_error("hfsi_scatter_marker_forces: invalid arguments");
-    return HFSI_ERR_ARG;
-  }
-  hipStream_t st = (hipStream_t)stream;
-  hfsi_marker_scatter_report rep{};
-  if (marker_count == 0) {
-    if (report) *report = rep;
-    return HFSI_OK;
-  }
-  // bounds of the finite particles
-  const int nb = std::max(1, std::min(1024, (particle_count + 255) / 256));
+namespace {
+
+// Particles binned by cells of edge >= the support radius (device arrays, stream-ordered).
+struct ParticleBins {
+  BinGrid g{};
+  uint32_t *count = nullptr, *start = nullptr, *cursor = nullptr;
+  int32_t *cell = nullptr, *sorted = nullptr;
   float* partial = nullptr;
-  HFSI_CHECK(hipMallocAsync((void**)&partial, 6 * nb * sizeof(float), st));
-  hipLaunchKernelGGL(k_particle_bounds, dim3(nb), dim3(256), 0, st, particle_position, particle_count, partial);
+  void* scan_tmp = nullptr;
+};
+
+int free_bins(ParticleBins& B, hipStream_t st) {
+  for (void* p : {(void*)B.count, (void*)B.start, (void*)B.cursor, (void*)B.cell, (void*)B.sorted, (void*)B.partial,
+                  B.scan_tmp})
+    if (p) HFSI_CHECK(hipFreeAsync(p, st));
+  B = ParticleBins{};
+  return HFSI_OK;
+}
+
+// Bounds of the finite particles, then a counting sort of them into cells (count, hipCUB
+// exclusive scan, fill).  Synchronises once (the bounds set the grid).
+int bin_particles(const float* xp, int32_t np, float support, hipStream_t st, ParticleBins& B) {
+  const int nb = std::max(1, std::min(1024, (np + 255) / 256));
+  HFSI_CHECK(hipMallocAsync((void**)&B.partial, 6 * nb * sizeof(float), st));
+  hipLaunchKernelGGL(k_particle_bounds, dim3(nb), dim3(256), 0, st, xp, np, B.partial);
   HFSI_LAUNCH_CHECK();
   std::vector<float> hp(6 * nb);
-  HFSI_CHECK(hipMemcpyAsync(hp.data(), partial, hp.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipMemcpyAsync(hp.data(), B.partial, hp.size() * sizeof(float), hipMemcpyDeviceToHost, st));
   HFSI_CHECK(hipStreamSynchronize(st));
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int b = 0; b < nb; ++b)
@@ -327,9 +489,9 @@ extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const do
       lo[d] = std::min(lo[d], hp[6 * b + d]);
       hi[d] = std::max(hi[d], hp[6 * b + 3 + d]);
     }
-  BinGrid g{};
+  BinGrid& g = B.g;
   const bool any_particle = lo[0] <= hi[0];
-  g.h = support_radius;
+  g.h = support;
   int dims[3] = {1, 1, 1};
   if (any_particle) {
     for (;;) {
@@ -349,40 +511,64 @@ extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const do
   g.ny = dims[1];
   g.nz = dims[2];
   const uint32_t ncell = (uint32_t)(g.nx * g.ny * g.nz);
-  uint32_t *count = nullptr, *start = nullptr, *cursor = nullptr;
-  int32_t *cell = nullptr, *sorted = nullptr;
-  ScatterTotals* tot = nullptr;
-  void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
-  HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count, start, (int)ncell, st));
-  HFSI_CHECK(hipMallocAsync((void**)&count, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&start, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&cursor, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&cell, (size_t)particle_count * sizeof(int32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&sorted, (size_t)particle_count * sizeof(int32_t), st));
+  HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, B.count, B.start, (int)ncell, st));
+  HFSI_CHECK(hipMallocAsync((void**)&B.count, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&B.start, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&B.cursor, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&B.cell, (size_t)np * sizeof(int32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&B.sorted, (size_t)np * sizeof(int32_t), st));
+  HFSI_CHECK(hipMallocAsync(&B.scan_tmp, std::max<size_t>(scan_bytes, 1), st));
+  HFSI_CHECK(hipMemsetAsync(B.count, 0, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMemsetAsync(B.cursor, 0, ncell * sizeof(uint32_t), st));
+  const int pb = std::max(1, std::min(4096, (np + 255) / 256));
+  hipLaunchKernelGGL(k_bin_particles, dim3(pb), dim3(256), 0, st, xp, np, g, B.cell, B.count);
+  HFSI_LAUNCH_CHECK();
+  HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(B.scan_tmp, scan_bytes, B.count, B.start, (int)ncell, st));
+  hipLaunchKernelGGL(k_fill_bins, dim3(pb), dim3(256), 0, st, np, B.cell, B.start, B.cursor, B.sorted);
+  HFSI_LAUNCH_CHECK();
+  return HFSI_OK;
+}
+
+}  // namespace
+
+extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const double* marker_force,
+                                          int32_t marker_count, const float* particle_position,
+                                          float* external_force, int32_t particle_count, float support_radius,
+                                          hfsi_marker_scatter_report* report, void* stream) {
+  if (marker_count < 0 || particle_count <= 0 || !particle_position || !external_force ||
+      (marker_count > 0 && (!marker_position || !marker_force)) || !(support_radius > 0.0f) ||
+      !std::isfinite(support_radius)) {
+    set_error("hfsi_scatter_marker_forces: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hfsi_marker_scatter_report rep{};
+  if (marker_count == 0) {
+    if (report) *report = rep;
+    return HFSI_OK;
+  }
+  ParticleBins B;
+  const int rc = bin_particles(particle_position, particle_count, support_radius, st, B);
+  if (rc != HFSI_OK) {
+    free_bins(B, st);
+    return rc;
+  }
   const uint32_t mb = (uint32_t)((marker_count + kScatterWaves - 1) / kScatterWaves);
-  ScatterTotals* parts = nullptr;
+  ScatterTotals *parts = nullptr, *tot = nullptr;
   HFSI_CHECK(hipMallocAsync((void**)&tot, sizeof(ScatterTotals), st));
   HFSI_CHECK(hipMallocAsync((void**)&parts, mb * sizeof(ScatterTotals), st));
-  HFSI_CHECK(hipMallocAsync(&scan_tmp, std::max<size_t>(scan_bytes, 1), st));
-  HFSI_CHECK(hipMemsetAsync(count, 0, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMemsetAsync(cursor, 0, ncell * sizeof(uint32_t), st));
-  const int pb = std::max(1, std::min(4096, (particle_count + 255) / 256));
-  hipLaunchKernelGGL(k_bin_particles, dim3(pb), dim3(256), 0, st, particle_position, particle_count, g, cell, count);
-  HFSI_LAUNCH_CHECK();
-  HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, count, start, (int)ncell, st));
-  hipLaunchKernelGGL(k_fill_bins, dim3(pb), dim3(256), 0, st, particle_count, cell, start, cursor, sorted);
-  HFSI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter_markers, dim3(mb), dim3(64 * kScatterWaves), 0, st, marker_position, marker_force,
-                     marker_count, particle_position, external_force, g, start, count, sorted, support_radius, parts);
+                     marker_count, particle_position, external_force, B.g, B.start, B.count, B.sorted,
+                     support_radius, parts);
   HFSI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter_finish, dim3(1), dim3(256), 0, st, parts, mb, tot);
   HFSI_LAUNCH_CHECK();
   ScatterTotals ht{};
   HFSI_CHECK(hipMemcpyAsync(&ht, tot, sizeof(ht), hipMemcpyDeviceToHost, st));
-  for (void* p : {(void*)partial, (void*)count, (void*)start, (void*)cursor, (void*)cell, (void*)sorted, (void*)tot,
-                  (void*)parts, scan_tmp})
-    HFSI_CHECK(hipFreeAsync(p, st));
+  HFSI_CHECK(hipFreeAsync(parts, st));
+  HFSI_CHECK(hipFreeAsync(tot, st));
+  HFSI_TRY_BINS(free_bins(B, st));
   HFSI_CHECK(hipStreamSynchronize(st));
   rep.active_marker_count = (int32_t)ht.active_markers;
   rep.invalid_marker_count = (int32_t)ht.invalid_markers;
@@ -391,6 +577,52 @@ extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const do
     rep.total_marker_force[d] = ht.marker_force[d];
     rep.total_external_force[d] = ht.external_force[d];
   }
+  if (report) *report = rep;
+  return HFSI_OK;
+}
+
+extern "C" int hfsi_marker_feedback(float* marker_position, float* marker_velocity, int32_t marker_count,
+                                    const float* particle_position, const float* particle_velocity,
+                                    int32_t particle_count, float support_radius, float dt,
+                                    hfsi_marker_feedback_report* report, void* stream) {
+  if (marker_count < 0 || particle_count <= 0 || !particle_position || !particle_velocity ||
+      (marker_count > 0 && (!marker_position || !marker_velocity)) || !(support_radius > 0.0f) ||
+      !std::isfinite(support_radius) || !(dt > 0.0f) || !std::isfinite(dt)) {
+    set_error("hfsi_marker_feedback: invalid arguments");
+    return HFSI_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hfsi_marker_feedback_report rep{};
+  if (marker_count == 0) {
+    if (report) *report = rep;
+    return HFSI_OK;
+  }
+  ParticleBins B;
+  const int rc = bin_particles(particle_position, particle_count, support_radius, st, B);
+  if (rc != HFSI_OK) {
+    free_bins(B, st);
+    return rc;
+  }
+  const uint32_t mb = (uint32_t)((marker_count + kScatterWaves - 1) / kScatterWaves);
+  FeedbackTotals *parts = nullptr, *tot = nullptr;
+  HFSI_CHECK(hipMallocAsync((void**)&tot, sizeof(FeedbackTotals), st));
+  HFSI_CHECK(hipMallocAsync((void**)&parts, mb * sizeof(FeedbackTotals), st));
+  hipLaunchKernelGGL(k_feedback_markers, dim3(mb), dim3(64 * kScatterWaves), 0, st, marker_position, marker_velocity,
+                     marker_count, particle_position, particle_velocity, B.g, B.start, B.count, B.sorted,
+                     support_radius, dt, parts);
+  HFSI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_feedback_finish, dim3(1), dim3(256), 0, st, parts, mb, tot);
+  HFSI_LAUNCH_CHECK();
+  FeedbackTotals ht{};
+  HFSI_CHECK(hipMemcpyAsync(&ht, tot, sizeof(ht), hipMemcpyDeviceToHost, st));
+  HFSI_CHECK(hipFreeAsync(parts, st));
+  HFSI_CHECK(hipFreeAsync(tot, st));
+  HFSI_TRY_BINS(free_bins(B, st));
+  HFSI_CHECK(hipStreamSynchronize(st));
+  rep.updated_marker_count = (int32_t)ht.updated;
+  rep.invalid_marker_count = (int32_t)ht.invalid;
+  rep.max_marker_displacement = (double)ht.max_disp;
+  rep.max_marker_speed = (double)ht.max_speed;
   if (report) *report = rep;
   return HFSI_OK;
 }

@@ -75,3 +75,52 @@ def scatter_marker_forces_to_mpm_particles(marker_position_m, marker_force_n, ex
         active_marker_count=int(rep.active_marker_count), invalid_marker_count=int(rep.invalid_marker_count),
         active_particle_count=int(rep.active_particle_count), total_marker_force_n=total_marker,
         total_mpm_external_force_n=total_external, action_reaction_residual_n=residual)
+
+
+@dataclass(frozen=True)
+class HibmMpmSurfaceUpdateReport:
+    """hibm_mpm.py:115-121 (the geometry counters belong to the surface-particle variant)."""
+
+    updated_marker_count: int
+    invalid_marker_count: int
+    max_marker_displacement_m: float
+    max_marker_speed_mps: float
+    geometry_updated_marker_count: int = 0
+    geometry_invalid_marker_count: int = 0
+
+
+def update_surface_feedback_from_mpm_particles(marker_position_m, marker_velocity_mps, particle_position_m,
+                                               particle_velocity_mps, *, particle_count: int,
+                                               support_radius_m: float, dt_s: float,
+                                               marker_count: int | None = None) -> HibmMpmSurfaceUpdateReport:
+    """Move every marker with the tent-weighted velocity of the particles in its support
+    (hibm_mpm.py:5182-5265).  ``marker_position_m`` ([M, 3] float32) is updated in place and
+    ``marker_velocity_mps`` ([M, 3] float32) receives the new velocities; markers without
+    particles in reach are left unchanged.  Field objects or CUDA tensors."""
+    particles = int(particle_count)
+    if particles <= 0:
+        raise ValueError("particle_count must be positive")
+    support_radius = float(support_radius_m)
+    if not math.isfinite(support_radius) or support_radius <= 0.0:
+        raise ValueError("support_radius_m must be a finite positive number")
+    feedback_dt = float(dt_s)
+    if not math.isfinite(feedback_dt) or feedback_dt <= 0.0:
+        raise ValueError("dt_s must be a finite positive number")
+    xm, vm = _tensor(marker_position_m), _tensor(marker_velocity_mps)
+    xp, vp = _tensor(particle_position_m), _tensor(particle_velocity_mps)
+    m = int(xm.shape[0]) if marker_count is None else int(marker_count)
+    if any(t.dtype != torch.float32 for t in (xm, vm, xp, vp)):
+        raise TypeError("marker and particle positions and velocities are float32")
+    if m > xm.shape[0] or m > vm.shape[0] or particles > xp.shape[0] or particles > vp.shape[0]:
+        raise ValueError("marker_count / particle_count exceed the arrays")
+    if not all(t.is_contiguous() for t in (xm, vm, xp, vp)):
+        raise ValueError("marker and particle arrays must be contiguous")
+    lib = _lib.lib()
+    rep = _lib.MarkerFeedbackReport()
+    _lib.check(lib.hfsi_marker_feedback(xm.data_ptr(), vm.data_ptr(), m, xp.data_ptr(), vp.data_ptr(), particles,
+                                        ctypes.c_float(support_radius), ctypes.c_float(feedback_dt),
+                                        ctypes.byref(rep), _lib.stream_handle()), "hfsi_marker_feedback")
+    return HibmMpmSurfaceUpdateReport(updated_marker_count=int(rep.updated_marker_count),
+                                      invalid_marker_count=int(rep.invalid_marker_count),
+                                      max_marker_displacement_m=float(rep.max_marker_displacement),
+                                      max_marker_speed_mps=float(rep.max_marker_speed))

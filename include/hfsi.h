@@ -431,6 +431,24 @@ int hfsi_scatter_marker_forces(const float* marker_position, const double* marke
                                const float* particle_position, float* external_force, int32_t particle_count,
                                float support_radius, hfsi_marker_scatter_report* report, void* stream);
 
+typedef struct hfsi_marker_feedback_report {
+  int32_t updated_marker_count;   /* markers with a weight sum above 1e-12 */
+  int32_t invalid_marker_count;   /* the others (left unchanged)           */
+  double max_marker_displacement; /* max |x_new - x_old| (float32 norm)     */
+  double max_marker_speed;        /* max |v_new|                           */
+} hfsi_marker_feedback_report;
+
+/* Marker surface feedback: each marker with particles inside its support takes the
+ * tent-weighted average velocity of those particles (float32) and moves by dt times it;
+ * the others are left unchanged.  Replaces
+ * HibmMpmSurfaceMarkers._update_surface_feedback_from_mpm_particles_kernel
+ * (simulation_core/hibm_mpm.py:5182-5224), binned like hfsi_scatter_marker_forces.
+ * marker_position [M][3] float32 (in/out), marker_velocity [M][3] float32 (out),
+ * particle_position / particle_velocity [P][3] float32 (device).  Synchronises. */
+int hfsi_marker_feedback(float* marker_position, float* marker_velocity, int32_t marker_count,
+                         const float* particle_position, const float* particle_velocity, int32_t particle_count,
+                         float support_radius, float dt, hfsi_marker_feedback_report* report, void* stream);
+
 /* Live kernel timing (no reference counterpart; measurement support):
  * while enabled, HIP events are recorded on the launch stream around every
  * launch of the selected level-0 kernel and their elapsed times summed.

@@ -94,3 +94,36 @@ def scatter_scalar_loop(marker_position, marker_force, particle_position, extern
         else:
             counts[1] += 1
     return ext, counts
+
+
+def surface_feedback(marker_position, particle_position, particle_velocity, support_radius, dt):
+    """``_update_surface_feedback_from_mpm_particles_kernel`` (hibm_mpm.py:5182-5224):
+    returns (new marker positions, new marker velocities (NaN where not updated), report)."""
+    xm = np.array(marker_position, F32, copy=True)
+    xp = np.asarray(particle_position, F32)
+    vp = np.asarray(particle_velocity, F32)
+    vm = np.full(xm.shape, np.nan, F32)
+    dt32 = F32(dt)
+    rep = {"updated_marker_count": 0, "invalid_marker_count": 0, "max_marker_displacement_m": 0.0,
+           "max_marker_speed_mps": 0.0}
+    finite_p = np.isfinite(xp).all(axis=1)
+    for m in range(xm.shape[0]):
+        old = xm[m].copy()
+        w = np.zeros(xp.shape[0], F32)
+        if np.isfinite(old).all():
+            w[finite_p] = tent_weight(old, xp[finite_p], support_radius)
+        idx = np.nonzero(w > 0.0)[0]
+        wsum = np.add.accumulate(w[idx], dtype=F32)[-1] if idx.size else F32(0.0)
+        if wsum > F32(1.0e-12):
+            vsum = np.zeros(3, F32)
+            for p in idx:  # particle order, float32
+                vsum = (vsum + w[p] * vp[p]).astype(F32)
+            nv = (vsum / wsum).astype(F32)
+            nx = (old + dt32 * nv).astype(F32)
+            xm[m], vm[m] = nx, nv
+            rep["updated_marker_count"] += 1
+            rep["max_marker_displacement_m"] = max(rep["max_marker_displacement_m"], float(np.linalg.norm(nx - old)))
+            rep["max_marker_speed_mps"] = max(rep["max_marker_speed_mps"], float(np.linalg.norm(nv)))
+        else:
+            rep["invalid_marker_count"] += 1
+    return xm, vm, rep

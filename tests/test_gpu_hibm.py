@@ -144,3 +144,46 @@ def test_scatter_conserves_force_at_million_particles():
     _, ref = H.scatter_marker_forces(xm[sample], fm[sample], xp, ext0, h)
     _, rep_s = _run(xm[sample], fm[sample], xp, ext0, h)
     assert rep_s.active_particle_count == ref["active_particle_count"]
+
+
+# ---------------------------------------------------------------------------
+# surface feedback (hibm_mpm.py:5182-5265)
+# ---------------------------------------------------------------------------
+
+def _feedback(xm, xp, vp, h, dt, vm0=None):
+    from easyfsi_amd import hibm
+
+    txm = _t(xm, torch.float32)
+    tvm = _t(np.zeros_like(xm) if vm0 is None else vm0, torch.float32)
+    rep = hibm.update_surface_feedback_from_mpm_particles(txm, tvm, _t(xp, torch.float32), _t(vp, torch.float32),
+                                                          particle_count=len(xp), support_radius_m=h, dt_s=dt)
+    torch.cuda.synchronize()
+    return txm.cpu().numpy(), tvm.cpu().numpy(), rep
+
+
+def test_reference_surface_feedback_updates_marker():  # test_hibm.py:736-778
+    xm, vm, rep = _feedback(np.array([[0.5, 0.5, 0.5]], np.float32), np.array([[0.6, 0.45, 0.7]], np.float32),
+                            np.array([[1.0, -2.0, 3.0]], np.float32), 0.5, 0.1)
+    assert rep.updated_marker_count == 1 and rep.invalid_marker_count == 0
+    np.testing.assert_allclose(xm[0], (0.6, 0.3, 0.8), rtol=0, atol=1e-6)
+    np.testing.assert_allclose(vm[0], (1.0, -2.0, 3.0), rtol=0, atol=1e-6)
+
+
+def test_surface_feedback_matches_oracle_on_a_particle_ball():
+    _say("hibm feedback parity: ball")
+    rng, xp, xm, _ = _cloud(5, 34, 300)
+    xm[7] = (4.0, 4.0, 4.0)  # nothing in reach: invalid, unchanged
+    vp = rng.normal(size=xp.shape).astype(np.float32)
+    h = 2.2 / 34
+    vm0 = np.full(xm.shape, 9.0, np.float32)
+    got_x, got_v, rep = _feedback(xm, xp, vp, h, 0.02, vm0)
+    want_x, want_v, ref = H.surface_feedback(xm, xp, vp, h, 0.02)
+    assert rep.updated_marker_count == ref["updated_marker_count"] == 299
+    assert rep.invalid_marker_count == ref["invalid_marker_count"] == 1
+    upd = np.isfinite(want_v).all(axis=1)
+    # float32 weighted sums reduced in another order: a few ulps of the averaged velocity
+    np.testing.assert_allclose(got_v[upd], want_v[upd], rtol=0, atol=2e-6 * np.abs(vp).max())
+    np.testing.assert_allclose(got_x, want_x, rtol=0, atol=1e-6)
+    assert np.array_equal(got_v[~upd], vm0[~upd])
+    assert rep.max_marker_speed_mps == pytest.approx(ref["max_marker_speed_mps"], rel=1e-5)
+    assert rep.max_marker_displacement_m == pytest.approx(ref["max_marker_displacement_m"], rel=1e-5)

@@ -62,3 +62,24 @@ def test_conservation_cutoff_and_scalar_loop():
     loop_ext, counts = H.scatter_scalar_loop(xm, fm, xp, ext0, h)
     assert counts == [rep["active_marker_count"], rep["invalid_marker_count"], rep["active_particle_count"]]
     np.testing.assert_allclose(ext, loop_ext, rtol=0, atol=2e-6 * np.abs(ext).max())
+
+
+def test_surface_feedback_moves_marker_with_particle_velocity():  # test_hibm.py:736-778
+    xm, vm, rep = H.surface_feedback([[0.5, 0.5, 0.5]], [[0.6, 0.45, 0.7]], [[1.0, -2.0, 3.0]], 0.5, 0.1)
+    assert rep["updated_marker_count"] == 1 and rep["invalid_marker_count"] == 0
+    np.testing.assert_allclose(xm[0], (0.6, 0.3, 0.8), atol=1e-6)
+    np.testing.assert_allclose(vm[0], (1.0, -2.0, 3.0), atol=1e-6)
+
+
+def test_surface_feedback_reproduces_uniform_and_linear_velocity_fields():
+    rng = np.random.default_rng(6)
+    xp = rng.uniform(0.0, 1.0, (4000, 3)).astype(np.float32)
+    xm = rng.uniform(0.2, 0.8, (30, 3)).astype(np.float32)
+    u = np.array([0.3, -0.1, 0.25], np.float32)
+    xm1, vm1, rep = H.surface_feedback(xm, xp, np.broadcast_to(u, xp.shape), 0.12, 0.01)
+    assert rep["updated_marker_count"] == 30
+    np.testing.assert_allclose(vm1, np.broadcast_to(u, vm1.shape), rtol=2e-6)  # normalised weights average a constant
+    np.testing.assert_allclose(xm1, xm + np.float32(0.01) * u, atol=1e-6)
+    # a far marker is invalid and stays put
+    xm2, _, rep2 = H.surface_feedback(np.array([[5.0, 5.0, 5.0]], np.float32), xp, np.zeros_like(xp), 0.12, 0.01)
+    assert rep2["invalid_marker_count"] == 1 and np.array_equal(xm2, [[5.0, 5.0, 5.0]])
