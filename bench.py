@@ -30,8 +30,8 @@ sequence (moving sphere, turning velocity) solved from p = 0, from the
 previous step's pressure and from the learned guess; the neo-Hookean MPM step
 (1M particles); the velocity predictor (Euler and RK2 semi-Lagrangian advection
 plus FV diffusion) on the same grid; the sharp-interface coupling pass and the
-operator with and without couplings; the HIBM marker -> particle force scatter
-(1M particles, 24576 markers); the CPU baseline (C + OpenMP restatement of the plain
+operator with and without couplings; the HIBM marker -> particle force scatter and
+the marker surface feedback (1M particles, 24576 markers); the CPU baseline (C + OpenMP restatement of the plain
 algorithm on the host cores).
 
 Prints one JSON line.  --gpus above 1 is refused: this build measures one GPU.
@@ -628,6 +628,21 @@ def run_marker_scatter(steps: int, cpu: bool, lattice: int = 100, per_face: int 
     out = {"particles": len(xp), "markers": len(xm), "support_radius_m": h, "ms_per_scatter": round(1e3 * dt, 4),
            "pairs_per_scatter": rep.active_particle_count,
            "action_reaction_residual_n": rep.action_reaction_residual_n}
+    # surface feedback of the same markers from the particles (random particle velocities)
+    tvp = torch.from_numpy(np.random.default_rng(4).normal(size=xp.shape).astype(np.float32)).cuda()
+    tvm = torch.zeros_like(txm)
+    txm0 = txm.clone()
+    for _ in range(2):
+        txm.copy_(txm0)
+        hibm.update_surface_feedback_from_mpm_particles(txm, tvm, txp, tvp, particle_count=len(xp),
+                                                        support_radius_m=h, dt_s=1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fb = hibm.update_surface_feedback_from_mpm_particles(txm, tvm, txp, tvp, particle_count=len(xp),
+                                                             support_radius_m=h, dt_s=1e-4)
+    out["ms_per_feedback"] = round(1e3 * (time.perf_counter() - t0) / steps, 4)
+    out["feedback_updated_markers"] = fb.updated_marker_count
     if cpu:
         from oracle import hibm as H
 
@@ -760,7 +775,8 @@ def main(argv=None) -> int:
     if args.scatter_steps > 0:
         print(f"[bench] HIBM marker force scatter: {args.scatter_steps} calls", file=sys.stderr, flush=True)
         scatter = run_marker_scatter(args.scatter_steps, not args.no_cpu)
-        print(f"[bench] scatter: {scatter['ms_per_scatter']:.3f} ms per call", file=sys.stderr, flush=True)
+        print(f"[bench] scatter: {scatter['ms_per_scatter']:.3f} ms, "
+              f"feedback {scatter['ms_per_feedback']:.3f} ms per call", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (C oracle)", file=sys.stderr, flush=True)
