@@ -80,24 +80,47 @@ __global__ void __launch_bounds__(256) k_particle_bounds(const float* x, int32_t
   if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// Bin of every particle (-1: not finite) and its rank within the bin.  Consecutive lanes of
+// a wavefront with equal bins (particles stored in lattice order) share one atomic: the first
+// lane of the run adds the run's length and each lane takes its offset from that lane's result.
 __global__ void __launch_bounds__(256) k_bin_particles(const float* x, int32_t n, BinGrid g, int32_t* cell,
-                                                       uint32_t* count) {
-  for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    const float a = x[3 * (size_t)p], b = x[3 * (size_t)p + 1], c = x[3 * (size_t)p + 2];
+                                                       uint32_t* rank, uint32_t* count) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+    const int64_t p = base + lane;
     int32_t id = -1;
-    if (finite3(a, b, c)) {
-      id = (bin_axis(a, g.ox, g.h, g.nx) * g.ny + bin_axis(b, g.oy, g.h, g.ny)) * g.nz + bin_axis(c, g.oz, g.h, g.nz);
-      atomicAdd(&count[id], 1u);
+    if (p < n) {
+      const float a = x[3 * (size_t)p], b = x[3 * (size_t)p + 1], c = x[3 * (size_t)p + 2];
+      if (finite3(a, b, c))
+        id = (bin_axis(a, g.ox, g.h, g.nx) * g.ny + bin_axis(b, g.oy, g.h, g.ny)) * g.nz + bin_axis(c, g.oz, g.h, g.nz);
     }
-    cell[p] = id;
+    const int32_t prev = __shfl_up(id, 1, 64);
+    const bool head = lane == 0 || prev != id;
+    const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane (all 64 for lane 63)
+    const int first = 63 - __builtin_clzll(heads & upto);  // lane 0 is always a head
+    const uint64_t above = heads & ~upto;
+    const int next = above ? __builtin_ctzll(above) : 64;
+    uint32_t r = 0;
+    if (head && id >= 0) r = atomicAdd(&count[id], (uint32_t)(next - lane));
+    r = (uint32_t)__shfl((int)r, first, 64) + (uint32_t)(lane - first);
+    if (p < n) {
+      cell[p] = id;
+      if (id >= 0) rank[p] = r;
+    }
   }
 }
 
-__global__ void __launch_bounds__(256) k_fill_bins(int32_t n, const int32_t* cell, const uint32_t* start,
-                                                   uint32_t* cursor, int32_t* sorted) {
+// sorted[start[bin] + rank] = particle: no atomics; the rank check keeps every write inside
+// the bin's range.
+__global__ void __launch_bounds__(256) k_fill_bins(int32_t n, const int32_t* cell, const uint32_t* rank,
+                                                   const uint32_t* start, const uint32_t* count, int32_t* sorted) {
   for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     const int32_t id = cell[p];
-    if (id >= 0) sorted[start[id] + atomicAdd(&cursor[id], 1u)] = p;
+    if (id < 0) continue;
+    const uint32_t r = rank[p];
+    if (r < count[id]) sorted[start[id] + r] = p;
   }
 }
 
@@ -459,14 +482,14 @@ namespace {
 // Particles binned by cells of edge >= the support radius (device arrays, stream-ordered).
 struct ParticleBins {
   BinGrid g{};
-  uint32_t *count = nullptr, *start = nullptr, *cursor = nullptr;
+  uint32_t *count = nullptr, *start = nullptr, *rank = nullptr;
   int32_t *cell = nullptr, *sorted = nullptr;
   float* partial = nullptr;
   void* scan_tmp = nullptr;
 };
 
 int free_bins(ParticleBins& B, hipStream_t st) {
-  for (void* p : {(void*)B.count, (void*)B.start, (void*)B.cursor, (void*)B.cell, (void*)B.sorted, (void*)B.partial,
+  for (void* p : {(void*)B.count, (void*)B.start, (void*)B.rank, (void*)B.cell, (void*)B.sorted, (void*)B.partial,
                   B.scan_tmp})
     if (p) HFSI_CHECK(hipFreeAsync(p, st));
   B = ParticleBins{};
@@ -515,17 +538,17 @@ int bin_particles(const float* xp, int32_t np, float support, hipStream_t st, Pa
   HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, B.count, B.start, (int)ncell, st));
   HFSI_CHECK(hipMallocAsync((void**)&B.count, ncell * sizeof(uint32_t), st));
   HFSI_CHECK(hipMallocAsync((void**)&B.start, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&B.cursor, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(hipMallocAsync((void**)&B.rank, (size_t)np * sizeof(uint32_t), st));
   HFSI_CHECK(hipMallocAsync((void**)&B.cell, (size_t)np * sizeof(int32_t), st));
   HFSI_CHECK(hipMallocAsync((void**)&B.sorted, (size_t)np * sizeof(int32_t), st));
   HFSI_CHECK(hipMallocAsync(&B.scan_tmp, std::max<size_t>(scan_bytes, 1), st));
   HFSI_CHECK(hipMemsetAsync(B.count, 0, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMemsetAsync(B.cursor, 0, ncell * sizeof(uint32_t), st));
   const int pb = std::max(1, std::min(4096, (np + 255) / 256));
-  hipLaunchKernelGGL(k_bin_particles, dim3(pb), dim3(256), 0, st, xp, np, g, B.cell, B.count);
+  hipLaunchKernelGGL(k_bin_particles, dim3(pb), dim3(256), 0, st, xp, np, g, B.cell, B.rank, B.count);
   HFSI_LAUNCH_CHECK();
   HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(B.scan_tmp, scan_bytes, B.count, B.start, (int)ncell, st));
-  hipLaunchKernelGGL(k_fill_bins, dim3(pb), dim3(256), 0, st, np, B.cell, B.start, B.cursor, B.sorted);
+  hipLaunchKernelGGL(k_fill_bins, dim3(pb), dim3(256), 0, st, np, B.cell, B.rank, B.start, B.count,
+                     B.sorted);
   HFSI_LAUNCH_CHECK();
   return HFSI_OK;
 }
