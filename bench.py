@@ -752,6 +752,9 @@ def main(argv=None) -> int:
         print(f"[bench] warm-start comparison: {args.sequence_steps} time-coherent projections", file=sys.stderr,
               flush=True)
         sequence = run_sequence(solver, w, args.sequence_steps, guess)
+        # the sequence leaves the sphere at its last position; the legs below use the centred one
+        solver.mark_sphere_obstacle((0.5, 0.5, 0.5), 0.06)
+        solver._set_masks()
     mpm = None
     if args.mpm_steps > 0:
         print(f"[bench] neo-Hookean MPM: 1M particles, {args.mpm_steps} steps", file=sys.stderr, flush=True)
