@@ -20,7 +20,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -479,6 +481,44 @@ using namespace hfsi;
 
 namespace {
 
+// Stream-ordered allocations of these calls come from one pool per device that keeps its
+// memory between calls (release threshold unlimited): the default pool hands its memory
+// back at every synchronisation, so each call would map its buffers again.  A pool holds
+// the largest working set of the calls on its device until the process ends.
+constexpr int kMaxPoolDevices = 64;
+
+hipError_t pool_malloc(void** p, size_t bytes, hipStream_t st) {
+  static std::mutex mu;
+  static hipMemPool_t pools[kMaxPoolDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxPoolDevices) return hipMallocAsync(p, bytes, st);
+  hipMemPool_t pool = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!pools[dev]) {
+      hipMemPoolProps props{};
+      props.allocType = hipMemAllocationTypePinned;
+      props.handleTypes = hipMemHandleTypeNone;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      hipMemPool_t created = nullptr;
+      e = hipMemPoolCreate(&created, &props);
+      if (e != hipSuccess) return e;
+      uint64_t keep = UINT64_MAX;
+      e = hipMemPoolSetAttribute(created, hipMemPoolAttrReleaseThreshold, &keep);
+      if (e != hipSuccess) {
+        (void)hipMemPoolDestroy(created);
+        return e;
+      }
+      pools[dev] = created;
+    }
+    pool = pools[dev];
+  }
+  return hipMallocFromPoolAsync(p, bytes, pool, st);
+}
+
 // Particles binned by cells of edge >= the support radius (device arrays, stream-ordered).
 struct ParticleBins {
   BinGrid g{};
@@ -500,7 +540,7 @@ int free_bins(ParticleBins& B, hipStream_t st) {
 // exclusive scan, fill).  Synchronises once (the bounds set the grid).
 int bin_particles(const float* xp, int32_t np, float support, hipStream_t st, ParticleBins& B) {
   const int nb = std::max(1, std::min(1024, (np + 255) / 256));
-  HFSI_CHECK(hipMallocAsync((void**)&B.partial, 6 * nb * sizeof(float), st));
+  HFSI_CHECK(pool_malloc((void**)&B.partial, 6 * nb * sizeof(float), st));
   hipLaunchKernelGGL(k_particle_bounds, dim3(nb), dim3(256), 0, st, xp, np, B.partial);
   HFSI_LAUNCH_CHECK();
   std::vector<float> hp(6 * nb);
@@ -536,12 +576,12 @@ int bin_particles(const float* xp, int32_t np, float support, hipStream_t st, Pa
   const uint32_t ncell = (uint32_t)(g.nx * g.ny * g.nz);
   size_t scan_bytes = 0;
   HFSI_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, B.count, B.start, (int)ncell, st));
-  HFSI_CHECK(hipMallocAsync((void**)&B.count, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&B.start, ncell * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&B.rank, (size_t)np * sizeof(uint32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&B.cell, (size_t)np * sizeof(int32_t), st));
-  HFSI_CHECK(hipMallocAsync((void**)&B.sorted, (size_t)np * sizeof(int32_t), st));
-  HFSI_CHECK(hipMallocAsync(&B.scan_tmp, std::max<size_t>(scan_bytes, 1), st));
+  HFSI_CHECK(pool_malloc((void**)&B.count, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(pool_malloc((void**)&B.start, ncell * sizeof(uint32_t), st));
+  HFSI_CHECK(pool_malloc((void**)&B.rank, (size_t)np * sizeof(uint32_t), st));
+  HFSI_CHECK(pool_malloc((void**)&B.cell, (size_t)np * sizeof(int32_t), st));
+  HFSI_CHECK(pool_malloc((void**)&B.sorted, (size_t)np * sizeof(int32_t), st));
+  HFSI_CHECK(pool_malloc(&B.scan_tmp, std::max<size_t>(scan_bytes, 1), st));
   HFSI_CHECK(hipMemsetAsync(B.count, 0, ncell * sizeof(uint32_t), st));
   const int pb = std::max(1, std::min(4096, (np + 255) / 256));
   hipLaunchKernelGGL(k_bin_particles, dim3(pb), dim3(256), 0, st, xp, np, g, B.cell, B.rank, B.count);
@@ -579,8 +619,8 @@ extern "C" int hfsi_scatter_marker_forces(const float* marker_position, const do
   }
   const uint32_t mb = (uint32_t)((marker_count + kScatterWaves - 1) / kScatterWaves);
   ScatterTotals *parts = nullptr, *tot = nullptr;
-  HFSI_CHECK(hipMallocAsync((void**)&tot, sizeof(ScatterTotals), st));
-  HFSI_CHECK(hipMallocAsync((void**)&parts, mb * sizeof(ScatterTotals), st));
+  HFSI_CHECK(pool_malloc((void**)&tot, sizeof(ScatterTotals), st));
+  HFSI_CHECK(pool_malloc((void**)&parts, mb * sizeof(ScatterTotals), st));
   hipLaunchKernelGGL(k_scatter_markers, dim3(mb), dim3(64 * kScatterWaves), 0, st, marker_position, marker_force,
                      marker_count, particle_position, external_force, B.g, B.start, B.count, B.sorted,
                      support_radius, parts);
@@ -628,8 +668,8 @@ extern "C" int hfsi_marker_feedback(float* marker_position, float* marker_veloci
   }
   const uint32_t mb = (uint32_t)((marker_count + kScatterWaves - 1) / kScatterWaves);
   FeedbackTotals *parts = nullptr, *tot = nullptr;
-  HFSI_CHECK(hipMallocAsync((void**)&tot, sizeof(FeedbackTotals), st));
-  HFSI_CHECK(hipMallocAsync((void**)&parts, mb * sizeof(FeedbackTotals), st));
+  HFSI_CHECK(pool_malloc((void**)&tot, sizeof(FeedbackTotals), st));
+  HFSI_CHECK(pool_malloc((void**)&parts, mb * sizeof(FeedbackTotals), st));
   hipLaunchKernelGGL(k_feedback_markers, dim3(mb), dim3(64 * kScatterWaves), 0, st, marker_position, marker_velocity,
                      marker_count, particle_position, particle_velocity, B.g, B.start, B.count, B.sorted,
                      support_radius, dt, parts);
