@@ -685,8 +685,8 @@ def main(argv=None) -> int:
                     help="velocity predictor launches timed beside, per advection scheme (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the solver and MPM paths run on a single device and have no "
+              "multi-GPU decomposition", file=sys.stderr)
         return 2
     if args.size % 4 or args.size < 16:
         print("--size must be a multiple of 4 and >= 16", file=sys.stderr)
